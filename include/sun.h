@@ -1,0 +1,167 @@
+/*
+ * sun.h -- C ABI of the MI355X SU(N) quenched ensemble-generation engine.
+ *
+ * Drop-in boundary for the generation path of nmrcardoso/sun (namespace CULQCD).
+ * Each entry point names the reference interface it replaces (file:line in the
+ * reference tree).  Differences in contract, on purpose:
+ *   - every call returns an int status (0 = ok, < 0 = error) and never exits;
+ *     sun_last_error() gives the message of the calling thread's last failure
+ *     (the reference prints and calls exit(), comm_mpi.h:116-120);
+ *   - geometry, beta and storage belong to a lattice handle, not to process
+ *     globals (PARAMS/DEVPARAMS, constants.h:53-229), so lattices of different
+ *     N, volume and beta coexist on one device;
+ *   - the RNG is counter-based: a heatbath sweep is a pure function of
+ *     (seed, sweep number), there is no RNG state array (random.h:42-101).
+ *
+ * Plain C types only.  A stream argument is a hipStream_t passed as void*; NULL
+ * selects the context's own stream.  Host link buffers use the canonical layout
+ *     double U[4][V][N][N][2]
+ * (direction, even/odd site index eo(x) = parity(x)*V/2 + lex(x)/2 with
+ * lex(x) = x0 + X0*(x1 + X1*(x2 + X2*x3)), row-major colour matrix, re/im).
+ */
+#ifndef SUN_H
+#define SUN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define SUN_OK 0
+#define SUN_ERR_INVALID (-1)     /* bad argument                                  */
+#define SUN_ERR_HIP (-2)         /* HIP runtime error                             */
+#define SUN_ERR_NOMEM (-3)       /* device or host allocation failed              */
+#define SUN_ERR_UNSUPPORTED (-4) /* combination not supported (e.g. cold 8-real)  */
+#define SUN_ERR_BUDGET (-5)      /* request exceeds the memory budget             */
+#define SUN_ERR_STATE (-6)       /* wrong object state (e.g. unknown job id)      */
+
+/* ---- enums mirroring include/modes.h ---------------------------------------- */
+#define SUN_STORE_18 18 /* full N*N complex ("SOA")          */
+#define SUN_STORE_12 12 /* SU(3) rows 0,1 ("SOA12")          */
+#define SUN_STORE_8 8   /* SU(3) 8 reals ("SOA8")            */
+#define SUN_PREC_F32 4
+#define SUN_PREC_F64 8
+#define SUN_SILENT 0    /* Verbosity, modes.h:30-35 */
+#define SUN_SUMMARIZE 1
+#define SUN_VERBOSE 2
+#define SUN_DEBUG_VERBOSE 3
+#define SUN_TUNE_NO 0   /* TuneMode, modes.h:38-41 */
+#define SUN_TUNE_YES 1
+#define SUN_LAYOUT_CANONICAL 0
+
+typedef struct sun_ctx sun_ctx;
+typedef struct sun_lattice sun_lattice;
+typedef struct sun_batch sun_batch;
+typedef struct sun_server sun_server;
+
+typedef struct {
+    int nc;          /* colours: 2, 3 or 4                                         */
+    int dims[4];     /* X, Y, Z, T; each even (constants.cu:170-174)               */
+    double beta;     /* Wilson coupling                                            */
+    int storage;     /* SUN_STORE_*; 12 and 8 only for nc == 3                     */
+    int precision;   /* SUN_PREC_F32 or SUN_PREC_F64                               */
+} sun_lattice_desc;
+
+/* ---- runtime ----------------------------------------------------------------- */
+/* initCULQCD(gpuid, verbosity, tune)            include/comm_mpi.h:43, comm_mpi.cpp:92-184 */
+int sun_init(int device, int verbosity, int tune, sun_ctx **out);
+/* EndCULQCD(status) without exit()               include/comm_mpi.h:45, comm_mpi.cpp:186-210 */
+int sun_finalize(sun_ctx *ctx);
+/* errorCULQCD message                            include/comm_mpi.h:116-120 */
+const char *sun_last_error(void);
+/* library version string */
+const char *sun_version(void);
+
+/* ---- lattices ------------------------------------------------------------------ */
+/* SETPARAMS(usetex, beta, nx, ny, nz, nt, verbose) + _gauge<Real>(type, Device, 4V, evenodd).Allocate
+ *                                                include/constants.h:128, constants.cu:149-270,
+ *                                                include/gaugearray.h:139-161, :603-648 */
+int sun_lattice_create(sun_ctx *ctx, const sun_lattice_desc *desc, sun_lattice **out);
+/* _gauge<Real>::Release                          include/gaugearray.h:657-688 */
+int sun_lattice_destroy(sun_lattice *lat);
+/* device bytes of the link field (gaugearray.h:447-464 Bytes()) */
+size_t sun_lattice_bytes(const sun_lattice_desc *desc);
+/* _gauge<Real>::Init() -> ColdStart; refuses 8-real storage as the reference does
+ *                                                gaugearray.cu:276-305, gaugearray.h:765-775 */
+int sun_cold_start(sun_lattice *lat, void *stream);
+/* RNG::Init(seed) + _gauge<Real>::Init(RNG) -> HotStart; links are Haar-random here
+ *                                                random.cu:93-136, gaugearray.cu:353-420 */
+int sun_hot_start(sun_lattice *lat, uint64_t seed, void *stream);
+
+/* ---- updates --------------------------------------------------------------------- */
+/* HeatBath<Real>(gauge, RNG).Run(stream): 8 launches, parity outer, mu inner.  The
+ * RNG stream of a link is keyed by (seed, sweep, parity, mu, site).
+ *                                                include/monte/monte.h:17-75, monte.cu:196-222 */
+int sun_heatbath(sun_lattice *lat, uint64_t seed, uint32_t sweep, void *stream);
+/* OverRelaxation<Real>(gauge).Run(stream)        include/monte/ovr.h:20, ovr.cu:182-208 */
+int sun_overrelax(sun_lattice *lat, void *stream);
+/* one (parity, mu) launch of each (HeatBath::apply / OverRelaxation::apply)
+ *                                                monte.cu:190-194, ovr.cu:176-180 */
+int sun_heatbath_sub(sun_lattice *lat, uint64_t seed, uint32_t sweep, int parity, int mu, void *stream);
+int sun_overrelax_sub(sun_lattice *lat, int parity, int mu, void *stream);
+/* Reunitarize<Real>(gauge).Run(stream)           include/reunitarize.h:22-60, reunitarize.cu:78-93 */
+int sun_reunitarize(sun_lattice *lat, void *stream);
+
+/* ---- observables --------------------------------------------------------------------- */
+/* PlaquetteCUB<Real>(gauge).Run() -> complex{spatial, temporal}, each / (3*N*V); blocking
+ *                                                include/meas/plaquette.h:120-176, plaquette_cub.cu:226-245 */
+int sun_plaquette(sun_lattice *lat, void *stream, double out[2]);
+/* OnePolyakovLoop<Real>(gauge).Run() -> complex, / (N*X*Y*Z); blocking
+ *                                                include/meas/polyakovloop.h:15-73, polyakovloop.cu:637-768 */
+int sun_polyakov(sun_lattice *lat, void *stream, double out[2]);
+/* asynchronous forms: write double[2] to DEVICE memory dev_out, no host sync */
+int sun_plaquette_async(sun_lattice *lat, void *stream, double *dev_out);
+int sun_polyakov_async(sun_lattice *lat, void *stream, double *dev_out);
+
+/* ---- host exchange (gaugearray.h:521-596 Get/Set, canonical FP64 layout) ---------------- */
+int sun_links_to_host(sun_lattice *lat, double *host, int layout);
+int sun_links_from_host(sun_lattice *lat, const double *host, int layout);
+
+/* ---- generation driver ------------------------------------------------------------------
+ * The loop the reference's API implies but does not ship (no HB/OR driver exists in
+ * test.cpp): for it in [0, n_iter): heatbath (sweep = first_sweep + it), n_or
+ * overrelaxation sweeps, reunitarize every reunit_every iterations (0 = never),
+ * plaquette and Polyakov loop every iteration.  obs (host, may be NULL) receives
+ * n_iter rows of {plaq_spatial, plaq_temporal, poly_re, poly_im}. */
+typedef struct {
+    uint64_t seed;
+    uint32_t first_sweep;
+    int n_iter;
+    int n_or;
+    int reunit_every;
+    int measure;     /* 0: no observables, 1: plaquette + Polyakov each iteration */
+    int use_graph;   /* capture one iteration in a hipGraph and replay it           */
+} sun_generate_desc;
+int sun_generate(sun_lattice *lat, const sun_generate_desc *desc, void *stream, double *obs);
+
+/* ---- batched lattices: one launch per (parity, mu) covers every lattice -------------------
+ * All lattices of a batch share N, storage and precision; volumes and beta differ. */
+int sun_batch_create(sun_ctx *ctx, int n, sun_lattice *const *lats, const uint64_t *seeds, sun_batch **out);
+int sun_batch_destroy(sun_batch *b);
+int sun_batch_heatbath(sun_batch *b, const uint32_t *sweeps, void *stream);
+int sun_batch_overrelax(sun_batch *b, void *stream);
+int sun_batch_reunitarize(sun_batch *b, void *stream);
+/* out: double[2*n] plaquette (spatial, temporal) per lattice, blocking */
+int sun_batch_plaquette(sun_batch *b, void *stream, double *out);
+/* n_iter x (HB + n_or OR [+ reunit]); hipGraph-captured when use_graph; per-lattice
+ * plaquette of every iteration into obs (host, n_iter*2*n doubles, may be NULL) */
+int sun_batch_generate(sun_batch *b, uint32_t first_sweep, int n_iter, int n_or, int reunit_every,
+                       int use_graph, void *stream, double *obs);
+
+/* ---- streams and events (timing helpers for callers without a HIP binding) ---------- */
+int sun_stream_create(void **stream);
+int sun_stream_destroy(void *stream);
+int sun_stream_synchronize(void *stream);
+int sun_event_create(void **ev);
+int sun_event_destroy(void *ev);
+int sun_event_record(void *ev, void *stream);
+int sun_event_elapsed_ms(void *start, void *stop, float *ms);
+int sun_device_name(sun_ctx *ctx, char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SUN_H */

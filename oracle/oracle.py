@@ -1,0 +1,290 @@
+"""ctypes binding of the FP64 CPU oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY.  The oracle is the checker the HIP engine is compared
+with; it is imported by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` and by nothing on the product path.
+
+Fields are numpy ``complex128`` arrays of shape ``(4, V, N, N)``: direction, then
+even/odd site index ``eo(x) = parity(x)*V/2 + lex(x)/2`` (include/index.h:133-141
+of the reference), then the row-major colour matrix.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+STORE_18, STORE_12, STORE_8 = 18, 12, 8
+TAG_HB, TAG_HOT, TAG_TEST = 0x10000000, 0x20000000, 0x30000000
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, OpenMP, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = ctypes.CDLL(_LIB_PATH)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    u64, u32, ci, cd = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_double
+    sig = {
+        "or_philox4x32_10": (None, [u32p, u32p, u32p]),
+        "or_u01": (cd, [u32, u32]),
+        "or_stream_uniforms": (None, [u64, u32, u32, u32, ci, dp]),
+        "or_generate_su2_milc": (ci, [cd, dp, ci, dp]),
+        "or_sample_a0": (None, [cd, u64, ci, dp]),
+        "or_eo_to_coords": (None, [ci, ci, ip, ip]),
+        "or_coords_to_eo": (ci, [ip, ip]),
+        "or_neighbor_eo": (ci, [ci, ci, ip, ci, ci, ci, ci]),
+        "or_cold_start": (None, [ci, ip, dp]),
+        "or_hot_start": (None, [ci, ip, u64, ci, dp]),
+        "or_encode12": (None, [dp, dp]),
+        "or_decode12": (None, [dp, dp]),
+        "or_encode8": (None, [dp, dp]),
+        "or_decode8": (None, [dp, dp]),
+        "or_project_storage": (None, [ci, ci, ctypes.c_long, dp]),
+        "or_staple": (None, [ci, ip, dp, ci, ci, ci, dp]),
+        "or_heatbath_sub": (ci, [ci, ip, cd, u64, u32, ci, dp, ci, ci]),
+        "or_overrelax_sub": (ci, [ci, ip, ci, dp, ci, ci]),
+        "or_heatbath_sweep": (ci, [ci, ip, cd, u64, u32, ci, dp]),
+        "or_overrelax_sweep": (ci, [ci, ip, ci, dp]),
+        "or_overrelax_link": (None, [ci, dp, dp]),
+        "or_heatbath_link": (None, [ci, cd, dp, dp, u64, u32, u32, u32]),
+        "or_reunit_link": (None, [ci, dp]),
+        "or_reunitarize": (None, [ci, ctypes.c_long, ci, dp]),
+        "or_plaquette": (None, [ci, ip, dp, dp]),
+        "or_polyakov": (None, [ci, ip, dp, dp]),
+        "or_matmul": (None, [ci, dp, dp, dp]),
+        "or_det": (None, [ci, dp, dp]),
+        "or_num_threads": (ci, []),
+        "or_set_num_threads": (None, [ci]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _dims(dims):
+    return (ctypes.c_int * 4)(*[int(d) for d in dims])
+
+
+def volume(dims) -> int:
+    return int(np.prod(dims))
+
+
+def empty_field(nc: int, dims) -> np.ndarray:
+    return np.zeros((4, volume(dims), nc, nc), dtype=np.complex128)
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise ValueError(f"oracle {what} rejected its arguments (code {rc})")
+
+
+# ---- RNG -----------------------------------------------------------------------
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().or_philox4x32_10(c, k, o)
+    return tuple(int(x) for x in o)
+
+
+def u01(lo: int, hi: int) -> float:
+    return lib().or_u01(lo, hi)
+
+
+def stream_uniforms(seed: int, c0: int, c2: int, c3: int, n: int) -> np.ndarray:
+    out = np.zeros(n)
+    lib().or_stream_uniforms(seed, c0, c2, c3, n, _dp(out))
+    return out
+
+
+def generate_su2(al: float, uniforms) -> tuple[np.ndarray, int]:
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    a = np.zeros(4)
+    used = lib().or_generate_su2_milc(al, _dp(u), len(u), _dp(a))
+    return a, used
+
+
+def sample_a0(al: float, seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n)
+    lib().or_sample_a0(al, seed, n, _dp(out))
+    return out
+
+
+# ---- geometry ------------------------------------------------------------------
+def eo_to_coords(idx: int, parity: int, dims):
+    x = (ctypes.c_int * 4)()
+    lib().or_eo_to_coords(idx, parity, _dims(dims), x)
+    return tuple(x)
+
+
+def coords_to_eo(x, dims) -> int:
+    return lib().or_coords_to_eo((ctypes.c_int * 4)(*x), _dims(dims))
+
+
+def neighbor_eo(idx, parity, dims, mu, lmu, nu=-1, lnu=0) -> int:
+    return lib().or_neighbor_eo(idx, parity, _dims(dims), mu, lmu, nu, lnu)
+
+
+# ---- fields ----------------------------------------------------------------------
+def cold_start(nc: int, dims) -> np.ndarray:
+    U = empty_field(nc, dims)
+    lib().or_cold_start(nc, _dims(dims), _dp(U))
+    return U
+
+
+def hot_start(nc: int, dims, seed: int, storage: int = STORE_18) -> np.ndarray:
+    U = empty_field(nc, dims)
+    lib().or_hot_start(nc, _dims(dims), seed, storage, _dp(U))
+    return U
+
+
+def project_storage(U: np.ndarray, storage: int) -> np.ndarray:
+    U = np.ascontiguousarray(U.copy())
+    nc = U.shape[-1]
+    lib().or_project_storage(nc, storage, U.size // (nc * nc), _dp(U))
+    return U
+
+
+def encode12(link):
+    link = np.ascontiguousarray(link, dtype=np.complex128)
+    out = np.zeros(12)
+    lib().or_encode12(_dp(link), _dp(out))
+    return out
+
+
+def decode12(p):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.zeros((3, 3), dtype=np.complex128)
+    lib().or_decode12(_dp(p), _dp(out))
+    return out
+
+
+def encode8(link):
+    link = np.ascontiguousarray(link, dtype=np.complex128)
+    out = np.zeros(8)
+    lib().or_encode8(_dp(link), _dp(out))
+    return out
+
+
+def decode8(p):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.zeros((3, 3), dtype=np.complex128)
+    lib().or_decode8(_dp(p), _dp(out))
+    return out
+
+
+def staple(U: np.ndarray, dims, parity: int, mu: int, idx: int) -> np.ndarray:
+    nc = U.shape[-1]
+    S = np.zeros((nc, nc), dtype=np.complex128)
+    lib().or_staple(nc, _dims(dims), _dp(U), parity, mu, idx, _dp(S))
+    return S
+
+
+def heatbath_sub(U, dims, beta, seed, sweep, parity, mu, storage=STORE_18):
+    nc = U.shape[-1]
+    _check(lib().or_heatbath_sub(nc, _dims(dims), beta, seed, sweep, storage, _dp(U), parity, mu), "heatbath")
+    return U
+
+
+def overrelax_sub(U, dims, parity, mu, storage=STORE_18):
+    nc = U.shape[-1]
+    _check(lib().or_overrelax_sub(nc, _dims(dims), storage, _dp(U), parity, mu), "overrelax")
+    return U
+
+
+def heatbath_sweep(U, dims, beta, seed, sweep, storage=STORE_18):
+    nc = U.shape[-1]
+    _check(lib().or_heatbath_sweep(nc, _dims(dims), beta, seed, sweep, storage, _dp(U)), "heatbath")
+    return U
+
+
+def overrelax_sweep(U, dims, storage=STORE_18):
+    nc = U.shape[-1]
+    _check(lib().or_overrelax_sweep(nc, _dims(dims), storage, _dp(U)), "overrelax")
+    return U
+
+
+def overrelax_link(U, F):
+    U = np.ascontiguousarray(U, dtype=np.complex128).copy()
+    F = np.ascontiguousarray(F, dtype=np.complex128)
+    lib().or_overrelax_link(U.shape[-1], _dp(U), _dp(F))
+    return U
+
+
+def heatbath_link(U, F, beta, seed, c0=0, c2=0, c3=TAG_HB):
+    U = np.ascontiguousarray(U, dtype=np.complex128).copy()
+    F = np.ascontiguousarray(F, dtype=np.complex128)
+    lib().or_heatbath_link(U.shape[-1], beta, _dp(U), _dp(F), seed, c0, c2, c3)
+    return U
+
+
+def reunit_link(link):
+    link = np.ascontiguousarray(link, dtype=np.complex128).copy()
+    lib().or_reunit_link(link.shape[-1], _dp(link))
+    return link
+
+
+def reunitarize(U, storage=STORE_18):
+    nc = U.shape[-1]
+    lib().or_reunitarize(nc, U.size // (nc * nc), storage, _dp(U))
+    return U
+
+
+def plaquette(U, dims):
+    out = np.zeros(2)
+    lib().or_plaquette(U.shape[-1], _dims(dims), _dp(U), _dp(out))
+    return out
+
+
+def polyakov(U, dims):
+    out = np.zeros(2)
+    lib().or_polyakov(U.shape[-1], _dims(dims), _dp(U), _dp(out))
+    return complex(out[0], out[1])
+
+
+def matmul(A, B):
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    B = np.ascontiguousarray(B, dtype=np.complex128)
+    C = np.zeros_like(A)
+    lib().or_matmul(A.shape[-1], _dp(A), _dp(B), _dp(C))
+    return C
+
+
+def det(A) -> complex:
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    out = np.zeros(2)
+    lib().or_det(A.shape[-1], _dp(A), _dp(out))
+    return complex(out[0], out[1])
+
+
+def set_threads(n: int):
+    lib().or_set_num_threads(n)
+
+
+def threads() -> int:
+    return lib().or_num_threads()

@@ -1,0 +1,898 @@
+/*
+ * sun_oracle.c -- FP64 CPU restatement of nmrcardoso/sun's quenched SU(N)
+ * generation path.  TEST INFRASTRUCTURE ONLY (the checker): it is never part of the
+ * product path.  See sun_oracle.h for the layout and the list of functions.
+ *
+ * Arithmetic follows the reference's operation order (complex.h:150-152,
+ * matrixsun.h:298-347) and is built with -ffp-contract=off, so every multiply-add
+ * is rounded the way the reference's source writes it.  Transcendentals come from
+ * the host libm (the reference uses CUDA libm; that boundary is unpinned and
+ * tolerances of the parity tests absorb it).
+ */
+#include "sun_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NMAX 4
+#define PII 6.2831853071795864769252867665590   /* 2*pi, include/constants.h:27 */
+
+#define RNG_TAG_HB   0x10000000u
+#define RNG_TAG_HOT  0x20000000u
+#define RNG_TAG_TEST 0x30000000u
+
+/* ------------------------------------------------------------------------- */
+/* complex arithmetic in the reference's operation order (complex.h:82-230)    */
+/* ------------------------------------------------------------------------- */
+typedef struct { double re, im; } cx;
+
+static inline cx cmk(double r, double i) { cx z; z.re = r; z.im = i; return z; }
+static inline cx cadd(cx a, cx b) { return cmk(a.re + b.re, a.im + b.im); }
+static inline cx csub(cx a, cx b) { return cmk(a.re - b.re, a.im - b.im); }
+/* complex.h:150-152: (x*bx - y*by, y*bx + x*by) */
+static inline cx cmul(cx a, cx b) { return cmk(a.re * b.re - a.im * b.im, a.im * b.re + a.re * b.im); }
+static inline cx cconj(cx a) { return cmk(a.re, -a.im); }
+static inline double cabs2(cx a) { return a.re * a.re + a.im * a.im; }
+static inline cx cscale(cx a, double s) { return cmk(a.re * s, a.im * s); }
+
+typedef struct { cx e[NMAX][NMAX]; } mat;
+
+static inline void mload(int nc, const double *p, mat *m) {
+    if (nc < NMAX) memset(m, 0, sizeof(*m));
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) {
+            m->e[i][j].re = p[2 * (i * nc + j)];
+            m->e[i][j].im = p[2 * (i * nc + j) + 1];
+        }
+}
+static inline void mstore(int nc, const mat *m, double *p) {
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) {
+            p[2 * (i * nc + j)] = m->e[i][j].re;
+            p[2 * (i * nc + j) + 1] = m->e[i][j].im;
+        }
+}
+static inline void mzero(int nc, mat *m) {
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) m->e[i][j] = cmk(0.0, 0.0);
+}
+static inline void munit(int nc, mat *m) {
+    mzero(nc, m);
+    for (int i = 0; i < nc; i++) m->e[i][i] = cmk(1.0, 0.0);
+}
+/* matrixsun.h:298-327: res_ij = e_i0*A_0j (+ e_ik*A_kj, k ascending) */
+static inline mat mmul(int nc, const mat *a, const mat *b) {
+    mat r;
+    if (nc < NMAX) memset(&r, 0, sizeof(r));
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) {
+            cx s = cmul(a->e[i][0], b->e[0][j]);
+            for (int k = 1; k < nc; k++) s = cadd(s, cmul(a->e[i][k], b->e[k][j]));
+            r.e[i][j] = s;
+        }
+    return r;
+}
+/* matrixsun.h:568-583 */
+static inline mat mdag(int nc, const mat *a) {
+    mat r;
+    if (nc < NMAX) memset(&r, 0, sizeof(r));
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) r.e[i][j] = cconj(a->e[j][i]);
+    return r;
+}
+static inline void madd_to(int nc, mat *acc, const mat *a) {
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) acc->e[i][j] = cadd(acc->e[i][j], a->e[i][j]);
+}
+/* matrixsun.h:608-630 */
+static inline double mrealtrace(int nc, const mat *a) {
+    double s = a->e[0][0].re;
+    for (int i = 1; i < nc; i++) s += a->e[i][i].re;
+    return s;
+}
+static inline cx mtrace(int nc, const mat *a) {
+    cx s = a->e[0][0];
+    for (int i = 1; i < nc; i++) s = cadd(s, a->e[i][i]);
+    return s;
+}
+
+/* matrixsun.h:481-505: SU(3) cofactor expansion; Crout LU (no pivoting) for N!=3 */
+static cx mdet(int nc, const mat *a) {
+    cx res;
+    if (nc == 3) {
+        res = cmul(cmul(a->e[0][1], a->e[1][2]), a->e[2][0]);
+        res = csub(res, cmul(cmul(a->e[0][2], a->e[1][1]), a->e[2][0]));
+        res = cadd(res, cmul(cmul(a->e[0][2], a->e[1][0]), a->e[2][1]));
+        res = csub(res, cmul(cmul(a->e[0][0], a->e[1][2]), a->e[2][1]));
+        res = csub(res, cmul(cmul(a->e[0][1], a->e[1][0]), a->e[2][2]));
+        res = cadd(res, cmul(cmul(a->e[0][0], a->e[1][1]), a->e[2][2]));
+        return res;
+    }
+    mat b = *a;
+    for (int j = 0; j < nc; j++) {
+        for (int i = 0; i <= j; i++) {
+            res = b.e[j][i];
+            for (int c = 0; c < i; c++) res = csub(res, cmul(b.e[c][i], b.e[j][c]));
+            b.e[j][i] = res;
+        }
+        for (int i = j + 1; i < nc; i++) {
+            res = b.e[j][i];
+            for (int c = 0; c < j; c++) res = csub(res, cmul(b.e[c][i], b.e[j][c]));
+            /* conj(b_jj)*res / |b_jj|^2 ; complex/real division per complex.h:199 */
+            cx t = cmul(cconj(b.e[j][j]), res);
+            double d = cabs2(b.e[j][j]);
+            b.e[j][i] = cmk(t.re / d, t.im / d);
+        }
+    }
+    res = cmul(b.e[0][0], b.e[1][1]);
+    for (int c = 2; c < nc; c++) res = cmul(res, b.e[c][c]);
+    return res;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10                                                               */
+/* ------------------------------------------------------------------------- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+double or_u01(uint32_t lo, uint32_t hi) {
+    uint64_t m = ((((uint64_t)hi) << 32) | lo) >> 11;
+    return (double)(m + 1) * 0x1p-53;
+}
+
+/* uniform source: a Philox stream, or an explicit list (tests) */
+typedef struct {
+    uint32_t key[2];
+    uint32_t c0, c2, c3, j;
+    double cache;
+    int has;
+    const double *list;
+    int nlist, pos;
+} usrc;
+
+static void usrc_stream(usrc *s, uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3) {
+    s->key[0] = (uint32_t)seed; s->key[1] = (uint32_t)(seed >> 32);
+    s->c0 = c0; s->c2 = c2; s->c3 = c3; s->j = 0; s->has = 0; s->list = NULL;
+    s->nlist = 0; s->pos = 0; s->cache = 0.0;
+}
+static inline double unext(usrc *s) {
+    if (s->list) {
+        double v = (s->pos < s->nlist) ? s->list[s->pos] : 1.0;
+        s->pos++;
+        return v;
+    }
+    if (s->has) { s->has = 0; s->pos++; return s->cache; }
+    uint32_t ctr[4] = { s->c0, s->j, s->c2, s->c3 }, w[4];
+    or_philox4x32_10(ctr, s->key, w);
+    s->j++;
+    s->cache = or_u01(w[2], w[3]);
+    s->has = 1;
+    s->pos++;
+    return or_u01(w[0], w[1]);
+}
+
+void or_stream_uniforms(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, double *out) {
+    usrc s;
+    usrc_stream(&s, seed, c0, c2, c3);
+    for (int i = 0; i < n; i++) out[i] = unext(&s);
+}
+
+/* ------------------------------------------------------------------------- */
+/* SU(2) quaternion helpers (msu2.h:141-155, 211-219)                          */
+/* ------------------------------------------------------------------------- */
+static inline double q_abs(const double r[4]) {
+    return sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+}
+static inline void q_conj_normalize(double r[4]) {
+    double norm = 1.0 / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+    r[0] *= norm; r[1] *= -norm; r[2] *= -norm; r[3] *= -norm;
+}
+/* msu2.h:211-219: b = v * u^dagger */
+static inline void q_mul_uv_dagger(const double v[4], const double u[4], double b[4]) {
+    b[0] = v[0] * u[0] + v[1] * u[1] + v[2] * u[2] + v[3] * u[3];
+    b[1] = v[1] * u[0] - v[0] * u[1] + v[2] * u[3] - v[3] * u[2];
+    b[2] = v[2] * u[0] - v[0] * u[2] + v[3] * u[1] - v[1] * u[3];
+    b[3] = v[3] * u[0] - v[0] * u[3] + v[1] * u[2] - v[2] * u[1];
+}
+
+/* device_PHB_OVR.h:162-223 (MILC Kennedy-Pendleton / Creutz generator) */
+static void gen_su2_milc(double al, usrc *st, double a[4]) {
+    double xr1, xr2, xr3, xr4, d, r;
+    int k;
+    double a0 = 0.0;
+    xr1 = unext(st);
+    xr1 = log(xr1 + 1.e-10);
+    xr2 = unext(st);
+    xr2 = log(xr2 + 1.e-10);
+    xr3 = unext(st);
+    xr4 = unext(st);
+    xr3 = cos(PII * xr3);
+    d = -(xr2 + xr1 * xr3 * xr3) / al;
+    int nacd = 0;
+    if ((1.00 - 0.5 * d) > xr4 * xr4) nacd = 1;
+    if (nacd == 0 && al > 2.0) {
+        for (k = 0; k < 20; k++) {
+            xr1 = unext(st);
+            xr1 = log(xr1 + 1.e-10);
+            xr2 = unext(st);
+            xr2 = log(xr2 + 1.e-10);
+            xr3 = unext(st);
+            xr4 = unext(st);
+            xr3 = cos(PII * xr3);
+            d = -(xr2 + xr1 * xr3 * xr3) / al;
+            if ((1.00 - 0.5 * d) > xr4 * xr4) break;
+        }
+    }
+    if (nacd == 0 && al <= 2.0) {
+        xr3 = exp(-2.0 * al);
+        xr4 = 1.0 - xr3;
+        for (k = 0; k < 20; k++) {
+            xr1 = unext(st);
+            xr2 = unext(st);
+            r = xr3 + xr4 * xr1;
+            a0 = 1.00 + log(r) / al;
+            if ((1.0 - a0 * a0) > xr2 * xr2) break;
+        }
+        d = 1.0 - a0;
+    }
+    a[0] = 1.0 - d;
+    xr3 = 1.0 - a[0] * a[0];
+    xr3 = fabs(xr3);
+    r = sqrt(xr3);
+    a[3] = (2.0 * unext(st) - 1.0) * r;
+    xr1 = xr3 - a[3] * a[3];
+    xr1 = fabs(xr1);
+    xr1 = sqrt(xr1);
+    xr2 = PII * unext(st);
+    a[1] = xr1 * cos(xr2);
+    a[2] = xr1 * sin(xr2);
+}
+
+int or_generate_su2_milc(double al, const double *u, int nu, double a[4]) {
+    usrc s;
+    memset(&s, 0, sizeof(s));
+    s.list = u; s.nlist = nu; s.pos = 0;
+    gen_su2_milc(al, &s, a);
+    return s.pos;
+}
+
+void or_sample_a0(double al, uint64_t seed, int n, double *a0_out) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        usrc s;
+        double a[4];
+        usrc_stream(&s, seed, (uint32_t)i, 0u, RNG_TAG_TEST);
+        gen_su2_milc(al, &s, a);
+        a0_out[i] = a[0];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* geometry (include/index.h:133-202)                                          */
+/* ------------------------------------------------------------------------- */
+void or_eo_to_coords(int id, int parity, const int X[4], int x[4]) {
+    int za = id / (X[0] / 2);
+    int zb = za / X[1];
+    x[1] = za - zb * X[1];
+    x[3] = zb / X[2];
+    x[2] = zb - x[3] * X[2];
+    int xodd = (x[1] + x[2] + x[3] + parity) & 1;
+    x[0] = (2 * id + xodd) - za * X[0];
+}
+
+int or_coords_to_eo(const int x[4], const int X[4]) {
+    int lex = ((x[3] * X[2] + x[2]) * X[1] + x[1]) * X[0] + x[0];
+    int par = (x[0] + x[1] + x[2] + x[3]) & 1;
+    int vol = X[0] * X[1] * X[2] * X[3];
+    return (lex >> 1) + par * (vol / 2);
+}
+
+int or_neighbor_eo(int id, int parity, const int X[4], int mu, int lmu, int nu, int lnu) {
+    int x[4];
+    or_eo_to_coords(id, parity, X, x);
+    x[mu] = (x[mu] + lmu + X[mu]) % X[mu];
+    if (nu >= 0) x[nu] = (x[nu] + lnu + X[nu]) % X[nu];
+    return or_coords_to_eo(x, X);
+}
+
+static inline long volume_of(const int X[4]) { return (long)X[0] * X[1] * X[2] * X[3]; }
+static inline const double *link_at(int nc, const double *U, long V, int mu, long s) {
+    return U + ((long)mu * V + s) * nc * nc * 2;
+}
+static inline double *link_at_w(int nc, double *U, long V, int mu, long s) {
+    return U + ((long)mu * V + s) * nc * nc * 2;
+}
+
+/* ------------------------------------------------------------------------- */
+/* storage codecs                                                              */
+/* ------------------------------------------------------------------------- */
+/* reconstruct_12p_8p.h:17-21: third row = conj(r0 x r1) */
+static void reconstruct12(mat *A) {
+    A->e[2][0] = cconj(csub(cmul(A->e[0][1], A->e[1][2]), cmul(A->e[0][2], A->e[1][1])));
+    A->e[2][1] = cconj(csub(cmul(A->e[0][2], A->e[1][0]), cmul(A->e[0][0], A->e[1][2])));
+    A->e[2][2] = cconj(csub(cmul(A->e[0][0], A->e[1][1]), cmul(A->e[0][1], A->e[1][0])));
+}
+
+/* reconstruct_12p_8p.h:38-82 */
+static void reconstruct8(mat *m, cx theta) {
+    double p1, p2, diff;
+    cx temp;
+    p1 = cabs2(m->e[0][1]) + cabs2(m->e[0][2]);
+    p2 = 1.0 / p1;
+    diff = 1.0 - p1;
+    p1 = sqrt(diff >= 0 ? diff : 0.0);
+    m->e[0][0].re = p1 * cos(theta.re);
+    m->e[0][0].im = p1 * sin(theta.re);
+    diff = 1.0 - cabs2(m->e[0][0]) - cabs2(m->e[1][0]);
+    p1 = sqrt(diff >= 0 ? diff : 0.0);
+    m->e[2][0].re = p1 * cos(theta.im);
+    m->e[2][0].im = p1 * sin(theta.im);
+    m->e[1][1] = cconj(cmul(m->e[2][0], m->e[0][2]));
+    m->e[2][2] = cmul(cconj(m->e[0][0]), m->e[1][0]);
+    m->e[1][1] = cadd(m->e[1][1], cmul(m->e[2][2], m->e[0][1]));
+    m->e[1][1] = cscale(m->e[1][1], -p2);
+    m->e[1][2] = cconj(cmul(m->e[2][0], m->e[0][1]));
+    m->e[1][2] = csub(m->e[1][2], cmul(m->e[2][2], m->e[0][2]));
+    m->e[1][2] = cscale(m->e[1][2], p2);
+    m->e[2][1] = cconj(cmul(m->e[1][0], m->e[0][2]));
+    temp = cmul(cconj(m->e[0][0]), m->e[2][0]);
+    m->e[2][1] = csub(m->e[2][1], cmul(temp, m->e[0][1]));
+    m->e[2][1] = cscale(m->e[2][1], p2);
+    m->e[2][2] = cconj(cmul(m->e[1][0], m->e[0][1]));
+    m->e[2][2] = cadd(m->e[2][2], cmul(temp, m->e[0][2]));
+    m->e[2][2] = cscale(m->e[2][2], -p2);
+}
+
+void or_encode12(const double *link, double *p12) { memcpy(p12, link, 12 * sizeof(double)); }
+void or_decode12(const double *p12, double *link) {
+    mat m;
+    memset(&m, 0, sizeof(m));
+    for (int j = 0; j < 3; j++) {
+        m.e[0][j] = cmk(p12[2 * j], p12[2 * j + 1]);
+        m.e[1][j] = cmk(p12[6 + 2 * j], p12[6 + 2 * j + 1]);
+    }
+    reconstruct12(&m);
+    mstore(3, &m, link);
+}
+/* device_load_save.h:112-120: (e01, e02, e10, (arg e00, arg e20)) */
+void or_encode8(const double *link, double *p8) {
+    mat m;
+    mload(3, link, &m);
+    p8[0] = m.e[0][1].re; p8[1] = m.e[0][1].im;
+    p8[2] = m.e[0][2].re; p8[3] = m.e[0][2].im;
+    p8[4] = m.e[1][0].re; p8[5] = m.e[1][0].im;
+    p8[6] = atan2(m.e[0][0].im, m.e[0][0].re);
+    p8[7] = atan2(m.e[2][0].im, m.e[2][0].re);
+}
+void or_decode8(const double *p8, double *link) {
+    mat m;
+    memset(&m, 0, sizeof(m));
+    m.e[0][1] = cmk(p8[0], p8[1]);
+    m.e[0][2] = cmk(p8[2], p8[3]);
+    m.e[1][0] = cmk(p8[4], p8[5]);
+    reconstruct8(&m, cmk(p8[6], p8[7]));
+    mstore(3, &m, link);
+}
+
+static inline void project_link(int nc, int storage, double *l) {
+    if (nc != 3 || storage == OR_STORE_18) return;
+    double p[12];
+    if (storage == OR_STORE_12) { or_encode12(l, p); or_decode12(p, l); }
+    else { or_encode8(l, p); or_decode8(p, l); }
+}
+
+void or_project_storage(int nc, int storage, long nlinks, double *U) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < nlinks; i++) project_link(nc, storage, U + i * nc * nc * 2);
+}
+
+/* ------------------------------------------------------------------------- */
+/* reunitarisation                                                             */
+/* ------------------------------------------------------------------------- */
+/* reunitlink.h:16-85 (N=3: row Gram-Schmidt + cross product; N>3: column G-S +
+ * determinant phase).  N=2 is not compilable in the reference (reunitlink.h:86);
+ * here the textbook SU(2) projection a=(U00+U11*)/2, b=(U01-U10*)/2 normalised. */
+static void reunit_mat(int nc, mat *U) {
+    if (nc == 3) {
+        double t1 = 0.0;
+        cx t2 = cmk(0.0, 0.0);
+        for (int c = 0; c < 3; c++) t1 += cabs2(U->e[0][c]);
+        t1 = 1.0 / sqrt(t1);
+        for (int c = 0; c < 3; c++) U->e[0][c] = cscale(U->e[0][c], t1);
+        for (int c = 0; c < 3; c++) t2 = cadd(t2, cmul(cconj(U->e[0][c]), U->e[1][c]));
+        for (int c = 0; c < 3; c++) U->e[1][c] = csub(U->e[1][c], cmul(t2, U->e[0][c]));
+        t1 = 0.0;
+        for (int c = 0; c < 3; c++) t1 += cabs2(U->e[1][c]);
+        t1 = 1.0 / sqrt(t1);
+        for (int c = 0; c < 3; c++) U->e[1][c] = cscale(U->e[1][c], t1);
+        reconstruct12(U);
+        return;
+    }
+    if (nc == 2) {
+        cx a = cmk((U->e[0][0].re + U->e[1][1].re) * 0.5, (U->e[0][0].im - U->e[1][1].im) * 0.5);
+        cx b = cmk((U->e[0][1].re - U->e[1][0].re) * 0.5, (U->e[0][1].im + U->e[1][0].im) * 0.5);
+        double n = 1.0 / sqrt(cabs2(a) + cabs2(b));
+        a = cscale(a, n);
+        b = cscale(b, n);
+        U->e[0][0] = a;
+        U->e[0][1] = b;
+        U->e[1][0] = cmk(-b.re, b.im);
+        U->e[1][1] = cconj(a);
+        return;
+    }
+    double t1 = 0.0;
+    cx t2;
+    for (int c = 0; c < nc; c++) t1 += cabs2(U->e[c][0]);
+    t1 = 1.0 / sqrt(t1);
+    for (int c = 0; c < nc; c++) U->e[c][0] = cscale(U->e[c][0], t1);
+    for (int j = 1; j < nc; j++) {
+        for (int i = 0; i < j; i++) {
+            t2 = cmul(cconj(U->e[0][i]), U->e[0][j]);
+            for (int c = 1; c < nc; ++c) t2 = cadd(t2, cmul(cconj(U->e[c][i]), U->e[c][j]));
+            for (int c = 0; c < nc; ++c) U->e[c][j] = csub(U->e[c][j], cmul(t2, U->e[c][i]));
+        }
+        t1 = 0.0;
+        for (int c = 0; c < nc; ++c) t1 += cabs2(U->e[c][j]);
+        t1 = 1.0 / sqrt(t1);
+        for (int c = 0; c < nc; ++c) U->e[c][j] = cscale(U->e[c][j], t1);
+    }
+    t2 = mdet(nc, U);
+    double ai = atan2(t2.im, t2.re);
+    t2 = cmk(cos(ai), -sin(ai));
+    for (int c = 0; c < nc; ++c) U->e[c][nc - 1] = cmul(U->e[c][nc - 1], t2);
+}
+
+void or_reunit_link(int nc, double *link) {
+    mat m;
+    mload(nc, link, &m);
+    reunit_mat(nc, &m);
+    mstore(nc, &m, link);
+}
+
+/* reunitarize.cu:23-30: load (decode), reunit, store (encode) for every link */
+void or_reunitarize(int nc, long nlinks, int storage, double *U) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < nlinks; i++) {
+        double *l = U + i * nc * nc * 2;
+        or_reunit_link(nc, l);
+        project_link(nc, storage, l);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* initialisation                                                              */
+/* ------------------------------------------------------------------------- */
+void or_cold_start(int nc, const int X[4], double *U) {
+    long V = volume_of(X);
+    mat one;
+    munit(nc, &one);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < 4 * V; i++) mstore(nc, &one, U + i * nc * nc * 2);
+}
+
+/* Haar hot start (replaces the reference's non-Haar randomize+reunit,
+ * device_PHB_OVR.h:645-663 / gaugearray.cu:353-374): complex Gaussian entries by
+ * Box-Muller from the stream (seed, link, 0, TAG_HOT|mu), one Philox call per
+ * entry, row-major; then the reunitarisation above (Gram-Schmidt of a Gaussian
+ * matrix is Haar on U(N); the det-phase / cross-product step makes it SU(N)). */
+void or_hot_start(int nc, const int X[4], uint64_t seed, int storage, double *U) {
+    long V = volume_of(X);
+#pragma omp parallel for schedule(static)
+    for (long l = 0; l < 4 * V; l++) {
+        int mu = (int)(l / V);
+        long s = l - (long)mu * V;
+        usrc st;
+        usrc_stream(&st, seed, (uint32_t)s, 0u, RNG_TAG_HOT | (uint32_t)mu);
+        mat m;
+        for (int i = 0; i < nc; i++)
+            for (int j = 0; j < nc; j++) {
+                double u1 = unext(&st), u2 = unext(&st);
+                double rr = sqrt(-2.0 * log(u1));
+                double ph = PII * u2;
+                m.e[i][j] = cmk(rr * cos(ph), rr * sin(ph));
+            }
+        reunit_mat(nc, &m);
+        double *dst = link_at_w(nc, U, V, mu, s);
+        mstore(nc, &m, dst);
+        project_link(nc, storage, dst);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* staple (monte.cu:63-80)                                                     */
+/* ------------------------------------------------------------------------- */
+static mat staple_at(int nc, const int X[4], const double *U, int parity, int mu, int id) {
+    long V = volume_of(X);
+    long half = V / 2;
+    long idxoddbit = id + parity * half;
+    mat staple, link, t;
+    mzero(nc, &staple);
+    int newidmu1 = or_neighbor_eo(id, parity, X, mu, 1, -1, 0);
+    for (int nu = 0; nu < 4; nu++) {
+        if (nu == mu) continue;
+        mat A, B, C;
+        /* up: U_nu(x) U_mu(x+nu) U_nu(x+mu)^dagger */
+        mload(nc, link_at(nc, U, V, nu, idxoddbit), &A);
+        mload(nc, link_at(nc, U, V, mu, or_neighbor_eo(id, parity, X, nu, 1, -1, 0)), &B);
+        mload(nc, link_at(nc, U, V, nu, newidmu1), &C);
+        link = mmul(nc, &A, &B);
+        t = mdag(nc, &C);
+        link = mmul(nc, &link, &t);
+        madd_to(nc, &staple, &link);
+        /* down: U_nu(x-nu)^dagger U_mu(x-nu) U_nu(x-nu+mu) */
+        int newidnum1 = or_neighbor_eo(id, parity, X, nu, -1, -1, 0);
+        mload(nc, link_at(nc, U, V, nu, newidnum1), &A);
+        mload(nc, link_at(nc, U, V, mu, newidnum1), &B);
+        mload(nc, link_at(nc, U, V, nu, or_neighbor_eo(id, parity, X, mu, 1, nu, -1)), &C);
+        link = mdag(nc, &A);
+        link = mmul(nc, &link, &B);
+        link = mmul(nc, &link, &C);
+        madd_to(nc, &staple, &link);
+    }
+    return staple;
+}
+
+void or_staple(int nc, const int X[4], const double *U, int parity, int mu, int id, double *S) {
+    mat s = staple_at(nc, X, U, parity, mu, id);
+    mstore(nc, &s, S);
+}
+
+/* ------------------------------------------------------------------------- */
+/* IndexBlock (constants.h:143-192)                                            */
+/* ------------------------------------------------------------------------- */
+static void index_block(int nc, int block, int *p, int *q) {
+    if (nc == 3) {
+        if (block == 0) { *p = 0; *q = 1; }
+        else if (block == 1) { *p = 1; *q = 2; }
+        else { *p = 0; *q = 2; }
+        return;
+    }
+    int i1 = 0, found = 0, del_i = 0, index = -1;
+    while (del_i < (nc - 1) && found == 0) {
+        del_i++;
+        for (i1 = 0; i1 < (nc - del_i); i1++) {
+            index++;
+            if (index == block) { found = 1; break; }
+        }
+    }
+    *q = i1 + del_i;
+    *p = i1;
+}
+
+/* device_PHB_OVR.h:296-303: link <- u * link on rows (x,y) */
+static void mul_block_sun(int nc, const double u[4], mat *link, int x, int y) {
+    cx a00 = cmk(u[0], u[3]), a01 = cmk(u[2], u[1]);
+    cx a10 = cmk(-u[2], u[1]), a11 = cmk(u[0], -u[3]);
+    for (int j = 0; j < nc; j++) {
+        cx tmp = cadd(cmul(a00, link->e[x][j]), cmul(a01, link->e[y][j]));
+        link->e[y][j] = cadd(cmul(a10, link->e[x][j]), cmul(a11, link->e[y][j]));
+        link->e[x][j] = tmp;
+    }
+}
+
+/* device_PHB_OVR.h:265-272 */
+static void get_block_su2(const mat *m, int x, int y, double r[4]) {
+    r[0] = m->e[x][x].re + m->e[y][y].re;
+    r[1] = m->e[x][y].im + m->e[y][x].im;
+    r[2] = m->e[x][y].re - m->e[y][x].re;
+    r[3] = m->e[x][x].im - m->e[y][y].im;
+}
+
+/* rows p,q of U times columns p,q of F (device_PHB_OVR.h:385-400) */
+static void block_proj_uf(const mat *U, const mat *F, int p, int q, double r[4]) {
+    cx a0 = cmk(0, 0), a1 = cmk(0, 0), a2 = cmk(0, 0), a3 = cmk(0, 0);
+    for (int j = 0; j < 3; j++) {
+        a0 = cadd(a0, cmul(U->e[p][j], F->e[j][p]));
+        a1 = cadd(a1, cmul(U->e[p][j], F->e[j][q]));
+        a2 = cadd(a2, cmul(U->e[q][j], F->e[j][p]));
+        a3 = cadd(a3, cmul(U->e[q][j], F->e[j][q]));
+    }
+    r[0] = a0.re + a3.re;
+    r[1] = a1.im + a2.im;
+    r[2] = a1.re - a2.re;
+    r[3] = a0.im - a3.im;
+}
+
+/* device_PHB_OVR.h:361-485 */
+static void heatbath_mat(int nc, double beta_over_nc, mat *U, const mat *F, usrc *st) {
+    if (nc == 3) {
+        for (int block = 0; block < 3; block++) {
+            int p, q;
+            index_block(3, block, &p, &q);
+            double r[4], a[4], rr[4];
+            block_proj_uf(U, F, p, q, r);
+            double k = q_abs(r);
+            double ap = beta_over_nc * k;
+            k = 1.0 / k;
+            for (int i = 0; i < 4; i++) r[i] *= k;
+            gen_su2_milc(ap, st, a);
+            q_mul_uv_dagger(a, r, rr);
+            cx a0 = cmk(rr[0], rr[3]), a1 = cmk(rr[2], rr[1]);
+            cx a2 = cmk(-rr[2], rr[1]), a3 = cmk(rr[0], -rr[3]);
+            for (int j = 0; j < 3; j++) {
+                cx tmp0 = cadd(cmul(a0, U->e[p][j]), cmul(a1, U->e[q][j]));
+                U->e[q][j] = cadd(cmul(a2, U->e[p][j]), cmul(a3, U->e[q][j]));
+                U->e[p][j] = tmp0;
+            }
+        }
+        return;
+    }
+    mat M = mmul(nc, U, F);
+    for (int block = 0; block < nc * (nc - 1) / 2; block++) {
+        int x, y;
+        index_block(nc, block, &x, &y);
+        double r[4], a[4], rr[4];
+        get_block_su2(&M, x, y, r);
+        double k = q_abs(r);
+        double ap = beta_over_nc * k;
+        k = 1.0 / k;
+        for (int i = 0; i < 4; i++) r[i] *= k;
+        gen_su2_milc(ap, st, a);
+        q_mul_uv_dagger(a, r, rr);
+        mul_block_sun(nc, rr, U, x, y);
+        mul_block_sun(nc, rr, &M, x, y);
+    }
+}
+
+/* device_PHB_OVR.h:493-608 */
+static void overrelax_mat(int nc, mat *U, const mat *F) {
+    if (nc == 3) {
+        for (int block = 0; block < 3; block++) {
+            int p, q;
+            index_block(3, block, &p, &q);
+            double r[4];
+            block_proj_uf(U, F, p, q, r);
+            q_conj_normalize(r);
+            cx a0 = cmk(r[0], r[3]), a1 = cmk(r[2], r[1]);
+            cx a2 = cmk(-r[2], r[1]), a3 = cmk(r[0], -r[3]);
+            for (int j = 0; j < 3; j++) {
+                cx tmp0 = cadd(cmul(a0, U->e[p][j]), cmul(a1, U->e[q][j]));
+                cx tmp1 = cadd(cmul(a2, U->e[p][j]), cmul(a3, U->e[q][j]));
+                U->e[p][j] = cadd(cmul(a0, tmp0), cmul(a1, tmp1));
+                U->e[q][j] = cadd(cmul(a2, tmp0), cmul(a3, tmp1));
+            }
+        }
+        return;
+    }
+    mat M = mmul(nc, U, F);
+    for (int block = 0; block < nc * (nc - 1) / 2; block++) {
+        int x, y;
+        index_block(nc, block, &x, &y);
+        double r[4];
+        get_block_su2(&M, x, y, r);
+        q_conj_normalize(r);
+        mul_block_sun(nc, r, U, x, y);
+        mul_block_sun(nc, r, U, x, y);
+        mul_block_sun(nc, r, &M, x, y);
+        mul_block_sun(nc, r, &M, x, y);
+    }
+}
+
+void or_overrelax_link(int nc, double *U, const double *F) {
+    mat u, f;
+    mload(nc, U, &u);
+    mload(nc, F, &f);
+    overrelax_mat(nc, &u, &f);
+    mstore(nc, &u, U);
+}
+
+void or_heatbath_link(int nc, double beta, double *U, const double *F,
+                      uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3) {
+    mat u, f;
+    usrc st;
+    usrc_stream(&st, seed, c0, c2, c3);
+    mload(nc, U, &u);
+    mload(nc, F, &f);
+    heatbath_mat(nc, beta / (double)nc, &u, &f, &st);
+    mstore(nc, &u, U);
+}
+
+static int check_args(int nc, const int X[4], int storage) {
+    if (nc < 2 || nc > NMAX) return -1;
+    for (int i = 0; i < 4; i++)
+        if (X[i] < 2 || (X[i] & 1)) return -2;
+    if (storage != OR_STORE_18 && storage != OR_STORE_12 && storage != OR_STORE_8) return -3;
+    if (nc != 3 && storage != OR_STORE_18) return -3;
+    return 0;
+}
+
+/* monte.cu:44-88: one (parity, mu) heatbath launch.  RNG stream of the site:
+ * key = seed, counter = (id, draw, sweep, TAG_HB | parity<<4 | mu). */
+int or_heatbath_sub(int nc, const int X[4], double beta, uint64_t seed, uint32_t sweep,
+                    int storage, double *U, int parity, int mu) {
+    int rc = check_args(nc, X, storage);
+    if (rc) return rc;
+    long V = volume_of(X);
+    long half = V / 2;
+    double bnc = beta / (double)nc;   /* constants.cu:443 */
+#pragma omp parallel for schedule(static)
+    for (long id = 0; id < half; id++) {
+        mat S = staple_at(nc, X, U, parity, mu, (int)id);
+        mat F = mdag(nc, &S);
+        usrc st;
+        usrc_stream(&st, seed, (uint32_t)id, sweep, RNG_TAG_HB | ((uint32_t)parity << 4) | (uint32_t)mu);
+        double *l = link_at_w(nc, U, V, mu, id + parity * half);
+        mat u;
+        mload(nc, l, &u);
+        heatbath_mat(nc, bnc, &u, &F, &st);
+        mstore(nc, &u, l);
+        project_link(nc, storage, l);
+    }
+    return 0;
+}
+
+/* ovr.cu:39-80 */
+int or_overrelax_sub(int nc, const int X[4], int storage, double *U, int parity, int mu) {
+    int rc = check_args(nc, X, storage);
+    if (rc) return rc;
+    long V = volume_of(X);
+    long half = V / 2;
+#pragma omp parallel for schedule(static)
+    for (long id = 0; id < half; id++) {
+        mat S = staple_at(nc, X, U, parity, mu, (int)id);
+        mat F = mdag(nc, &S);
+        double *l = link_at_w(nc, U, V, mu, id + parity * half);
+        mat u;
+        mload(nc, l, &u);
+        overrelax_mat(nc, &u, &F);
+        mstore(nc, &u, l);
+        project_link(nc, storage, l);
+    }
+    return 0;
+}
+
+int or_heatbath_sweep(int nc, const int X[4], double beta, uint64_t seed, uint32_t sweep,
+                      int storage, double *U) {
+    for (int parity = 0; parity < 2; parity++)
+        for (int mu = 0; mu < 4; mu++) {
+            int rc = or_heatbath_sub(nc, X, beta, seed, sweep, storage, U, parity, mu);
+            if (rc) return rc;
+        }
+    return 0;
+}
+
+int or_overrelax_sweep(int nc, const int X[4], int storage, double *U) {
+    for (int parity = 0; parity < 2; parity++)
+        for (int mu = 0; mu < 4; mu++) {
+            int rc = or_overrelax_sub(nc, X, storage, U, parity, mu);
+            if (rc) return rc;
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* observables; partial sums over fixed chunks of sites, then summed in order,  */
+/* so the result does not depend on the thread count                          */
+/* ------------------------------------------------------------------------- */
+#define CHUNK 4096
+
+void or_plaquette(int nc, const int X[4], const double *U, double out[2]) {
+    long V = volume_of(X);
+    long half = V / 2;
+    long nchunk = (V + CHUNK - 1) / CHUNK;
+    double *part = (double *)calloc((size_t)nchunk * 2, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nchunk; c++) {
+        double sp = 0.0, tp = 0.0;
+        long end = (c + 1) * CHUNK < V ? (c + 1) * CHUNK : V;
+        for (long idd = c * CHUNK; idd < end; idd++) {
+            int parity = idd >= half ? 1 : 0;
+            int id = (int)(parity ? idd - half : idd);
+            long idxoddbit = id + parity * half;
+            for (int mu = 0; mu < 3; mu++) {
+                mat link1, A, B, C, t, link;
+                mload(nc, link_at(nc, U, V, mu, idxoddbit), &link1);
+                int newidmu1 = or_neighbor_eo(id, parity, X, mu, 1, -1, 0);
+                for (int nu = mu + 1; nu < 4; nu++) {
+                    mload(nc, link_at(nc, U, V, nu, newidmu1), &A);
+                    mload(nc, link_at(nc, U, V, mu, or_neighbor_eo(id, parity, X, nu, 1, -1, 0)), &B);
+                    mload(nc, link_at(nc, U, V, nu, idxoddbit), &C);
+                    t = mdag(nc, &B);
+                    link = mmul(nc, &A, &t);
+                    t = mdag(nc, &C);
+                    link = mmul(nc, &link, &t);
+                    t = mmul(nc, &link1, &link);
+                    double tr = mrealtrace(nc, &t);
+                    if (nu == 3) tp += tr; else sp += tr;
+                }
+            }
+        }
+        part[2 * c] = sp;
+        part[2 * c + 1] = tp;
+    }
+    double s = 0.0, t = 0.0;
+    for (long c = 0; c < nchunk; c++) { s += part[2 * c]; t += part[2 * c + 1]; }
+    free(part);
+    double norm = (double)(3 * nc) * (double)V;
+    out[0] = s / norm;
+    out[1] = t / norm;
+}
+
+void or_polyakov(int nc, const int X[4], const double *U, double out[2]) {
+    long V = volume_of(X);
+    long half = V / 2;
+    long ts = (long)X[0] * X[1] * X[2];
+    long nchunk = (ts + CHUNK - 1) / CHUNK;
+    double *part = (double *)calloc((size_t)nchunk * 2, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nchunk; c++) {
+        double sr = 0.0, si = 0.0;
+        long end = (c + 1) * CHUNK < ts ? (c + 1) * CHUNK : ts;
+        for (long idd = c * CHUNK; idd < end; idd++) {
+            int parity = idd >= ts / 2 ? 1 : 0;
+            int id = (int)(parity ? idd - ts / 2 : idd);
+            int x[4];
+            or_eo_to_coords(id, parity, X, x);
+            long idx = ((long)x[2] * X[1] + x[1]) * X[0] + x[0];
+            int idl = x[0] + x[1] + x[2];
+            mat L, A;
+            munit(nc, &L);
+            for (int t = 0; t < X[3]; t++) {
+                long id0 = (idx + (long)t * ts) >> 1;
+                id0 += ((idl + t) & 1) * half;
+                mload(nc, link_at(nc, U, V, 3, id0), &A);
+                L = mmul(nc, &L, &A);
+            }
+            cx tr = mtrace(nc, &L);
+            sr += tr.re;
+            si += tr.im;
+        }
+        part[2 * c] = sr;
+        part[2 * c + 1] = si;
+    }
+    double s = 0.0, t = 0.0;
+    for (long c = 0; c < nchunk; c++) { s += part[2 * c]; t += part[2 * c + 1]; }
+    free(part);
+    double norm = (double)nc * (double)ts;
+    out[0] = s / norm;
+    out[1] = t / norm;
+}
+
+/* ------------------------------------------------------------------------- */
+void or_matmul(int nc, const double *A, const double *B, double *C) {
+    mat a, b, c;
+    mload(nc, A, &a);
+    mload(nc, B, &b);
+    c = mmul(nc, &a, &b);
+    mstore(nc, &c, C);
+}
+
+void or_det(int nc, const double *A, double out[2]) {
+    mat a;
+    mload(nc, A, &a);
+    cx d = mdet(nc, &a);
+    out[0] = d.re;
+    out[1] = d.im;
+}
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void or_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

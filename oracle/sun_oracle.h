@@ -1,0 +1,98 @@
+/*
+ * sun_oracle.h -- FP64 CPU restatement of the quenched SU(N) ensemble-generation
+ * path of nmrcardoso/sun (CULQCD).  TEST INFRASTRUCTURE ONLY: this library is the
+ * checker that the HIP engine is compared against.  It is linked by the tests, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and by nothing else.
+ *
+ * Host link layout used throughout ("canonical layout"):
+ *     double U[4][V][N][N][2]      (mu-major, then even/odd site index, row-major
+ *                                    complex entries, re/im interleaved)
+ * where the even/odd site index of x is  eo(x) = parity(x)*V/2 + lex(x)/2,
+ * lex(x) = x0 + N0*(x1 + N1*(x2 + N2*x3)),  as in include/index.h:133-141,171-183.
+ *
+ * Every function states the reference file:line it restates.  Pinning of each
+ * function (known-answer vectors, closed forms, invariants, brute force) lives in
+ * tests/test_oracle_*.py.
+ */
+#ifndef SUN_ORACLE_H
+#define SUN_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Storage codes, matching the reference's ArrayType (include/modes.h): full
+ * N*N complex ("SOA", 18 reals for SU(3)), 12-real ("SOA12") and 8-real ("SOA8"). */
+enum { OR_STORE_18 = 18, OR_STORE_12 = 12, OR_STORE_8 = 8 };
+
+/* ---- counter-based RNG (Philox4x32-10, Salmon et al. SC'11) ---------------- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* uniform double in (0,1] from two 32-bit words: ((hi<<32|lo)>>11)+1) * 2^-53 */
+double or_u01(uint32_t lo, uint32_t hi);
+/* n uniforms of the per-site stream (seed, c0, c2, c3), draw counter starting at 0 */
+void or_stream_uniforms(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, double *out);
+
+/* ---- SU(2) sampler (device_PHB_OVR.h:162-223), driven by a uniform list ----- */
+/* Draws from u[] sequentially; returns number of uniforms consumed; a[4] out.    */
+int or_generate_su2_milc(double al, const double *u, int nu, double a[4]);
+/* n samples of a0 at weight al, drawn from stream (seed, i, 0, 0x30000000). */
+void or_sample_a0(double al, uint64_t seed, int n, double *a0_out);
+
+/* ---- geometry (include/index.h:133-202) ----------------------------------- */
+void or_eo_to_coords(int id, int parity, const int dims[4], int x[4]);
+int  or_coords_to_eo(const int x[4], const int dims[4]);
+int  or_neighbor_eo(int id, int parity, const int dims[4], int mu, int lmu, int nu, int lnu);
+
+/* ---- field initialisation -------------------------------------------------- */
+void or_cold_start(int nc, const int dims[4], double *U);             /* gaugearray.cu:276-305 */
+void or_hot_start(int nc, const int dims[4], uint64_t seed, int storage, double *U); /* Haar */
+
+/* ---- storage codecs (device_load_save.h:76-322, reconstruct_12p_8p.h:17-82) - */
+void or_encode12(const double *link, double *p12);       /* rows 0,1: 12 reals        */
+void or_decode12(const double *p12, double *link);       /* row 2 = conj(r0 x r1)      */
+void or_encode8(const double *link, double *p8);         /* e01,e02,e10,(arg e00,arg e20) */
+void or_decode8(const double *p8, double *link);
+void or_project_storage(int nc, int storage, long nlinks, double *U);  /* decode(encode(U)) */
+
+/* ---- staple (monte.cu:63-80) ------------------------------------------------ */
+void or_staple(int nc, const int dims[4], const double *U, int parity, int mu, int id, double *S);
+
+/* ---- one (parity,mu) sub-sweep and full sweeps ------------------------------ */
+/* heatbath: monte.cu:44-88 + device_PHB_OVR.h:361-485; overrelax: ovr.cu:39-80 +
+ * device_PHB_OVR.h:493-608.  Returns 0, or <0 on invalid arguments.             */
+int or_heatbath_sub(int nc, const int dims[4], double beta, uint64_t seed, uint32_t sweep,
+                    int storage, double *U, int parity, int mu);
+int or_overrelax_sub(int nc, const int dims[4], int storage, double *U, int parity, int mu);
+/* parity-outer, mu-inner schedule (monte.cu:205-206, ovr.cu:191-192) */
+int or_heatbath_sweep(int nc, const int dims[4], double beta, uint64_t seed, uint32_t sweep,
+                      int storage, double *U);
+int or_overrelax_sweep(int nc, const int dims[4], int storage, double *U);
+
+/* ---- single-link updates (exposed for closed-form tests) --------------------- */
+void or_overrelax_link(int nc, double *U, const double *F);
+/* heatbath of one link; uniforms from stream (seed, c0, c2, c3) */
+void or_heatbath_link(int nc, double beta, double *U, const double *F,
+                      uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3);
+
+/* ---- reunitarisation (reunitlink.h:16-85; SU(2) textbook projection) -------- */
+void or_reunit_link(int nc, double *link);
+void or_reunitarize(int nc, long nlinks, int storage, double *U);
+
+/* ---- observables ------------------------------------------------------------ */
+/* plaquette_cub.cu:96-149,234: out[0]=spatial, out[1]=temporal, each / (3*N*V) */
+void or_plaquette(int nc, const int dims[4], const double *U, double out[2]);
+/* polyakovloop.cu:53-99,651: complex mean trace / (N*Nx*Ny*Nz) */
+void or_polyakov(int nc, const int dims[4], const double *U, double out[2]);
+
+/* ---- helpers ---------------------------------------------------------------- */
+void or_matmul(int nc, const double *A, const double *B, double *C);     /* C = A*B */
+void or_det(int nc, const double *A, double out[2]);
+int  or_num_threads(void);
+void or_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

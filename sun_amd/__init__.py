@@ -1,0 +1,318 @@
+"""sun_amd -- Python binding of the MI355X SU(N) quenched ensemble-generation engine.
+
+Everything here calls the C ABI of ``libsun_hip.so`` (declared in ``include/sun.h``)
+through ctypes; there is no fallback.  If the library is missing or does not load,
+``lib()`` raises: build it with ``python sun_amd/build.py`` (or
+``__graft_entry__.build()``).
+
+The classes mirror the reference's host API (namespace CULQCD):
+
+=====================  =========================================================
+reference              here
+=====================  =========================================================
+initCULQCD/EndCULQCD   ``Context`` (``sun_init`` / ``sun_finalize``)
+SETPARAMS + _gauge     ``Lattice`` (``sun_lattice_create`` / ``destroy``)
+gauge.Init()           ``Lattice.cold_start()``
+gauge.Init(RNG)        ``Lattice.hot_start(seed)``
+HeatBath(...).Run()    ``Lattice.heatbath(seed, sweep)``
+OverRelaxation.Run()   ``Lattice.overrelax()``
+Reunitarize.Run()      ``Lattice.reunitarize()``
+PlaquetteCUB.Run()     ``Lattice.plaquette()``  -> (spatial, temporal)
+OnePolyakovLoop.Run()  ``Lattice.polyakov()``   -> complex
+=====================  =========================================================
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsun_hip.so")
+
+STORE_18, STORE_12, STORE_8 = 18, 12, 8
+F32, F64 = 4, 8
+_lib = None
+
+EXPORTS = [
+    "sun_init", "sun_finalize", "sun_last_error", "sun_version",
+    "sun_lattice_create", "sun_lattice_destroy", "sun_lattice_bytes",
+    "sun_cold_start", "sun_hot_start",
+    "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
+    "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
+    "sun_links_to_host", "sun_links_from_host",
+    "sun_generate",
+    "sun_batch_create", "sun_batch_destroy", "sun_batch_heatbath", "sun_batch_overrelax",
+    "sun_batch_reunitarize", "sun_batch_plaquette", "sun_batch_generate",
+    "sun_stream_create", "sun_stream_destroy", "sun_stream_synchronize",
+    "sun_event_create", "sun_event_destroy", "sun_event_record", "sun_event_elapsed_ms",
+    "sun_device_name",
+]
+
+
+class SunError(RuntimeError):
+    pass
+
+
+class LatticeDesc(ctypes.Structure):
+    _fields_ = [("nc", ctypes.c_int), ("dims", ctypes.c_int * 4), ("beta", ctypes.c_double),
+                ("storage", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class GenerateDesc(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("first_sweep", ctypes.c_uint32), ("n_iter", ctypes.c_int),
+                ("n_or", ctypes.c_int), ("reunit_every", ctypes.c_int), ("measure", ctypes.c_int),
+                ("use_graph", ctypes.c_int)]
+
+
+def lib() -> ctypes.CDLL:
+    """Load libsun_hip.so; raises if it is absent (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SunError(f"{LIB_PATH} is missing: build it with `python sun_amd/build.py`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    u32, u64 = ctypes.c_uint32, ctypes.c_uint64
+    dp = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "sun_init": (ci, [ci, ci, ci, ctypes.POINTER(vp)]),
+        "sun_finalize": (ci, [vp]),
+        "sun_last_error": (ctypes.c_char_p, []),
+        "sun_version": (ctypes.c_char_p, []),
+        "sun_lattice_create": (ci, [vp, ctypes.POINTER(LatticeDesc), ctypes.POINTER(vp)]),
+        "sun_lattice_destroy": (ci, [vp]),
+        "sun_lattice_bytes": (ctypes.c_size_t, [ctypes.POINTER(LatticeDesc)]),
+        "sun_cold_start": (ci, [vp, vp]),
+        "sun_hot_start": (ci, [vp, u64, vp]),
+        "sun_heatbath": (ci, [vp, u64, u32, vp]),
+        "sun_overrelax": (ci, [vp, vp]),
+        "sun_heatbath_sub": (ci, [vp, u64, u32, ci, ci, vp]),
+        "sun_overrelax_sub": (ci, [vp, ci, ci, vp]),
+        "sun_reunitarize": (ci, [vp, vp]),
+        "sun_plaquette": (ci, [vp, vp, dp]),
+        "sun_polyakov": (ci, [vp, vp, dp]),
+        "sun_plaquette_async": (ci, [vp, vp, vp]),
+        "sun_polyakov_async": (ci, [vp, vp, vp]),
+        "sun_links_to_host": (ci, [vp, dp, ci]),
+        "sun_links_from_host": (ci, [vp, dp, ci]),
+        "sun_generate": (ci, [vp, ctypes.POINTER(GenerateDesc), vp, dp]),
+        "sun_batch_create": (ci, [vp, ci, ctypes.POINTER(vp), ctypes.POINTER(u64), ctypes.POINTER(vp)]),
+        "sun_batch_destroy": (ci, [vp]),
+        "sun_batch_heatbath": (ci, [vp, ctypes.POINTER(u32), vp]),
+        "sun_batch_overrelax": (ci, [vp, vp]),
+        "sun_batch_reunitarize": (ci, [vp, vp]),
+        "sun_batch_plaquette": (ci, [vp, vp, dp]),
+        "sun_batch_generate": (ci, [vp, u32, ci, ci, ci, ci, vp, dp]),
+        "sun_stream_create": (ci, [ctypes.POINTER(vp)]),
+        "sun_stream_destroy": (ci, [vp]),
+        "sun_stream_synchronize": (ci, [vp]),
+        "sun_event_create": (ci, [ctypes.POINTER(vp)]),
+        "sun_event_destroy": (ci, [vp]),
+        "sun_event_record": (ci, [vp, vp]),
+        "sun_event_elapsed_ms": (ci, [vp, vp, ctypes.POINTER(ctypes.c_float)]),
+        "sun_device_name": (ci, [vp, ctypes.c_char_p, ci]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().sun_last_error().decode(errors="replace")
+        raise SunError(f"{what} failed ({rc}): {msg}")
+
+
+def _dp(a: np.ndarray):
+    assert a.flags.c_contiguous and a.dtype in (np.float64, np.complex128)
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Context:
+    """initCULQCD / EndCULQCD (include/comm_mpi.h:43-45)."""
+
+    def __init__(self, device: int = 0, verbosity: int = 0, tune: int = 0):
+        self.h = ctypes.c_void_p()
+        _check(lib().sun_init(device, verbosity, tune, ctypes.byref(self.h)), "sun_init")
+
+    def close(self):
+        if self.h:
+            _check(lib().sun_finalize(self.h), "sun_finalize")
+            self.h = ctypes.c_void_p()
+
+    def device_name(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().sun_device_name(self.h, buf, 256), "sun_device_name")
+        return buf.value.decode()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def lattice_bytes(nc, dims, storage=STORE_18, precision=F64, beta=6.0) -> int:
+    d = LatticeDesc(nc, (ctypes.c_int * 4)(*dims), beta, storage, precision)
+    return int(lib().sun_lattice_bytes(ctypes.byref(d)))
+
+
+class Lattice:
+    """One gauge field resident in HBM (SETPARAMS + _gauge<Real>, per handle)."""
+
+    def __init__(self, ctx: Context, nc: int, dims, beta: float, storage: int = STORE_18,
+                 precision: int = F64):
+        self.ctx = ctx
+        self.nc, self.dims, self.beta = nc, tuple(int(d) for d in dims), float(beta)
+        self.storage, self.precision = storage, precision
+        self.V = int(np.prod(self.dims))
+        d = LatticeDesc(nc, (ctypes.c_int * 4)(*self.dims), beta, storage, precision)
+        self.h = ctypes.c_void_p()
+        _check(lib().sun_lattice_create(ctx.h, ctypes.byref(d), ctypes.byref(self.h)), "sun_lattice_create")
+
+    def close(self):
+        if self.h:
+            _check(lib().sun_lattice_destroy(self.h), "sun_lattice_destroy")
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- initialisation
+    def cold_start(self, stream=None):
+        _check(lib().sun_cold_start(self.h, stream), "sun_cold_start")
+
+    def hot_start(self, seed: int, stream=None):
+        _check(lib().sun_hot_start(self.h, seed, stream), "sun_hot_start")
+
+    # -- updates
+    def heatbath(self, seed: int, sweep: int, stream=None):
+        _check(lib().sun_heatbath(self.h, seed, sweep, stream), "sun_heatbath")
+
+    def overrelax(self, stream=None):
+        _check(lib().sun_overrelax(self.h, stream), "sun_overrelax")
+
+    def heatbath_sub(self, seed: int, sweep: int, parity: int, mu: int, stream=None):
+        _check(lib().sun_heatbath_sub(self.h, seed, sweep, parity, mu, stream), "sun_heatbath_sub")
+
+    def overrelax_sub(self, parity: int, mu: int, stream=None):
+        _check(lib().sun_overrelax_sub(self.h, parity, mu, stream), "sun_overrelax_sub")
+
+    def reunitarize(self, stream=None):
+        _check(lib().sun_reunitarize(self.h, stream), "sun_reunitarize")
+
+    # -- observables
+    def plaquette(self, stream=None) -> np.ndarray:
+        out = np.zeros(2)
+        _check(lib().sun_plaquette(self.h, stream, _dp(out)), "sun_plaquette")
+        return out
+
+    def polyakov(self, stream=None) -> complex:
+        out = np.zeros(2)
+        _check(lib().sun_polyakov(self.h, stream, _dp(out)), "sun_polyakov")
+        return complex(out[0], out[1])
+
+    # -- host exchange (canonical layout, complex128 [4, V, N, N])
+    def to_host(self) -> np.ndarray:
+        U = np.zeros((4, self.V, self.nc, self.nc), dtype=np.complex128)
+        _check(lib().sun_links_to_host(self.h, _dp(U), 0), "sun_links_to_host")
+        return U
+
+    def from_host(self, U: np.ndarray):
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        assert U.shape == (4, self.V, self.nc, self.nc)
+        _check(lib().sun_links_from_host(self.h, _dp(U), 0), "sun_links_from_host")
+
+    # -- generation driver
+    def generate(self, seed: int, n_iter: int, n_or: int = 4, reunit_every: int = 1, first_sweep: int = 0,
+                 measure: bool = True, use_graph: bool = True, stream=None) -> np.ndarray:
+        d = GenerateDesc(seed, first_sweep, n_iter, n_or, reunit_every, int(measure), int(use_graph))
+        obs = np.zeros((max(n_iter, 1), 4))
+        _check(lib().sun_generate(self.h, ctypes.byref(d), stream, _dp(obs) if measure else None),
+               "sun_generate")
+        return obs[:n_iter]
+
+
+class Batch:
+    """Many lattices of one (N, storage, precision), one launch per (parity, mu)."""
+
+    def __init__(self, ctx: Context, lattices, seeds):
+        self.ctx = ctx
+        self.lattices = list(lattices)
+        n = len(self.lattices)
+        arr = (ctypes.c_void_p * n)(*[l.h.value for l in self.lattices])
+        sd = (ctypes.c_uint64 * n)(*[int(s) for s in seeds])
+        self.h = ctypes.c_void_p()
+        _check(lib().sun_batch_create(ctx.h, n, arr, sd, ctypes.byref(self.h)), "sun_batch_create")
+
+    def close(self):
+        if self.h:
+            _check(lib().sun_batch_destroy(self.h), "sun_batch_destroy")
+            self.h = ctypes.c_void_p()
+
+    def heatbath(self, sweeps, stream=None):
+        n = len(self.lattices)
+        sw = (ctypes.c_uint32 * n)(*[int(s) for s in sweeps])
+        _check(lib().sun_batch_heatbath(self.h, sw, stream), "sun_batch_heatbath")
+
+    def overrelax(self, stream=None):
+        _check(lib().sun_batch_overrelax(self.h, stream), "sun_batch_overrelax")
+
+    def reunitarize(self, stream=None):
+        _check(lib().sun_batch_reunitarize(self.h, stream), "sun_batch_reunitarize")
+
+    def plaquette(self, stream=None) -> np.ndarray:
+        out = np.zeros((len(self.lattices), 2))
+        _check(lib().sun_batch_plaquette(self.h, stream, _dp(out)), "sun_batch_plaquette")
+        return out
+
+    def generate(self, first_sweep: int, n_iter: int, n_or: int = 4, reunit_every: int = 1,
+                 use_graph: bool = True, measure: bool = True, stream=None) -> np.ndarray:
+        n = len(self.lattices)
+        obs = np.zeros((max(n_iter, 1), n, 2))
+        _check(lib().sun_batch_generate(self.h, first_sweep, n_iter, n_or, reunit_every, int(use_graph), stream,
+                                        _dp(obs) if measure else None), "sun_batch_generate")
+        return obs[:n_iter]
+
+
+class Timer:
+    """HIP events on a given stream (sun_event_*), for timing without torch."""
+
+    def __init__(self):
+        self.a, self.b = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().sun_event_create(ctypes.byref(self.a)), "sun_event_create")
+        _check(lib().sun_event_create(ctypes.byref(self.b)), "sun_event_create")
+
+    def start(self, stream=None):
+        _check(lib().sun_event_record(self.a, stream), "sun_event_record")
+
+    def stop(self, stream=None):
+        _check(lib().sun_event_record(self.b, stream), "sun_event_record")
+
+    def ms(self) -> float:
+        v = ctypes.c_float()
+        _check(lib().sun_event_elapsed_ms(self.a, self.b, ctypes.byref(v)), "sun_event_elapsed_ms")
+        return float(v.value)
+
+    def close(self):
+        lib().sun_event_destroy(self.a)
+        lib().sun_event_destroy(self.b)
+
+
+def new_stream():
+    s = ctypes.c_void_p()
+    _check(lib().sun_stream_create(ctypes.byref(s)), "sun_stream_create")
+    return s
+
+
+def sync(stream):
+    _check(lib().sun_stream_synchronize(stream), "sun_stream_synchronize")

@@ -1,0 +1,101 @@
+// sun_common.hip -- variant-independent kernels: the fixed-order final reduction
+// of per-block partial sums (single lattice and batched).
+#include <hip/hip_runtime.h>
+
+#include "sun_internal.hpp"
+
+namespace sun {
+
+namespace {
+
+__device__ __forceinline__ double2 block_sum_all(double2 v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_down(v.x, off, 64);
+        v.y += __shfl_down(v.y, off, 64);
+    }
+    __shared__ double2 ws[kReduceBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) ws[w] = v;
+    __syncthreads();
+    double2 r = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < kReduceBlock / 64; i++) {
+            r.x += ws[i].x;
+            r.y += ws[i].y;
+        }
+    }
+    return r;
+}
+
+// thread t sums partials t, t+256, t+512, ... in order; then the block tree
+__device__ __forceinline__ double2 ordered_sum(const double2* p, int n) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < n; i += kReduceBlock) {
+        acc.x += p[i].x;
+        acc.y += p[i].y;
+    }
+    return block_sum_all(acc);
+}
+
+__global__ void __launch_bounds__(kReduceBlock) k_finalize(const double2* partials, int n, double s0, double s1,
+                                                           double2* out) {
+    double2 v = ordered_sum(partials, n);
+    if (threadIdx.x == 0) out[0] = make_double2(v.x * s0, v.y * s1);
+}
+
+__global__ void __launch_bounds__(kReduceBlock) k_batch_finalize(const double2* partials, const int* base,
+                                                                 const int* count, const double2* scales,
+                                                                 double2* out) {
+    const int i = blockIdx.x;
+    double2 v = ordered_sum(partials + base[i], count[i]);
+    if (threadIdx.x == 0) out[i] = make_double2(v.x * scales[i].x, v.y * scales[i].y);
+}
+
+__global__ void k_advance_counter(uint32_t* ctr) {
+    if (threadIdx.x == 0) ctr[0] += 1u;
+}
+
+__global__ void k_record_obs(const double2* src, int n, double2* history, const uint32_t* ctr, uint32_t first) {
+    const int i = threadIdx.x;
+    if (i < n) history[(size_t)(ctr[0] - first) * n + i] = src[i];
+}
+
+__global__ void k_batch_advance(BatchEntry* table, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) table[i].sweep += 1u;
+}
+
+}  // namespace
+
+hipError_t launch_advance_counter(uint32_t* ctr, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance_counter, dim3(1), dim3(64), 0, s, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_record_obs(const double2* src, int n, double2* history, const uint32_t* ctr, uint32_t first,
+                             hipStream_t s) {
+    if (n > 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_record_obs, dim3(1), dim3(((n + 63) / 64) * 64), 0, s, src, n, history, ctr, first);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(BatchEntry* table, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_advance, dim3((n + 255) / 256), dim3(256), 0, s, table, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double2* partials, int n, double s0, double s1, double2* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kReduceBlock), 0, s, partials, n, s0, s1, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finalize(const double2* partials, const int* base, const int* count, const double2* scales,
+                                 int nlat, double2* out, hipStream_t s) {
+    if (nlat <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_finalize, dim3(nlat), dim3(kReduceBlock), 0, s, partials, base, count, scales, out);
+    return hipGetLastError();
+}
+
+}  // namespace sun

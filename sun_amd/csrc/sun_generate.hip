@@ -1,0 +1,346 @@
+// sun_generate.hip -- the ensemble-generation driver for one lattice and for a
+// batch of lattices.  One iteration = heatbath sweep, n_or overrelaxation sweeps,
+// optional reunitarisation, plaquette (+ Polyakov loop for a single lattice).  With
+// use_graph the iteration is captured once into a hipGraph and replayed; the
+// heatbath sweep number and the observable slot live in device counters that the
+// graph itself advances, so every replay continues the Markov chain.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "sun_internal.hpp"
+
+using namespace sun;
+
+namespace {
+
+struct GraphGuard {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~GraphGuard() {
+        if (x) hipGraphExecDestroy(x);
+        if (g) hipGraphDestroy(g);
+    }
+};
+
+int ensure_obs(double2** buf, int* cap, int n) {
+    if (*cap >= n) return SUN_OK;
+    if (*buf) hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    HIP_TRY(hipMalloc(buf, sizeof(double2) * (size_t)n));
+    *cap = n;
+    return SUN_OK;
+}
+
+// one iteration of a single lattice; sweep taken from lat->sweep_dev
+int lattice_iteration(sun_lattice* lat, uint64_t seed, int n_or, bool reunit, bool measure, uint32_t first,
+                      hipStream_t s) {
+    SUN_TRY(heatbath_sweep(lat, seed, 0, lat->sweep_dev, s));
+    for (int k = 0; k < n_or; k++) SUN_TRY(overrelax_sweep(lat, s));
+    if (reunit) HIP_TRY(lat->ops->reunitarize(lat->f, 4 * lat->g.V, s));
+    if (measure) {
+        SUN_TRY(plaquette_to(lat, s, lat->dres));
+        SUN_TRY(polyakov_to(lat, s, lat->dres + 1));
+        HIP_TRY(launch_record_obs(lat->dres, 2, lat->obs_dev, lat->sweep_dev, first, s));
+    }
+    HIP_TRY(launch_advance_counter(lat->sweep_dev, s));
+    return SUN_OK;
+}
+
+int capture(hipStream_t s, GraphGuard& gg, const std::function<int()>& body) {
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = body();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != SUN_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(SUN_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    gg.g = g;
+    HIP_TRY(hipGraphInstantiate(&gg.x, g, nullptr, nullptr, 0));
+    return SUN_OK;
+}
+
+}  // namespace
+
+extern "C" int sun_generate(sun_lattice* lat, const sun_generate_desc* d, void* stream, double* obs) {
+    if (!lat || !d) return fail(SUN_ERR_INVALID, "sun_generate: NULL argument");
+    if (d->n_iter < 0 || d->n_or < 0 || d->reunit_every < 0)
+        return fail(SUN_ERR_INVALID, "sun_generate: negative count");
+    if (d->n_iter == 0) return SUN_OK;
+    hipStream_t s = pick_stream(lat, stream);
+    const bool measure = d->measure != 0;
+    if (measure) SUN_TRY(ensure_obs(&lat->obs_dev, &lat->obs_cap, 2 * d->n_iter));
+    const uint32_t first = d->first_sweep;
+    SUN_TRY(tune_lattice(lat, s));
+    HIP_TRY(hipMemcpyAsync(lat->sweep_dev, &first, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+
+    if (d->use_graph) {
+        GraphGuard plain, with_reunit;
+        const bool need_plain = d->reunit_every != 1;
+        const bool need_reunit = d->reunit_every >= 1;
+        if (need_plain)
+            SUN_TRY(capture(s, plain, [&] { return lattice_iteration(lat, d->seed, d->n_or, false, measure, first, s); }));
+        if (need_reunit)
+            SUN_TRY(capture(s, with_reunit,
+                            [&] { return lattice_iteration(lat, d->seed, d->n_or, true, measure, first, s); }));
+        for (int it = 0; it < d->n_iter; it++) {
+            const bool ru = d->reunit_every > 0 && ((it + 1) % d->reunit_every) == 0;
+            HIP_TRY(hipGraphLaunch(ru ? with_reunit.x : plain.x, s));
+        }
+    } else {
+        for (int it = 0; it < d->n_iter; it++) {
+            const bool ru = d->reunit_every > 0 && ((it + 1) % d->reunit_every) == 0;
+            SUN_TRY(lattice_iteration(lat, d->seed, d->n_or, ru, measure, first, s));
+        }
+    }
+    if (measure && obs) {
+        std::vector<double2> h((size_t)2 * d->n_iter);
+        HIP_TRY(hipMemcpyAsync(h.data(), lat->obs_dev, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int it = 0; it < d->n_iter; it++) {
+            obs[4 * it + 0] = h[2 * it].x;
+            obs[4 * it + 1] = h[2 * it].y;
+            obs[4 * it + 2] = h[2 * it + 1].x;
+            obs[4 * it + 3] = h[2 * it + 1].y;
+        }
+    } else {
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return SUN_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------------
+static void batch_free(sun_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->ctx->device);
+    if (b->t_update) hipFree(b->t_update);
+    if (b->t_reunit) hipFree(b->t_reunit);
+    if (b->t_plaq) hipFree(b->t_plaq);
+    if (b->partials) hipFree(b->partials);
+    if (b->d_base) hipFree(b->d_base);
+    if (b->d_count) hipFree(b->d_count);
+    if (b->d_scales) hipFree(b->d_scales);
+    if (b->d_out) hipFree(b->d_out);
+    if (b->h_out) hipHostFree(b->h_out);
+    if (b->obs_dev) hipFree(b->obs_dev);
+    if (b->iter_dev) hipFree(b->iter_dev);
+    delete b;
+}
+
+extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, const uint64_t* seeds,
+                                sun_batch** out) {
+    if (!ctx || !lats || !out || n <= 0 || n > 1024)
+        return fail(SUN_ERR_INVALID, "sun_batch_create: bad argument (n=%d, 1..1024)", n);
+    *out = nullptr;
+    const OpsTable* ops = lats[0] ? lats[0]->ops : nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!lats[i]) return fail(SUN_ERR_INVALID, "sun_batch_create: lattice %d is NULL", i);
+        if (lats[i]->ops != ops)
+            return fail(SUN_ERR_INVALID, "sun_batch_create: lattice %d differs in N/storage/precision", i);
+        if (lats[i]->ctx != ctx) return fail(SUN_ERR_INVALID, "sun_batch_create: lattice %d has another context", i);
+    }
+    sun_batch* b = new (std::nothrow) sun_batch();
+    if (!b) return fail(SUN_ERR_NOMEM, "host allocation");
+    b->ctx = ctx;
+    b->ops = ops;
+    b->lats.assign(lats, lats + n);
+    std::vector<BatchEntry> tu(n), tr(n), tp(n);
+    std::vector<int> base(n), count(n);
+    std::vector<double2> scales(n);
+    int cu = 0, cr = 0, cp = 0;
+    for (int i = 0; i < n; i++) {
+        sun_lattice* L = lats[i];
+        BatchEntry e;
+        memset(&e, 0, sizeof(e));
+        e.f = L->f;
+        e.g = L->g;
+        e.seed = seeds ? seeds[i] : 0;
+        e.bnc = L->d.beta / L->d.nc;
+        e.sweep = 0;
+        BatchEntry a = e, r = e, p = e;
+        a.first_block = cu;
+        a.nblocks = update_blocks(L->g, 256);
+        cu += a.nblocks;
+        r.first_block = cr;
+        r.nblocks = link_blocks(L->g);
+        cr += r.nblocks;
+        p.first_block = cp;
+        p.nblocks = plaquette_blocks(L->g);
+        p.partial_base = cp;
+        base[i] = cp;
+        count[i] = p.nblocks;
+        cp += p.nblocks;
+        const double norm = 1.0 / ((double)(3 * L->d.nc) * (double)L->g.V);
+        scales[i] = make_double2(norm, norm);
+        tu[i] = a;
+        tr[i] = r;
+        tp[i] = p;
+    }
+    b->blocks_update = cu;
+    b->blocks_reunit = cr;
+    b->blocks_plaq = cp;
+    b->h_update = tu;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipMalloc(&b->t_update, sizeof(BatchEntry) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->t_reunit, sizeof(BatchEntry) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->t_plaq, sizeof(BatchEntry) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double2) * (size_t)cp);
+    if (e == hipSuccess) e = hipMalloc(&b->d_base, sizeof(int) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_count, sizeof(int) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_scales, sizeof(double2) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_out, sizeof(double2) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->iter_dev, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc(&b->h_out, sizeof(double2) * n, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemcpy(b->t_update, tu.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->t_reunit, tr.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->t_plaq, tp.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_base, base.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_count, count.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_scales, scales.data(), sizeof(double2) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipGetLastError();
+        batch_free(b);
+        return fail(SUN_ERR_NOMEM, "sun_batch_create: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return SUN_OK;
+}
+
+extern "C" int sun_batch_destroy(sun_batch* b) {
+    if (!b) return fail(SUN_ERR_INVALID, "sun_batch_destroy: NULL");
+    hipSetDevice(b->ctx->device);
+    hipDeviceSynchronize();
+    batch_free(b);
+    return SUN_OK;
+}
+
+static hipStream_t batch_stream(sun_batch* b, void* stream) {
+    return stream ? (hipStream_t)stream : b->ctx->stream;
+}
+
+static int batch_set_sweeps(sun_batch* b, const uint32_t* sweeps, uint32_t uniform, hipStream_t s) {
+    const int n = (int)b->lats.size();
+    for (int i = 0; i < n; i++) b->h_update[i].sweep = sweeps ? sweeps[i] : uniform;
+    HIP_TRY(hipMemcpyAsync(b->t_update, b->h_update.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SUN_OK;
+}
+
+static int batch_hb(sun_batch* b, hipStream_t s) {
+    const int n = (int)b->lats.size();
+    for (int parity = 0; parity < 2; parity++)
+        for (int mu = 0; mu < 4; mu++) HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, true, s));
+    return SUN_OK;
+}
+
+static int batch_or(sun_batch* b, hipStream_t s) {
+    const int n = (int)b->lats.size();
+    for (int parity = 0; parity < 2; parity++)
+        for (int mu = 0; mu < 4; mu++)
+            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, false, s));
+    return SUN_OK;
+}
+
+static int batch_plaq(sun_batch* b, hipStream_t s) {
+    const int n = (int)b->lats.size();
+    HIP_TRY(b->ops->batch_plaquette_partial(b->t_plaq, n, b->blocks_plaq, b->partials, s));
+    HIP_TRY(launch_batch_finalize(b->partials, b->d_base, b->d_count, b->d_scales, n, b->d_out, s));
+    return SUN_OK;
+}
+
+extern "C" int sun_batch_heatbath(sun_batch* b, const uint32_t* sweeps, void* stream) {
+    if (!b || !sweeps) return fail(SUN_ERR_INVALID, "sun_batch_heatbath: NULL argument");
+    hipStream_t s = batch_stream(b, stream);
+    SUN_TRY(batch_set_sweeps(b, sweeps, 0, s));
+    return batch_hb(b, s);
+}
+
+extern "C" int sun_batch_overrelax(sun_batch* b, void* stream) {
+    if (!b) return fail(SUN_ERR_INVALID, "sun_batch_overrelax: NULL");
+    return batch_or(b, batch_stream(b, stream));
+}
+
+extern "C" int sun_batch_reunitarize(sun_batch* b, void* stream) {
+    if (!b) return fail(SUN_ERR_INVALID, "sun_batch_reunitarize: NULL");
+    HIP_TRY(b->ops->batch_reunitarize(b->t_reunit, (int)b->lats.size(), b->blocks_reunit, batch_stream(b, stream)));
+    return SUN_OK;
+}
+
+extern "C" int sun_batch_plaquette(sun_batch* b, void* stream, double* out) {
+    if (!b || !out) return fail(SUN_ERR_INVALID, "sun_batch_plaquette: NULL");
+    hipStream_t s = batch_stream(b, stream);
+    const int n = (int)b->lats.size();
+    SUN_TRY(batch_plaq(b, s));
+    HIP_TRY(hipMemcpyAsync(b->h_out, b->d_out, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++) {
+        out[2 * i] = b->h_out[i].x;
+        out[2 * i + 1] = b->h_out[i].y;
+    }
+    return SUN_OK;
+}
+
+static int batch_iteration(sun_batch* b, int n_or, bool reunit, bool measure, hipStream_t s) {
+    const int n = (int)b->lats.size();
+    SUN_TRY(batch_hb(b, s));
+    for (int k = 0; k < n_or; k++) SUN_TRY(batch_or(b, s));
+    if (reunit) HIP_TRY(b->ops->batch_reunitarize(b->t_reunit, n, b->blocks_reunit, s));
+    if (measure) {
+        SUN_TRY(batch_plaq(b, s));
+        HIP_TRY(launch_record_obs(b->d_out, n, b->obs_dev, b->iter_dev, 0, s));
+    }
+    HIP_TRY(launch_batch_advance(b->t_update, n, s));
+    HIP_TRY(launch_advance_counter(b->iter_dev, s));
+    return SUN_OK;
+}
+
+extern "C" int sun_batch_generate(sun_batch* b, uint32_t first_sweep, int n_iter, int n_or, int reunit_every,
+                                  int use_graph, void* stream, double* obs) {
+    if (!b || n_iter < 0 || n_or < 0 || reunit_every < 0) return fail(SUN_ERR_INVALID, "sun_batch_generate: bad argument");
+    if (n_iter == 0) return SUN_OK;
+    hipStream_t s = batch_stream(b, stream);
+    const int n = (int)b->lats.size();
+    const bool measure = obs != nullptr;
+    if (measure) SUN_TRY(ensure_obs(&b->obs_dev, &b->obs_cap, n * n_iter));
+    SUN_TRY(batch_set_sweeps(b, nullptr, first_sweep, s));
+    const uint32_t zero = 0;
+    HIP_TRY(hipMemcpyAsync(b->iter_dev, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (use_graph) {
+        GraphGuard plain, with_reunit;
+        if (reunit_every != 1) SUN_TRY(capture(s, plain, [&] { return batch_iteration(b, n_or, false, measure, s); }));
+        if (reunit_every >= 1)
+            SUN_TRY(capture(s, with_reunit, [&] { return batch_iteration(b, n_or, true, measure, s); }));
+        for (int it = 0; it < n_iter; it++) {
+            const bool ru = reunit_every > 0 && ((it + 1) % reunit_every) == 0;
+            HIP_TRY(hipGraphLaunch(ru ? with_reunit.x : plain.x, s));
+        }
+    } else {
+        for (int it = 0; it < n_iter; it++) {
+            const bool ru = reunit_every > 0 && ((it + 1) % reunit_every) == 0;
+            SUN_TRY(batch_iteration(b, n_or, ru, measure, s));
+        }
+    }
+    // keep the host copy of the sweep counters in step with the device table
+    for (int i = 0; i < n; i++) b->h_update[i].sweep = first_sweep + (uint32_t)n_iter;
+    if (measure) {
+        std::vector<double2> h((size_t)n * n_iter);
+        HIP_TRY(hipMemcpyAsync(h.data(), b->obs_dev, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (size_t k = 0; k < h.size(); k++) {
+            obs[2 * k] = h[k].x;
+            obs[2 * k + 1] = h[k].y;
+        }
+    } else {
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return SUN_OK;
+}

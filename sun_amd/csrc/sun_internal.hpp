@@ -1,0 +1,136 @@
+// sun_internal.hpp -- handle structures and helpers shared by the C-ABI sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "sun.h"
+#include "sun_ops.hpp"
+
+namespace sun {
+
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(x)                                                                                 \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) {                                                                    \
+            hipGetLastError();                                                                     \
+            return ::sun::fail(SUN_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #x,                \
+                               hipGetErrorString(e_));                                             \
+        }                                                                                          \
+    } while (0)
+
+#define HIP_TRY_CLEANUP(x, cleanup)                                                                \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) {                                                                    \
+            hipGetLastError();                                                                     \
+            cleanup;                                                                               \
+            return ::sun::fail(SUN_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #x,                \
+                               hipGetErrorString(e_));                                             \
+        }                                                                                          \
+    } while (0)
+
+#define SUN_TRY(x)              \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_ != SUN_OK) return rc_; \
+    } while (0)
+
+// Launch-shape autotuner (replaces the reference's tune.cpp:158-414): per
+// (kernel, N, storage, precision, volume, device) the fastest block size, found by
+// timing the real kernel on a scratch copy of the field, cached in memory and in
+// $SUN_RESOURCE_PATH/sun_tunecache.json.
+class Tuner {
+   public:
+    void load(int mode);
+    void save();
+    LaunchCfg lookup(const char* kernel, const sun_lattice_desc& d);
+    void record(const char* kernel, const sun_lattice_desc& d, const LaunchCfg& c, float ms);
+    bool enabled() const { return mode_ != 0; }
+    std::string arch;
+
+   private:
+    std::string key(const char* kernel, const sun_lattice_desc& d) const;
+    std::string path() const;
+    int mode_ = 0;
+    bool dirty_ = false;
+    std::map<std::string, std::pair<LaunchCfg, float>> cache_;
+    std::mutex mu_;
+};
+
+}  // namespace sun
+
+struct sun_ctx {
+    int device = 0;
+    int verbosity = 0;
+    int tune = 0;
+    std::string name, gcn_arch;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    sun::Tuner tuner;
+};
+
+struct sun_lattice {
+    sun_ctx* ctx = nullptr;
+    sun_lattice_desc d{};
+    const sun::OpsTable* ops = nullptr;
+    sun::Geom g{};
+    sun::Field f{};
+    void* buf = nullptr;
+    size_t bytes = 0;
+    double2* partials = nullptr;   // per-block reduction scratch
+    int npart = 0;
+    double2* dres = nullptr;       // device results: [0] plaquette/polyakov scratch
+    double2* hres = nullptr;       // pinned host copy
+    uint32_t* sweep_dev = nullptr; // device sweep counter for graph replay
+    double2* obs_dev = nullptr;    // device observable history of sun_generate
+    int obs_cap = 0;
+    sun::LaunchCfg cfg_hb, cfg_or;
+    bool tuned = false;
+};
+
+struct sun_batch {
+    sun_ctx* ctx = nullptr;
+    std::vector<sun_lattice*> lats;
+    const sun::OpsTable* ops = nullptr;
+    // device tables: update blocks (half/256), reunit blocks (4V/256), plaquette blocks (V/256)
+    sun::BatchEntry* t_update = nullptr;
+    sun::BatchEntry* t_reunit = nullptr;
+    sun::BatchEntry* t_plaq = nullptr;
+    int blocks_update = 0, blocks_reunit = 0, blocks_plaq = 0;
+    std::vector<sun::BatchEntry> h_update;
+    double2* partials = nullptr;
+    int* d_base = nullptr;
+    int* d_count = nullptr;
+    double2* d_scales = nullptr;
+    double2* d_out = nullptr;
+    double2* h_out = nullptr;
+    double2* obs_dev = nullptr;
+    int obs_cap = 0;
+    uint32_t* iter_dev = nullptr;
+};
+
+namespace sun {
+int check_desc(const sun_lattice_desc* d);
+size_t link_bytes(const sun_lattice_desc* d);
+int lattice_alloc(sun_ctx* ctx, const sun_lattice_desc* d, sun_lattice** out);
+void lattice_free(sun_lattice* L);
+hipStream_t pick_stream(sun_lattice* lat, void* stream);
+int heatbath_sweep(sun_lattice* lat, uint64_t seed, uint32_t sweep, const uint32_t* sweep_dev, hipStream_t s);
+int overrelax_sweep(sun_lattice* lat, hipStream_t s);
+int plaquette_to(sun_lattice* lat, hipStream_t s, double2* dev_out);
+int polyakov_to(sun_lattice* lat, hipStream_t s, double2* dev_out);
+int tune_lattice(sun_lattice* lat, hipStream_t s);
+// small helper kernels (sun_common.hip)
+hipError_t launch_advance_counter(uint32_t* ctr, hipStream_t s);
+hipError_t launch_record_obs(const double2* src, int n, double2* history, const uint32_t* ctr, uint32_t first,
+                             hipStream_t s);
+hipError_t launch_batch_advance(BatchEntry* table, int n, hipStream_t s);
+}  // namespace sun

@@ -1,0 +1,92 @@
+// sun_ops.hpp -- the per-variant launcher table shared by the kernel translation
+// units and the C-ABI layer.  One table exists for every (N, storage, precision)
+// combination the engine supports; the C-ABI picks the table of a lattice once at
+// creation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sun_field.hpp"
+
+namespace sun {
+
+// Launch shape chosen by the autotuner (threads per block, and sites per thread
+// along the fastest lattice direction for the tiled update kernel).
+struct LaunchCfg {
+    int block = 256;
+};
+
+// One lattice of a batched launch.  `first_block` is the prefix sum of the blocks
+// of the preceding lattices; a block finds its lattice by binary search.
+struct BatchEntry {
+    Field f;
+    Geom g;
+    uint64_t seed;
+    double bnc;          // beta / N
+    uint32_t sweep;      // heatbath sweep counter of this lattice
+    int first_block;
+    int nblocks;
+    int partial_base;    // first slot of this lattice in the reduction scratch
+};
+
+struct OpsTable {
+    int nc, storage, real_bytes;
+    // one (parity, mu) heatbath (hb=true) or overrelaxation launch
+    // sweep_dev, when not null, overrides sweep with a device-resident counter (graph replay)
+    hipError_t (*update)(const Field&, const Geom&, int parity, int mu, bool hb, uint32_t sweep,
+                         const uint32_t* sweep_dev, double bnc, uint64_t seed, const LaunchCfg&, hipStream_t);
+    hipError_t (*cold_start)(const Field&, int nlinks, hipStream_t);
+    hipError_t (*hot_start)(const Field&, const Geom&, uint64_t seed, hipStream_t);
+    hipError_t (*reunitarize)(const Field&, int nlinks, hipStream_t);
+    // per-block partial sums (double2) of the plaquette (spatial, temporal) and of
+    // the Polyakov loop (re, im); the number of partials is *_blocks(geom)
+    hipError_t (*plaquette_partial)(const Field&, const Geom&, double2* partials, hipStream_t);
+    hipError_t (*polyakov_partial)(const Field&, const Geom&, double2* partials, hipStream_t);
+    hipError_t (*to_canonical)(const Field&, int nlinks, double* out, hipStream_t);
+    hipError_t (*from_canonical)(const Field&, int nlinks, const double* in, hipStream_t);
+    // batched: every lattice of the table, one (parity, mu) launch
+    hipError_t (*batch_update)(const BatchEntry* dev_table, int nlat, int total_blocks, int parity,
+                               int mu, bool hb, hipStream_t);
+    hipError_t (*batch_reunitarize)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
+    hipError_t (*batch_plaquette_partial)(const BatchEntry* dev_table, int nlat, int total_blocks,
+                                          double2* partials, hipStream_t);
+};
+
+constexpr int kReduceBlock = 256;   // threads of every reduction block
+
+inline int plaquette_blocks(const Geom& g) { return (g.V + kReduceBlock - 1) / kReduceBlock; }
+inline int polyakov_blocks(const Geom& g) {
+    int ts = g.X[0] * g.X[1] * g.X[2];
+    return (ts + kReduceBlock - 1) / kReduceBlock;
+}
+inline int update_blocks(const Geom& g, int block) { return (g.half + block - 1) / block; }
+inline int link_blocks(const Geom& g) { return (4 * g.V + kReduceBlock - 1) / kReduceBlock; }
+
+// sum of partials[0..n) in a fixed order, times `scale`, into out (device)
+hipError_t launch_finalize(const double2* partials, int n, double scale0, double scale1, double2* out,
+                           hipStream_t s);
+// finalize for every lattice of a batch: out[i] = sum of its partials * scales[i]
+hipError_t launch_batch_finalize(const double2* partials, const int* base, const int* count,
+                                 const double2* scales, int nlat, double2* out, hipStream_t s);
+
+const OpsTable* ops_n2_f18_r4();
+const OpsTable* ops_n2_f18_r8();
+const OpsTable* ops_n3_f18_r4();
+const OpsTable* ops_n3_f18_r8();
+const OpsTable* ops_n3_f12_r4();
+const OpsTable* ops_n3_f12_r8();
+const OpsTable* ops_n3_f8_r4();
+const OpsTable* ops_n3_f8_r8();
+const OpsTable* ops_n4_f18_r4();
+const OpsTable* ops_n4_f18_r8();
+
+inline const OpsTable* find_ops(int nc, int storage, int real_bytes) {
+    const OpsTable* all[] = {ops_n2_f18_r4(), ops_n2_f18_r8(), ops_n3_f18_r4(), ops_n3_f18_r8(),
+                             ops_n3_f12_r4(), ops_n3_f12_r8(), ops_n3_f8_r4(),  ops_n3_f8_r8(),
+                             ops_n4_f18_r4(), ops_n4_f18_r8()};
+    for (const OpsTable* t : all)
+        if (t->nc == nc && t->storage == storage && t->real_bytes == real_bytes) return t;
+    return nullptr;
+}
+
+}  // namespace sun

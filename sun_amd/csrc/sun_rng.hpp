@@ -1,0 +1,127 @@
+// sun_rng.hpp -- counter-based RNG and the SU(2) subgroup sampler (host + device).
+//
+// The reference keeps one cuRAND MRG32k3a state per half-volume site
+// (random.h:16-22, random.cu:32-136) and streams it through global memory on every
+// heatbath launch (monte.cu:82,86).  Here every draw is a pure function of
+// (seed, site, sweep, parity, mu, draw index) -- Philox4x32-10 (Salmon, Moraes,
+// Dror, Shaw, SC'11) -- so there is no RNG state in HBM at all and any lattice can
+// be replayed bit-for-bit from (seed, sweep).
+#pragma once
+
+#include "sun_math.hpp"
+
+namespace sun {
+
+enum : uint32_t {
+    kTagHeatbath = 0x10000000u,
+    kTagHotStart = 0x20000000u,
+    kTagTest = 0x30000000u,
+};
+
+struct Philox {
+    SUN_HD static void round10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+                               uint32_t k0, uint32_t k1) {
+#pragma unroll
+        for (int r = 0; r < 10; r++) {
+            uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+            uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+            uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+            uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+            uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+            c0 = n0;
+            c1 = lo1;
+            c2 = n2;
+            c3 = lo0;
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+    }
+};
+
+// Uniform double in (0,1]: ((hi<<32|lo)>>11)+1) * 2^-53
+SUN_HD double u01(uint32_t lo, uint32_t hi) {
+    uint64_t m = ((((uint64_t)hi) << 32) | lo) >> 11;
+    return (double)(m + 1) * 0x1p-53;
+}
+
+// Stream of uniforms for one (key, c0, c2, c3); counter word 1 is the draw index.
+// Each Philox call yields two doubles.
+struct Stream {
+    uint32_t k0, k1, c0, c2, c3, j;
+    double cache;
+    bool has;
+
+    SUN_HD Stream(uint64_t seed, uint32_t c0_, uint32_t c2_, uint32_t c3_)
+        : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), c0(c0_), c2(c2_), c3(c3_), j(0),
+          cache(0.0), has(false) {}
+
+    SUN_HD double next() {
+        if (has) {
+            has = false;
+            return cache;
+        }
+        uint32_t a = c0, b = j, c = c2, d = c3;
+        Philox::round10(a, b, c, d, k0, k1);
+        j++;
+        cache = u01(c, d);
+        has = true;
+        return u01(a, b);
+    }
+};
+
+// SU(2) element a = (a0,a1,a2,a3) with a0 ~ sqrt(1-a0^2) exp(al*a0): the MILC
+// Kennedy-Pendleton generator with the Creutz fallback for al <= 2, restating
+// device_PHB_OVR.h:162-223 (including the 1e-10 guard and the 20-try caps).  The
+// reference's double literals make this arithmetic double even for Real=float;
+// here it is double for every storage precision.
+SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
+    double xr1, xr2, xr3, xr4, d, r;
+    double a0 = 0.0;
+    xr1 = st.next();
+    xr1 = log(xr1 + 1.e-10);
+    xr2 = st.next();
+    xr2 = log(xr2 + 1.e-10);
+    xr3 = st.next();
+    xr4 = st.next();
+    xr3 = cos(kTwoPi * xr3);
+    d = -(xr2 + xr1 * xr3 * xr3) / al;
+    bool accepted = (1.00 - 0.5 * d) > xr4 * xr4;
+    if (!accepted && al > 2.0) {
+        for (int k = 0; k < 20; k++) {
+            xr1 = st.next();
+            xr1 = log(xr1 + 1.e-10);
+            xr2 = st.next();
+            xr2 = log(xr2 + 1.e-10);
+            xr3 = st.next();
+            xr4 = st.next();
+            xr3 = cos(kTwoPi * xr3);
+            d = -(xr2 + xr1 * xr3 * xr3) / al;
+            if ((1.00 - 0.5 * d) > xr4 * xr4) break;
+        }
+    }
+    if (!accepted && al <= 2.0) {
+        xr3 = exp(-2.0 * al);
+        xr4 = 1.0 - xr3;
+        for (int k = 0; k < 20; k++) {
+            xr1 = st.next();
+            xr2 = st.next();
+            r = xr3 + xr4 * xr1;
+            a0 = 1.00 + log(r) / al;
+            if ((1.0 - a0 * a0) > xr2 * xr2) break;
+        }
+        d = 1.0 - a0;
+    }
+    a[0] = 1.0 - d;
+    xr3 = 1.0 - a[0] * a[0];
+    xr3 = fabs(xr3);
+    r = sqrt(xr3);
+    a[3] = (2.0 * st.next() - 1.0) * r;
+    xr1 = xr3 - a[3] * a[3];
+    xr1 = fabs(xr1);
+    xr1 = sqrt(xr1);
+    xr2 = kTwoPi * st.next();
+    a[1] = xr1 * cos(xr2);
+    a[2] = xr1 * sin(xr2);
+}
+
+}  // namespace sun

@@ -1,0 +1,141 @@
+// sun_tune.hip -- launch-shape autotuner.  Replaces the reference's QUDA-derived
+// tuner (tune.h:119-318, tune.cpp:158-414): for each (kernel, N, storage,
+// precision, lattice extents, device arch) it times the real update kernel at every
+// candidate block size on a scratch copy of the field (the reference's
+// preTune/postTune Backup/Restore, monte.h:73-74), keeps the fastest, and persists
+// the table in $SUN_RESOURCE_PATH/sun_tunecache.json (the reference's
+// $CULQCD_RESOURCE_PATH/tunecache.csv).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "sun_internal.hpp"
+
+namespace sun {
+
+static const int kCandidates[] = {64, 128, 192, 256};
+
+std::string Tuner::key(const char* kernel, const sun_lattice_desc& d) const {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%s|N%d|S%d|P%d|%dx%dx%dx%d|%s", kernel, d.nc, d.storage, d.precision, d.dims[0],
+             d.dims[1], d.dims[2], d.dims[3], arch.c_str());
+    return buf;
+}
+
+std::string Tuner::path() const {
+    const char* dir = getenv("SUN_RESOURCE_PATH");
+    if (!dir || !*dir) return std::string();
+    return std::string(dir) + "/sun_tunecache.json";
+}
+
+// The cache file is a JSON object {"key": [block, ms], ...}, one entry per line.
+void Tuner::load(int mode) {
+    std::lock_guard<std::mutex> lk(mu_);
+    mode_ = mode;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
+    std::string p = path();
+    if (p.empty()) return;
+    std::ifstream in(p);
+    std::string line;
+    while (std::getline(in, line)) {
+        size_t q0 = line.find('"');
+        size_t q1 = q0 == std::string::npos ? q0 : line.find('"', q0 + 1);
+        size_t b0 = line.find('[');
+        if (q1 == std::string::npos || b0 == std::string::npos) continue;
+        LaunchCfg c;
+        float ms = 0.f;
+        if (sscanf(line.c_str() + b0, "[%d, %f]", &c.block, &ms) != 2) continue;
+        bool ok = false;
+        for (int k : kCandidates) ok |= (k == c.block);
+        if (ok) cache_[line.substr(q0 + 1, q1 - q0 - 1)] = {c, ms};
+    }
+}
+
+void Tuner::save() {
+    std::lock_guard<std::mutex> lk(mu_);
+    std::string p = path();
+    if (p.empty() || !dirty_) return;
+    std::ofstream out(p);
+    out << "{\n";
+    size_t i = 0;
+    for (auto& kv : cache_) {
+        out << "  \"" << kv.first << "\": [" << kv.second.first.block << ", " << kv.second.second << "]"
+            << (++i < cache_.size() ? ",\n" : "\n");
+    }
+    out << "}\n";
+    dirty_ = false;
+}
+
+LaunchCfg Tuner::lookup(const char* kernel, const sun_lattice_desc& d) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = cache_.find(key(kernel, d));
+    if (it != cache_.end()) return it->second.first;
+    return LaunchCfg();
+}
+
+void Tuner::record(const char* kernel, const sun_lattice_desc& d, const LaunchCfg& c, float ms) {
+    std::lock_guard<std::mutex> lk(mu_);
+    cache_[key(kernel, d)] = {c, ms};
+    dirty_ = true;
+}
+
+// Times one (parity 0, mu 0) launch of the heatbath or overrelaxation kernel for
+// each candidate block size and stores the fastest in lat->cfg_hb / cfg_or.
+int tune_lattice(sun_lattice* lat, hipStream_t s) {
+    sun_ctx* ctx = lat->ctx;
+    if (!ctx->tuner.enabled() || lat->tuned) return SUN_OK;
+    void* backup = nullptr;
+    HIP_TRY(hipMalloc(&backup, lat->bytes));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SUN_OK;
+    hipError_t e = hipMemcpyAsync(backup, lat->buf, lat->bytes, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const double bnc = lat->d.beta / lat->d.nc;
+    for (int which = 0; which < 2 && e == hipSuccess; which++) {
+        const bool hb = which == 0;
+        LaunchCfg best;
+        float best_ms = 1e30f;
+        for (int cand : kCandidates) {
+            LaunchCfg c;
+            c.block = cand;
+            // warm-up, then 3 timed launches
+            e = lat->ops->update(lat->f, lat->g, 0, 0, hb, 0, nullptr, bnc, 1, c, s);
+            if (e == hipSuccess) e = hipEventRecord(e0, s);
+            for (int r = 0; r < 3 && e == hipSuccess; r++)
+                e = lat->ops->update(lat->f, lat->g, 0, 0, hb, 0, nullptr, bnc, 1, c, s);
+            if (e == hipSuccess) e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) break;
+            ms /= 3.f;
+            if (ctx->verbosity >= SUN_VERBOSE)
+                fprintf(stderr, "[sun tune] %s block=%d %.4f ms\n", hb ? "heatbath" : "overrelax", cand, ms);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        if (e == hipSuccess) {
+            (hb ? lat->cfg_hb : lat->cfg_or) = best;
+            ctx->tuner.record(hb ? "heatbath" : "overrelax", lat->d, best, best_ms);
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(lat->buf, backup, lat->bytes, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = fail(SUN_ERR_HIP, "autotuner: %s", hipGetErrorString(e));
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(backup);
+    if (rc == SUN_OK) lat->tuned = true;
+    return rc;
+}
+
+}  // namespace sun

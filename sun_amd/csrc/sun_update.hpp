@@ -1,0 +1,238 @@
+// sun_update.hpp -- single-link Cabibbo-Marinari updates (host + device).
+//
+// heatbath_link restates heatBathSUN (device_PHB_OVR.h:361-485) and overrelax_link
+// restates overrelaxationSUN (device_PHB_OVR.h:493-608): N=3 uses the reference's
+// block-projection fast path, other N the M = U*F path.  The link, the staple and
+// the SU(2) algebra run in the storage precision R; the sampler runs in double.
+#pragma once
+
+#include "sun_rng.hpp"
+
+namespace sun {
+
+template <typename R>
+struct Quat {
+    R a[4];
+};
+
+// v * u^dagger (msu2.h:211-219)
+template <typename R>
+SUN_HD Quat<R> mul_uv_dagger(const Quat<R>& v, const Quat<R>& u) {
+    Quat<R> b;
+    b.a[0] = v.a[0] * u.a[0] + v.a[1] * u.a[1] + v.a[2] * u.a[2] + v.a[3] * u.a[3];
+    b.a[1] = v.a[1] * u.a[0] - v.a[0] * u.a[1] + v.a[2] * u.a[3] - v.a[3] * u.a[2];
+    b.a[2] = v.a[2] * u.a[0] - v.a[0] * u.a[2] + v.a[3] * u.a[1] - v.a[1] * u.a[3];
+    b.a[3] = v.a[3] * u.a[0] - v.a[0] * u.a[3] + v.a[1] * u.a[2] - v.a[2] * u.a[1];
+    return b;
+}
+
+template <typename R>
+SUN_HD R qabs(const Quat<R>& r) {
+    return sqrt(r.a[0] * r.a[0] + r.a[1] * r.a[1] + r.a[2] * r.a[2] + r.a[3] * r.a[3]);
+}
+
+// msu2.h:148-155
+template <typename R>
+SUN_HD void conj_normalize(Quat<R>& r) {
+    R norm = R(1) / sqrt(r.a[0] * r.a[0] + r.a[1] * r.a[1] + r.a[2] * r.a[2] + r.a[3] * r.a[3]);
+    r.a[0] *= norm;
+    r.a[1] *= -norm;
+    r.a[2] *= -norm;
+    r.a[3] *= -norm;
+}
+
+// rows p,q of U times columns p,q of F, projected on SU(2) (device_PHB_OVR.h:385-400)
+template <typename R, int N>
+SUN_HD Quat<R> block_projection(const Mat<R, N>& U, const Mat<R, N>& F, int p, int q) {
+    cplx<R> a0 = mk<R>(0, 0), a1 = mk<R>(0, 0), a2 = mk<R>(0, 0), a3 = mk<R>(0, 0);
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        a0 = a0 + U.e[p][j] * F.e[j][p];
+        a1 = a1 + U.e[p][j] * F.e[j][q];
+        a2 = a2 + U.e[q][j] * F.e[j][p];
+        a3 = a3 + U.e[q][j] * F.e[j][q];
+    }
+    Quat<R> r;
+    r.a[0] = a0.re + a3.re;
+    r.a[1] = a1.im + a2.im;
+    r.a[2] = a1.re - a2.re;
+    r.a[3] = a0.im - a3.im;
+    return r;
+}
+
+// device_PHB_OVR.h:265-272
+template <typename R, int N>
+SUN_HD Quat<R> get_block_su2(const Mat<R, N>& m, int x, int y) {
+    Quat<R> r;
+    r.a[0] = m.e[x][x].re + m.e[y][y].re;
+    r.a[1] = m.e[x][y].im + m.e[y][x].im;
+    r.a[2] = m.e[x][y].re - m.e[y][x].re;
+    r.a[3] = m.e[x][x].im - m.e[y][y].im;
+    return r;
+}
+
+// link <- u * link on rows (x,y) (device_PHB_OVR.h:296-303)
+template <typename R, int N>
+SUN_HD void mul_block(const Quat<R>& u, Mat<R, N>& link, int x, int y) {
+    cplx<R> a00 = mk<R>(u.a[0], u.a[3]), a01 = mk<R>(u.a[2], u.a[1]);
+    cplx<R> a10 = mk<R>(-u.a[2], u.a[1]), a11 = mk<R>(u.a[0], -u.a[3]);
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        cplx<R> tmp = a00 * link.e[x][j] + a01 * link.e[y][j];
+        link.e[y][j] = a10 * link.e[x][j] + a11 * link.e[y][j];
+        link.e[x][j] = tmp;
+    }
+}
+
+template <typename R, int N>
+SUN_HD void heatbath_link(Mat<R, N>& U, const Mat<R, N>& F, double beta_over_nc, Stream& st) {
+    if constexpr (N == 3) {
+#pragma unroll
+        for (int block = 0; block < 3; block++) {
+            const int p = block_p(3, block), q = block_q(3, block);
+            Quat<R> r = block_projection<R, 3>(U, F, p, q);
+            R k = qabs(r);
+            R ap = R(beta_over_nc) * k;
+            k = R(1) / k;
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.a[i] *= k;
+            double ad[4];
+            generate_su2_milc((double)ap, st, ad);
+            Quat<R> a;
+#pragma unroll
+            for (int i = 0; i < 4; i++) a.a[i] = R(ad[i]);
+            r = mul_uv_dagger(a, r);
+            cplx<R> b0 = mk<R>(r.a[0], r.a[3]), b1 = mk<R>(r.a[2], r.a[1]);
+            cplx<R> b2 = mk<R>(-r.a[2], r.a[1]), b3 = mk<R>(r.a[0], -r.a[3]);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                cplx<R> tmp0 = b0 * U.e[p][j] + b1 * U.e[q][j];
+                U.e[q][j] = b2 * U.e[p][j] + b3 * U.e[q][j];
+                U.e[p][j] = tmp0;
+            }
+        }
+    } else {
+        Mat<R, N> M = U * F;
+#pragma unroll
+        for (int block = 0; block < N * (N - 1) / 2; block++) {
+            const int x = block_p(N, block), y = block_q(N, block);
+            Quat<R> r = get_block_su2<R, N>(M, x, y);
+            R k = qabs(r);
+            R ap = R(beta_over_nc) * k;
+            k = R(1) / k;
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.a[i] *= k;
+            double ad[4];
+            generate_su2_milc((double)ap, st, ad);
+            Quat<R> a;
+#pragma unroll
+            for (int i = 0; i < 4; i++) a.a[i] = R(ad[i]);
+            Quat<R> rr = mul_uv_dagger(a, r);
+            mul_block<R, N>(rr, U, x, y);
+            mul_block<R, N>(rr, M, x, y);
+        }
+    }
+}
+
+template <typename R, int N>
+SUN_HD void overrelax_link(Mat<R, N>& U, const Mat<R, N>& F) {
+    if constexpr (N == 3) {
+#pragma unroll
+        for (int block = 0; block < 3; block++) {
+            const int p = block_p(3, block), q = block_q(3, block);
+            Quat<R> r = block_projection<R, 3>(U, F, p, q);
+            conj_normalize(r);
+            cplx<R> b0 = mk<R>(r.a[0], r.a[3]), b1 = mk<R>(r.a[2], r.a[1]);
+            cplx<R> b2 = mk<R>(-r.a[2], r.a[1]), b3 = mk<R>(r.a[0], -r.a[3]);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                cplx<R> tmp0 = b0 * U.e[p][j] + b1 * U.e[q][j];
+                cplx<R> tmp1 = b2 * U.e[p][j] + b3 * U.e[q][j];
+                U.e[p][j] = b0 * tmp0 + b1 * tmp1;
+                U.e[q][j] = b2 * tmp0 + b3 * tmp1;
+            }
+        }
+    } else {
+        Mat<R, N> M = U * F;
+#pragma unroll
+        for (int block = 0; block < N * (N - 1) / 2; block++) {
+            const int x = block_p(N, block), y = block_q(N, block);
+            Quat<R> r = get_block_su2<R, N>(M, x, y);
+            conj_normalize(r);
+            mul_block<R, N>(r, U, x, y);
+            mul_block<R, N>(r, U, x, y);
+            mul_block<R, N>(r, M, x, y);
+            mul_block<R, N>(r, M, x, y);
+        }
+    }
+}
+
+// Reunitarisation (reunitlink.h:16-85).  N=3: row Gram-Schmidt + cross product;
+// N>3: column Gram-Schmidt + determinant phase on the last column.  N=2 is not
+// compilable in the reference (reunitlink.h:86); here the textbook projection
+// a=(U00+conj U11)/2, b=(U01-conj U10)/2, normalised.
+template <typename R, int N>
+SUN_HD void reunit_link(Mat<R, N>& U) {
+    if constexpr (N == 3) {
+        R t1 = R(0);
+        cplx<R> t2 = mk<R>(0, 0);
+#pragma unroll
+        for (int c = 0; c < 3; c++) t1 += abs2(U.e[0][c]);
+        t1 = R(1) / sqrt(t1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) U.e[0][c] = scale(U.e[0][c], t1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) t2 = t2 + conj(U.e[0][c]) * U.e[1][c];
+#pragma unroll
+        for (int c = 0; c < 3; c++) U.e[1][c] = U.e[1][c] - t2 * U.e[0][c];
+        t1 = R(0);
+#pragma unroll
+        for (int c = 0; c < 3; c++) t1 += abs2(U.e[1][c]);
+        t1 = R(1) / sqrt(t1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) U.e[1][c] = scale(U.e[1][c], t1);
+        reconstruct_row2(U);
+    } else if constexpr (N == 2) {
+        cplx<R> a = mk<R>((U.e[0][0].re + U.e[1][1].re) * R(0.5), (U.e[0][0].im - U.e[1][1].im) * R(0.5));
+        cplx<R> b = mk<R>((U.e[0][1].re - U.e[1][0].re) * R(0.5), (U.e[0][1].im + U.e[1][0].im) * R(0.5));
+        R n = R(1) / sqrt(abs2(a) + abs2(b));
+        a = scale(a, n);
+        b = scale(b, n);
+        U.e[0][0] = a;
+        U.e[0][1] = b;
+        U.e[1][0] = mk<R>(-b.re, b.im);
+        U.e[1][1] = conj(a);
+    } else {
+        R t1 = R(0);
+        cplx<R> t2;
+#pragma unroll
+        for (int c = 0; c < N; c++) t1 += abs2(U.e[c][0]);
+        t1 = R(1) / sqrt(t1);
+#pragma unroll
+        for (int c = 0; c < N; c++) U.e[c][0] = scale(U.e[c][0], t1);
+#pragma unroll
+        for (int j = 1; j < N; j++) {
+#pragma unroll
+            for (int i = 0; i < j; i++) {
+                t2 = conj(U.e[0][i]) * U.e[0][j];
+#pragma unroll
+                for (int c = 1; c < N; ++c) t2 = t2 + conj(U.e[c][i]) * U.e[c][j];
+#pragma unroll
+                for (int c = 0; c < N; ++c) U.e[c][j] = U.e[c][j] - t2 * U.e[c][i];
+            }
+            t1 = R(0);
+#pragma unroll
+            for (int c = 0; c < N; ++c) t1 += abs2(U.e[c][j]);
+            t1 = R(1) / sqrt(t1);
+#pragma unroll
+            for (int c = 0; c < N; ++c) U.e[c][j] = scale(U.e[c][j], t1);
+        }
+        t2 = det<R, N>(U);
+        R ai = atan2(t2.im, t2.re);
+        t2 = mk<R>(cos(ai), -sin(ai));
+#pragma unroll
+        for (int c = 0; c < N; ++c) U.e[c][N - 1] = U.e[c][N - 1] * t2;
+    }
+}
+
+}  // namespace sun

@@ -1,0 +1,7 @@
+// Kernels of the SU(3), 12-real, double variant (see sun_kernels.inc).
+#define SUN_R double
+#define SUN_N 3
+#define SUN_STORE 12
+#define SUN_OPS_FN ops_n3_f12_r8
+#define SUN_VARIANT_NS v_n3_f12_r8
+#include "sun_kernels.inc"

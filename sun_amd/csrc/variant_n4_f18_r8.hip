@@ -1,0 +1,7 @@
+// Kernels of the SU(4), 18-real, double variant (see sun_kernels.inc).
+#define SUN_R double
+#define SUN_N 4
+#define SUN_STORE 18
+#define SUN_OPS_FN ops_n4_f18_r8
+#define SUN_VARIANT_NS v_n4_f18_r8
+#include "sun_kernels.inc"

@@ -1,0 +1,145 @@
+"""GPU tests of the generation driver, hipGraph replay, batching and tuning.
+
+* sun_generate on the reference's own small case (SU(3) 8^4, beta=6.0, Haar hot
+  start seed 1, FP64 18-real, 10 x (1 heatbath + 4 overrelaxation), plaquette every
+  iteration) tracks the FP64 oracle iteration by iteration;
+* a captured-and-replayed hipGraph produces bit-identical links and observables to
+  eager launches, on every replay;
+* a lattice gives bit-identical results alone and inside a batch of lattices of
+  other volumes and couplings (the serving determinism requirement);
+* the autotuner's block size never changes results.
+"""
+import numpy as np
+import pytest
+
+import sun_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def test_generate_config1_matches_oracle(ctx, oracle):
+    dims, beta, seed = (8, 8, 8, 8), 6.0, 1
+    with sun_amd.Lattice(ctx, 3, dims, beta, 18, 8) as lat:
+        lat.hot_start(seed)
+        U = lat.to_host()
+        obs = lat.generate(seed=seed, n_iter=10, n_or=4, reunit_every=0, use_graph=True)
+        G = lat.to_host()
+    po = []
+    for it in range(10):
+        oracle.heatbath_sweep(U, dims, beta, seed, it)
+        for _ in range(4):
+            oracle.overrelax_sweep(U, dims)
+        po.append(oracle.plaquette(U, dims))
+    po = np.array(po)
+    assert np.abs(obs[:, :2] - po).max() < 1e-11
+    assert np.abs(G - U).max() < 1e-9
+    # Polyakov loop of the final configuration
+    pl = oracle.polyakov(U, dims)
+    assert abs(complex(obs[-1, 2], obs[-1, 3]) - pl) < 1e-11
+
+
+@pytest.mark.parametrize("storage,prec", [(12, 4), (18, 8), (8, 4)])
+def test_graph_replay_bitwise_equals_eager(ctx, storage, prec):
+    dims = (8, 8, 8, 12)
+    out = []
+    for use_graph in (False, True):
+        with sun_amd.Lattice(ctx, 3, dims, 5.9, storage, prec) as lat:
+            lat.hot_start(3)
+            obs = lat.generate(seed=9, n_iter=4, n_or=2, reunit_every=2, use_graph=use_graph)
+            out.append((obs, lat.to_host()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_generate_continues_chain(ctx):
+    """two calls of n iterations == one call of 2n (sweep counter and RNG continue)"""
+    dims = (4, 4, 8, 8)
+    res = []
+    for split in (False, True):
+        with sun_amd.Lattice(ctx, 3, dims, 6.0, 12, 4) as lat:
+            lat.cold_start()
+            if split:
+                a = lat.generate(seed=5, n_iter=3, first_sweep=0)
+                b = lat.generate(seed=5, n_iter=3, first_sweep=3)
+                obs = np.concatenate([a, b])
+            else:
+                obs = lat.generate(seed=5, n_iter=6, first_sweep=0)
+            res.append((obs, lat.to_host()))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+
+
+@pytest.mark.parametrize("storage,prec", [(12, 4), (18, 8)])
+def test_batch_bitwise_equals_alone(ctx, storage, prec):
+    specs = [((8, 8, 8, 8), 5.8, 11, True), ((4, 6, 8, 10), 6.1, 12, False), ((6, 6, 6, 12), 6.3, 13, True)]
+    alone = []
+    for dims, beta, seed, hot in specs:
+        with sun_amd.Lattice(ctx, 3, dims, beta, storage, prec) as lat:
+            lat.hot_start(seed) if hot else lat.cold_start()
+            obs = lat.generate(seed=seed, n_iter=3, n_or=4, reunit_every=1, first_sweep=0, use_graph=False)
+            alone.append((obs[:, :2], lat.to_host()))
+    lats = []
+    for dims, beta, seed, hot in specs:
+        lat = sun_amd.Lattice(ctx, 3, dims, beta, storage, prec)
+        lat.hot_start(seed) if hot else lat.cold_start()
+        lats.append(lat)
+    b = sun_amd.Batch(ctx, lats, [s[2] for s in specs])
+    obs = b.generate(first_sweep=0, n_iter=3, n_or=4, reunit_every=1, use_graph=True)
+    for i, lat in enumerate(lats):
+        assert np.array_equal(obs[:, i, :], alone[i][0])
+        assert np.array_equal(lat.to_host(), alone[i][1])
+    b.close()
+    for lat in lats:
+        lat.close()
+
+
+def test_batch_step_api_matches_generate(ctx):
+    specs = [((8, 8, 8, 8), 6.0, 21), ((4, 4, 4, 8), 5.7, 22)]
+    lats = [sun_amd.Lattice(ctx, 3, d, b, 12, 4) for d, b, _ in specs]
+    for l, (_, _, s) in zip(lats, specs):
+        l.hot_start(s)
+    b = sun_amd.Batch(ctx, lats, [s for _, _, s in specs])
+    b.heatbath([0, 0])
+    for _ in range(2):
+        b.overrelax()
+    b.reunitarize()
+    p = b.plaquette()
+    ref = []
+    for (d, beta, s) in specs:
+        with sun_amd.Lattice(ctx, 3, d, beta, 12, 4) as l:
+            l.hot_start(s)
+            l.heatbath(s, 0)
+            l.overrelax()
+            l.overrelax()
+            l.reunitarize()
+            ref.append(l.plaquette())
+    assert np.array_equal(p, np.array(ref))
+    b.close()
+    for l in lats:
+        l.close()
+
+
+def test_autotuned_block_size_does_not_change_results(oracle):
+    res = []
+    for tune in (0, 1):
+        c = sun_amd.Context(0, 0, tune)
+        with sun_amd.Lattice(c, 3, (8, 8, 8, 8), 6.0, 12, 4) as lat:
+            lat.hot_start(2)
+            lat.heatbath(2, 0)
+            lat.overrelax()
+            res.append(lat.to_host())
+        c.close()
+    assert np.array_equal(res[0], res[1])
+
+
+def test_errors_are_reported_not_fatal(ctx):
+    with pytest.raises(sun_amd.SunError, match="even"):
+        sun_amd.Lattice(ctx, 3, (5, 4, 4, 4), 6.0, 18, 8)
+    with pytest.raises(sun_amd.SunError, match="SU\\(3\\) only"):
+        sun_amd.Lattice(ctx, 4, (4, 4, 4, 4), 6.0, 12, 4)
+    with pytest.raises(sun_amd.SunError, match="nc=5"):
+        sun_amd.Lattice(ctx, 5, (4, 4, 4, 4), 6.0, 18, 4)
+    # the context keeps working after errors
+    with sun_amd.Lattice(ctx, 3, (4, 4, 4, 4), 6.0, 18, 8) as lat:
+        lat.cold_start()
+        assert lat.plaquette()[0] == 1.0

@@ -1,0 +1,168 @@
+"""GPU parity: the HIP engine, called through the C ABI, against the FP64 oracle.
+
+Tolerances (north star): link-by-link after an overrelaxation or heatbath sweep
+<= 1e-12 in FP64 storage and <= 5e-5 in FP32 storage.  In FP32 a heatbath
+accept/reject branch may flip relative to the FP64 oracle (its input differs by
+FP32 rounding); such a flip changes one link by O(1) and, through the staples, a
+few neighbours updated after it.  FP32 heatbath tests therefore bound the count
+of links beyond 5e-5 and require every other link within 5e-5.
+"""
+import numpy as np
+import pytest
+
+import sun_amd
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [  # (nc, storage, precision)
+    (3, 18, 8), (3, 12, 8), (3, 8, 8), (2, 18, 8), (4, 18, 8),
+    (3, 18, 4), (3, 12, 4), (3, 8, 4), (2, 18, 4), (4, 18, 4),
+]
+IDS = [f"su{n}_s{s}_{'f64' if p == 8 else 'f32'}" for n, s, p in VARIANTS]
+
+
+def tol(prec):
+    return 1e-12 if prec == 8 else 5e-5
+
+
+def store_tol(storage, prec):
+    """Rounding of one stored link: FP64 exact to ~1e-13; FP32 to a few ulp, more
+    for 8-real storage, whose decode divides by |e01|^2+|e02|^2."""
+    if prec == 8:
+        return 1e-13
+    return 2e-5 if storage == 8 else 2e-6
+
+
+def maxdiff(A, B):
+    return float(np.abs(A - B).max())
+
+
+def per_link_diff(A, B):
+    return np.abs(A - B).reshape(A.shape[0] * A.shape[1], -1).max(axis=1)
+
+
+def make(ctx, nc, dims, beta, storage, prec):
+    return sun_amd.Lattice(ctx, nc, dims, beta, storage, prec)
+
+
+def start_config(oracle, lat, seed):
+    """Hot-start the GPU lattice and return its decoded links (the common input)."""
+    lat.hot_start(seed)
+    return lat.to_host()
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+def test_cold_start_observables(ctx, nc, storage, prec):
+    dims = (4, 4, 4, 4)
+    with make(ctx, nc, dims, 6.0, storage, prec) as lat:
+        if storage == 8:
+            with pytest.raises(sun_amd.SunError, match="8-real"):
+                lat.cold_start()
+            return
+        lat.cold_start()
+        p = lat.plaquette()
+        assert p[0] == 1.0 and p[1] == 1.0
+        assert lat.polyakov() == 1.0 + 0.0j
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+def test_hot_start_matches_oracle(ctx, oracle, nc, storage, prec):
+    dims = (4, 4, 4, 6)
+    with make(ctx, nc, dims, 6.0, storage, prec) as lat:
+        lat.hot_start(11)
+        G = lat.to_host()
+    O = oracle.hot_start(nc, dims, 11, storage)
+    assert maxdiff(G, O) <= store_tol(storage, prec)
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+def test_host_roundtrip(ctx, oracle, nc, storage, prec):
+    dims = (4, 4, 4, 4)
+    O = oracle.hot_start(nc, dims, 5, storage)
+    with make(ctx, nc, dims, 6.0, storage, prec) as lat:
+        lat.from_host(O)
+        G = lat.to_host()
+    assert maxdiff(G, O) <= store_tol(storage, prec)
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (4, 6, 8, 10)])
+def test_overrelax_sweep_parity(ctx, oracle, nc, storage, prec, dims):
+    with make(ctx, nc, dims, 6.0, storage, prec) as lat:
+        U = start_config(oracle, lat, 21)
+        lat.overrelax()
+        G = lat.to_host()
+    oracle.overrelax_sweep(U, dims, storage)
+    assert maxdiff(G, U) <= tol(prec)
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (4, 6, 8, 10)])
+def test_heatbath_sweep_parity(ctx, oracle, nc, storage, prec, dims):
+    beta = {2: 2.3, 3: 6.0, 4: 11.0}[nc]
+    seed, sweep = 1234, 7
+    with make(ctx, nc, dims, beta, storage, prec) as lat:
+        U = start_config(oracle, lat, 31)
+        lat.heatbath(seed, sweep)
+        G = lat.to_host()
+    oracle.heatbath_sweep(U, dims, beta, seed, sweep, storage)
+    d = per_link_diff(G, U)
+    if prec == 8:
+        assert d.max() <= 1e-12
+    else:
+        bad = int((d > 5e-5).sum())
+        assert bad <= max(4, d.size // 200), f"{bad} of {d.size} links beyond 5e-5"
+
+
+@pytest.mark.parametrize("nc,storage,prec", [(3, 18, 8), (3, 12, 4), (2, 18, 8), (4, 18, 8)],
+                         ids=["su3_s18_f64", "su3_s12_f32", "su2_f64", "su4_f64"])
+def test_sub_sweep_parity(ctx, oracle, nc, storage, prec):
+    """Every (parity, mu) launch separately, heatbath and overrelaxation."""
+    dims = (4, 4, 6, 8)
+    beta = {2: 2.3, 3: 5.8, 4: 10.5}[nc]
+    with make(ctx, nc, dims, beta, storage, prec) as lat:
+        for parity in (0, 1):
+            for mu in range(4):
+                U = start_config(oracle, lat, 100 + 4 * parity + mu)
+                lat.heatbath_sub(9, 3, parity, mu)
+                G = lat.to_host()
+                oracle.heatbath_sub(U, dims, beta, 9, 3, parity, mu, storage)
+                d = per_link_diff(G, U)
+                assert int((d > tol(prec)).sum()) <= (0 if prec == 8 else 2)
+                U = start_config(oracle, lat, 200 + 4 * parity + mu)
+                lat.overrelax_sub(parity, mu)
+                G = lat.to_host()
+                oracle.overrelax_sub(U, dims, parity, mu, storage)
+                assert maxdiff(G, U) <= tol(prec)
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+def test_reunitarize_parity(ctx, oracle, nc, storage, prec):
+    dims = (4, 4, 4, 4)
+    rng = np.random.default_rng(3)
+    O = oracle.hot_start(nc, dims, 8, 18)
+    O = O + 1e-3 * (rng.standard_normal(O.shape) + 1j * rng.standard_normal(O.shape))
+    with make(ctx, nc, dims, 6.0, storage, prec) as lat:
+        lat.from_host(O)
+        U = lat.to_host()
+        lat.reunitarize()
+        G = lat.to_host()
+    oracle.reunitarize(U, storage)
+    assert maxdiff(G, U) <= store_tol(storage, prec)
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+def test_observables_parity(ctx, oracle, nc, storage, prec):
+    dims = (6, 4, 4, 8)
+    with make(ctx, nc, dims, 6.0, storage, prec) as lat:
+        lat.hot_start(4)
+        U = lat.to_host()
+        for _ in range(2):
+            lat.overrelax()
+        U2 = lat.to_host()
+        p, pl = lat.plaquette(), lat.polyakov()
+    po = oracle.plaquette(U2, dims)
+    plo = oracle.polyakov(U2, dims)
+    t = 1e-13 if prec == 8 else 2e-6
+    assert np.abs(p - po).max() <= t
+    assert abs(pl - plo) <= t
