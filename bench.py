@@ -57,22 +57,57 @@ def time_launches(sun_amd, lat, stream, hb: bool, reps: int) -> float:
     return ms
 
 
-def cpu_baseline(seconds_hint: float = 20.0):
+def cpu_baseline(min_seconds: float = 10.0, max_iters: int = 20):
+    """FP64 oracle on this host's cores: whole iterations (1 HB + 4 OR sweeps) of the
+    same 32^4 workload until at least min_seconds have passed."""
     from oracle import oracle as o
     cores = min(os.cpu_count() or 1, 16)
     o.set_threads(cores)
     U = o.cold_start(3, DIMS)
-    # one heatbath sweep first so the timed iteration starts from a non-trivial field
+    # one heatbath sweep first so the timed iterations start from a non-trivial field
     o.heatbath_sweep(U, DIMS, BETA, 2, 0)
+    n = 0
     t0 = time.perf_counter()
-    o.heatbath_sweep(U, DIMS, BETA, 2, 1)
-    for _ in range(4):
-        o.overrelax_sweep(U, DIMS)
+    while n < max_iters and (n == 0 or time.perf_counter() - t0 < min_seconds):
+        o.heatbath_sweep(U, DIMS, BETA, 2, 1 + n)
+        for _ in range(4):
+            o.overrelax_sweep(U, DIMS)
+        n += 1
     dt = time.perf_counter() - t0
     V = int(np.prod(DIMS))
-    return {"value": 5 * 4 * V / dt, "unit": "link-updates/s", "cores": cores, "kind": "port",
-            "sample": "FP64 oracle (C, OpenMP), 1 iteration = 1 heatbath + 4 overrelaxation sweeps of "
-                      f"SU(3) 32^4 beta=6.0 after one heatbath sweep from cold; {dt:.1f} s"}
+    return {"value": n * 5 * 4 * V / dt, "unit": "link-updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (C, OpenMP, {cores} threads): {n} iterations of 1 heatbath + 4 "
+                      f"overrelaxation sweeps of SU(3) 32^4 beta=6.0 (after one heatbath sweep from cold), "
+                      f"{dt:.1f} s"}
+
+
+def kernels_only(sun_amd):
+    """For rocprofv3 counter passes: on the 32^4 FP32 12-real field after a few
+    iterations, 2 copies of the field (calibration: a plain copy of known size),
+    then 8 overrelaxation launches, 8 heatbath launches, reunitarize, plaquette,
+    Polyakov loop."""
+    ctx = sun_amd.Context(0, 0, 0)
+    lat = sun_amd.Lattice(ctx, 3, DIMS, BETA, 12, 4)
+    tmp = sun_amd.Lattice(ctx, 3, DIMS, BETA, 12, 4)
+    s = sun_amd.new_stream()
+    lat.cold_start(s)
+    lat.generate(seed=2, n_iter=3, n_or=4, reunit_every=1, stream=s)
+    for _ in range(2):
+        tmp.copy_from(lat, s)
+    for parity in (0, 1):
+        for mu in range(4):
+            lat.overrelax_sub(parity, mu, s)
+    for parity in (0, 1):
+        for mu in range(4):
+            lat.heatbath_sub(2, 3, parity, mu, s)
+    lat.reunitarize(s)
+    lat.plaquette(s)
+    lat.polyakov(s)
+    sun_amd.sync(s)
+    print(f"kernels-only: field {lat.bytes} bytes, copy calibration size {tmp.bytes} bytes")
+    tmp.close()
+    lat.close()
+    ctx.close()
 
 
 def main():
@@ -82,6 +117,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU leg")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="fixed short launch sequence for profiler counter passes (no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -89,6 +126,9 @@ def main():
         sys.exit(2)
 
     import sun_amd
+    if args.kernels_only:
+        kernels_only(sun_amd)
+        return
     ctx = sun_amd.Context(0, 0, 1)
     V = int(np.prod(DIMS))
     updates_per_step = 5 * 4 * V

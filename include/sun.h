@@ -82,6 +82,9 @@ const char *sun_version(void);
 int sun_lattice_create(sun_ctx *ctx, const sun_lattice_desc *desc, sun_lattice **out);
 /* _gauge<Real>::Release                          include/gaugearray.h:657-688 */
 int sun_lattice_destroy(sun_lattice *lat);
+/* _gauge<Real>::Copy / Backup: device-to-device copy of the whole link field; the two
+ * lattices must share N, extents, storage and precision   include/gaugearray.h:279, :282-314 */
+int sun_lattice_copy(sun_lattice *dst, const sun_lattice *src, void *stream);
 /* device bytes of the link field (gaugearray.h:447-464 Bytes()) */
 size_t sun_lattice_bytes(const sun_lattice_desc *desc);
 /* _gauge<Real>::Init() -> ColdStart; refuses 8-real storage as the reference does

@@ -37,7 +37,7 @@ _lib = None
 
 EXPORTS = [
     "sun_init", "sun_finalize", "sun_last_error", "sun_version",
-    "sun_lattice_create", "sun_lattice_destroy", "sun_lattice_bytes",
+    "sun_lattice_create", "sun_lattice_destroy", "sun_lattice_bytes", "sun_lattice_copy",
     "sun_cold_start", "sun_hot_start",
     "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
     "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
@@ -85,6 +85,7 @@ def lib() -> ctypes.CDLL:
         "sun_lattice_create": (ci, [vp, ctypes.POINTER(LatticeDesc), ctypes.POINTER(vp)]),
         "sun_lattice_destroy": (ci, [vp]),
         "sun_lattice_bytes": (ctypes.c_size_t, [ctypes.POINTER(LatticeDesc)]),
+        "sun_lattice_copy": (ci, [vp, vp, vp]),
         "sun_cold_start": (ci, [vp, vp]),
         "sun_hot_start": (ci, [vp, u64, vp]),
         "sun_heatbath": (ci, [vp, u64, u32, vp]),
@@ -186,6 +187,14 @@ class Lattice:
 
     def __exit__(self, *exc):
         self.close()
+
+    def copy_from(self, other: "Lattice", stream=None):
+        """device-to-device copy of other's links (_gauge::Copy)"""
+        _check(lib().sun_lattice_copy(self.h, other.h, stream), "sun_lattice_copy")
+
+    @property
+    def bytes(self) -> int:
+        return lattice_bytes(self.nc, self.dims, self.storage, self.precision, self.beta)
 
     # -- initialisation
     def cold_start(self, stream=None):

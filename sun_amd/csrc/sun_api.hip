@@ -97,7 +97,9 @@ int sun::check_desc(const sun_lattice_desc* d) {
             return fail(SUN_ERR_INVALID, "dims[%d]=%d: extents must be even and >= 2", i, d->dims[i]);
         V *= d->dims[i];
     }
-    if (4 * V >= (1ll << 31)) return fail(SUN_ERR_INVALID, "volume %lld too large (4V must fit in int32)", V);
+    // a plane holds 4V records of 16 bytes addressed by 32-bit byte offsets
+    if (4 * V * 16 > 0xFFFFFFFFll)
+        return fail(SUN_ERR_INVALID, "volume %lld too large (4V x 16 bytes must fit in 32 bits)", V);
     if (d->storage != SUN_STORE_18 && d->storage != SUN_STORE_12 && d->storage != SUN_STORE_8)
         return fail(SUN_ERR_INVALID, "storage=%d: use 18, 12 or 8", d->storage);
     if (d->nc != 3 && d->storage != SUN_STORE_18)
@@ -165,6 +167,7 @@ int sun::lattice_alloc(sun_ctx* ctx, const sun_lattice_desc* d, sun_lattice** ou
     }
     L->cfg_hb = ctx->tuner.lookup("heatbath", *d);
     L->cfg_or = ctx->tuner.lookup("overrelax", *d);
+    L->tuned = ctx->tuner.has("heatbath", *d) && ctx->tuner.has("overrelax", *d);
     *out = L;
     return SUN_OK;
 }
@@ -193,6 +196,14 @@ extern "C" int sun_lattice_destroy(sun_lattice* lat) {
     hipSetDevice(lat->ctx->device);
     hipDeviceSynchronize();
     lattice_free(lat);
+    return SUN_OK;
+}
+
+extern "C" int sun_lattice_copy(sun_lattice* dst, const sun_lattice* src, void* stream) {
+    if (!dst || !src) return fail(SUN_ERR_INVALID, "sun_lattice_copy: NULL lattice");
+    if (dst->ops != src->ops || memcmp(dst->d.dims, src->d.dims, sizeof(dst->d.dims)) != 0)
+        return fail(SUN_ERR_INVALID, "sun_lattice_copy: lattices differ in N, extents, storage or precision");
+    HIP_TRY(hipMemcpyAsync(dst->buf, src->buf, src->bytes, hipMemcpyDeviceToDevice, pick_stream(dst, stream)));
     return SUN_OK;
 }
 

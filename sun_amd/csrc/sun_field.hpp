@@ -111,21 +111,25 @@ struct Layout {
     static_assert(NPLANES <= kMaxPlanes, "too many planes");
 };
 
+// Byte offsets within a plane are 32-bit (a plane holds 4V records of 16 bytes and
+// 4V < 2^28 for every supported volume), so each access is a uniform 64-bit plane
+// base in SGPRs plus one 32-bit VGPR offset.
 template <typename R, int N, int STORE>
 SUN_HD void load_reals(const Field& f, uint32_t l, R* r) {
     using L = Layout<R, N, STORE>;
+    const uint32_t off = l * 16u;
 #pragma unroll
     for (int c = 0; c < L::NFULL; c++) {
         if constexpr (sizeof(R) == 4) {
-            float4 v = *reinterpret_cast<const float4*>(f.p[c] + (size_t)l * 16u);
+            float4 v = *reinterpret_cast<const float4*>(f.p[c] + off);
             r[4 * c + 0] = v.x; r[4 * c + 1] = v.y; r[4 * c + 2] = v.z; r[4 * c + 3] = v.w;
         } else {
-            double2 v = *reinterpret_cast<const double2*>(f.p[c] + (size_t)l * 16u);
+            double2 v = *reinterpret_cast<const double2*>(f.p[c] + off);
             r[2 * c + 0] = v.x; r[2 * c + 1] = v.y;
         }
     }
     if constexpr (L::TAIL != 0) {
-        float2 v = *reinterpret_cast<const float2*>(f.p[L::NFULL] + (size_t)l * 8u);
+        float2 v = *reinterpret_cast<const float2*>(f.p[L::NFULL] + l * 8u);
         r[L::NREAL - 2] = v.x;
         r[L::NREAL - 1] = v.y;
     }
@@ -134,23 +138,24 @@ SUN_HD void load_reals(const Field& f, uint32_t l, R* r) {
 template <typename R, int N, int STORE>
 SUN_HD void store_reals(const Field& f, uint32_t l, const R* r) {
     using L = Layout<R, N, STORE>;
+    const uint32_t off = l * 16u;
 #pragma unroll
     for (int c = 0; c < L::NFULL; c++) {
         if constexpr (sizeof(R) == 4) {
             float4 v;
             v.x = r[4 * c + 0]; v.y = r[4 * c + 1]; v.z = r[4 * c + 2]; v.w = r[4 * c + 3];
-            *reinterpret_cast<float4*>(f.p[c] + (size_t)l * 16u) = v;
+            *reinterpret_cast<float4*>(f.p[c] + off) = v;
         } else {
             double2 v;
             v.x = r[2 * c + 0]; v.y = r[2 * c + 1];
-            *reinterpret_cast<double2*>(f.p[c] + (size_t)l * 16u) = v;
+            *reinterpret_cast<double2*>(f.p[c] + off) = v;
         }
     }
     if constexpr (L::TAIL != 0) {
         float2 v;
         v.x = r[L::NREAL - 2];
         v.y = r[L::NREAL - 1];
-        *reinterpret_cast<float2*>(f.p[L::NFULL] + (size_t)l * 8u) = v;
+        *reinterpret_cast<float2*>(f.p[L::NFULL] + l * 8u) = v;
     }
 }
 

@@ -51,6 +51,7 @@ class Tuner {
     void load(int mode);
     void save();
     LaunchCfg lookup(const char* kernel, const sun_lattice_desc& d);
+    bool has(const char* kernel, const sun_lattice_desc& d);
     void record(const char* kernel, const sun_lattice_desc& d, const LaunchCfg& c, float ms);
     bool enabled() const { return mode_ != 0; }
     std::string arch;

@@ -14,6 +14,11 @@ namespace sun {
 // along the fastest lattice direction for the tiled update kernel).
 struct LaunchCfg {
     int block = 256;
+    // dynamic LDS reserved per workgroup and not used by the kernel: caps the
+    // workgroups resident per CU (160 KB of LDS per CU), i.e. the lattice volume in
+    // flight per XCD, which decides whether neighbour links are still in L2 when
+    // the next staple direction needs them
+    int lds = 0;
 };
 
 // One lattice of a batched launch.  `first_block` is the prefix sum of the blocks
@@ -53,6 +58,18 @@ struct OpsTable {
 };
 
 constexpr int kReduceBlock = 256;   // threads of every reduction block
+
+// gfx950 dispatches workgroup b of a grid to XCD (b mod 8); each XCD has its own
+// 4 MB L2.  xcd_remap turns the dispatch order into a contiguous range of tiles per
+// XCD, so the lattice neighbours that share links (+-1 site, +-1 row, +-1 plane) run
+// on the same L2.  A bijection of [0, nblocks) for any nblocks.
+constexpr int kXcds = 8;
+
+SUN_HD int xcd_remap(int b, int nblocks) {
+    const int xcd = b % kXcds, local = b / kXcds;
+    const int per = nblocks / kXcds, rem = nblocks % kXcds;
+    return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
 
 inline int plaquette_blocks(const Geom& g) { return (g.V + kReduceBlock - 1) / kReduceBlock; }
 inline int polyakov_blocks(const Geom& g) {

@@ -124,4 +124,58 @@ SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
     a[2] = xr1 * sin(xr2);
 }
 
+// The same generator evaluated in FP32 (uniforms rounded to float), used by the
+// FP32-storage engine: the link it updates is FP32, so the sampler's precision
+// matches it, and FP32 log/cos/sin/exp/sqrt cost a fraction of the FP64 ones.
+SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
+    constexpr float kTwoPiF = (float)kTwoPi;
+    float xr1, xr2, xr3, xr4, d, r;
+    float a0 = 0.f;
+    xr1 = (float)st.next();
+    xr1 = logf(xr1 + 1.e-10f);
+    xr2 = (float)st.next();
+    xr2 = logf(xr2 + 1.e-10f);
+    xr3 = (float)st.next();
+    xr4 = (float)st.next();
+    xr3 = cosf(kTwoPiF * xr3);
+    d = -(xr2 + xr1 * xr3 * xr3) / al;
+    bool accepted = (1.00f - 0.5f * d) > xr4 * xr4;
+    if (!accepted && al > 2.f) {
+        for (int k = 0; k < 20; k++) {
+            xr1 = (float)st.next();
+            xr1 = logf(xr1 + 1.e-10f);
+            xr2 = (float)st.next();
+            xr2 = logf(xr2 + 1.e-10f);
+            xr3 = (float)st.next();
+            xr4 = (float)st.next();
+            xr3 = cosf(kTwoPiF * xr3);
+            d = -(xr2 + xr1 * xr3 * xr3) / al;
+            if ((1.00f - 0.5f * d) > xr4 * xr4) break;
+        }
+    }
+    if (!accepted && al <= 2.f) {
+        xr3 = expf(-2.f * al);
+        xr4 = 1.f - xr3;
+        for (int k = 0; k < 20; k++) {
+            xr1 = (float)st.next();
+            xr2 = (float)st.next();
+            r = xr3 + xr4 * xr1;
+            a0 = 1.00f + logf(r) / al;
+            if ((1.f - a0 * a0) > xr2 * xr2) break;
+        }
+        d = 1.f - a0;
+    }
+    a[0] = 1.f - d;
+    xr3 = 1.f - a[0] * a[0];
+    xr3 = fabsf(xr3);
+    r = sqrtf(xr3);
+    a[3] = (2.f * (float)st.next() - 1.f) * r;
+    xr1 = xr3 - a[3] * a[3];
+    xr1 = fabsf(xr1);
+    xr1 = sqrtf(xr1);
+    xr2 = kTwoPiF * (float)st.next();
+    a[1] = xr1 * cosf(xr2);
+    a[2] = xr1 * sinf(xr2);
+}
+
 }  // namespace sun

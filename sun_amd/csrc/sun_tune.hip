@@ -17,7 +17,8 @@
 
 namespace sun {
 
-static const int kCandidates[] = {64, 128, 192, 256};
+static const int kBlocks[] = {64, 128, 256};
+static const int kLds[] = {0, 16384, 24576, 32768, 40960, 54272, 81920};
 
 std::string Tuner::key(const char* kernel, const sun_lattice_desc& d) const {
     char buf[256];
@@ -32,7 +33,7 @@ std::string Tuner::path() const {
     return std::string(dir) + "/sun_tunecache.json";
 }
 
-// The cache file is a JSON object {"key": [block, ms], ...}, one entry per line.
+// The cache file is a JSON object {"key": [block, lds, ms], ...}, one entry per line.
 void Tuner::load(int mode) {
     std::lock_guard<std::mutex> lk(mu_);
     mode_ = mode;
@@ -50,10 +51,12 @@ void Tuner::load(int mode) {
         if (q1 == std::string::npos || b0 == std::string::npos) continue;
         LaunchCfg c;
         float ms = 0.f;
-        if (sscanf(line.c_str() + b0, "[%d, %f]", &c.block, &ms) != 2) continue;
+        if (sscanf(line.c_str() + b0, "[%d, %d, %f]", &c.block, &c.lds, &ms) != 3) continue;
         bool ok = false;
-        for (int k : kCandidates) ok |= (k == c.block);
-        if (ok) cache_[line.substr(q0 + 1, q1 - q0 - 1)] = {c, ms};
+        for (int k : kBlocks) ok |= (k == c.block);
+        bool okl = false;
+        for (int k : kLds) okl |= (k == c.lds);
+        if (ok && okl) cache_[line.substr(q0 + 1, q1 - q0 - 1)] = {c, ms};
     }
 }
 
@@ -65,7 +68,8 @@ void Tuner::save() {
     out << "{\n";
     size_t i = 0;
     for (auto& kv : cache_) {
-        out << "  \"" << kv.first << "\": [" << kv.second.first.block << ", " << kv.second.second << "]"
+        out << "  \"" << kv.first << "\": [" << kv.second.first.block << ", " << kv.second.first.lds << ", "
+            << kv.second.second << "]"
             << (++i < cache_.size() ? ",\n" : "\n");
     }
     out << "}\n";
@@ -77,6 +81,11 @@ LaunchCfg Tuner::lookup(const char* kernel, const sun_lattice_desc& d) {
     auto it = cache_.find(key(kernel, d));
     if (it != cache_.end()) return it->second.first;
     return LaunchCfg();
+}
+
+bool Tuner::has(const char* kernel, const sun_lattice_desc& d) {
+    std::lock_guard<std::mutex> lk(mu_);
+    return cache_.count(key(kernel, d)) != 0;
 }
 
 void Tuner::record(const char* kernel, const sun_lattice_desc& d, const LaunchCfg& c, float ms) {
@@ -102,9 +111,10 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
         const bool hb = which == 0;
         LaunchCfg best;
         float best_ms = 1e30f;
-        for (int cand : kCandidates) {
+        for (int cand = 0; cand < (int)(sizeof(kBlocks) / sizeof(int)) * (int)(sizeof(kLds) / sizeof(int)); cand++) {
             LaunchCfg c;
-            c.block = cand;
+            c.block = kBlocks[cand / (int)(sizeof(kLds) / sizeof(int))];
+            c.lds = kLds[cand % (int)(sizeof(kLds) / sizeof(int))];
             // warm-up, then 3 timed launches
             e = lat->ops->update(lat->f, lat->g, 0, 0, hb, 0, nullptr, bnc, 1, c, s);
             if (e == hipSuccess) e = hipEventRecord(e0, s);
@@ -117,7 +127,8 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
             if (e != hipSuccess) break;
             ms /= 3.f;
             if (ctx->verbosity >= SUN_VERBOSE)
-                fprintf(stderr, "[sun tune] %s block=%d %.4f ms\n", hb ? "heatbath" : "overrelax", cand, ms);
+                fprintf(stderr, "[sun tune] %s block=%d lds=%d %.4f ms\n", hb ? "heatbath" : "overrelax", c.block,
+                        c.lds, ms);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = c;

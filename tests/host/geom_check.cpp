@@ -4,8 +4,10 @@
 // and bound every index by V.  Usage: geom_check X Y Z T
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #include "sun_field.hpp"
+#include "sun_ops.hpp"
 
 int main(int argc, char** argv) {
     if (argc != 5) return 2;
@@ -40,6 +42,14 @@ int main(int argc, char** argv) {
             if (f.div(n) != n / d) { printf("FASTDIV FAIL %u %u\n", n, d); return 1; }
         for (uint32_t n = 0x7fffffffu - 70000; n <= 0x7fffffffu - 1; n++)
             if (f.div(n) != n / d) { printf("FASTDIV FAIL %u %u\n", n, d); return 1; }
+    }
+    // XCD remap of workgroups is a bijection of [0, n) for every grid size
+    for (int n = 1; n <= 5000; n++) {
+        std::vector<int> seen(n, 0);
+        for (int b = 0; b < n; b++) {
+            int v = sun::xcd_remap(b, n);
+            if (v < 0 || v >= n || seen[v]++) { printf("REMAP FAIL %d %d\n", n, b); return 1; }
+        }
     }
     return 0;
 }

@@ -17,25 +17,61 @@ import sun_amd
 pytestmark = pytest.mark.gpu
 
 
-def test_generate_config1_matches_oracle(ctx, oracle):
+def self_sensitivity(fn, U, eps):
+    rng = np.random.default_rng(7)
+    A = U.copy()
+    B = U * (1 + eps * rng.standard_normal(U.shape))
+    fn(A)
+    fn(B)
+    return float(np.abs(A - B).max())
+
+
+def test_generate_config1_lockstep(ctx, oracle):
+    """Each of the 10 iterations, from the same input on both sides, matches the
+    oracle link by link within the conditioning of that iteration (Haar start:
+    ~1e-6 per 1e-15 of input at iteration 0, ~1e-12 once thermalised)."""
     dims, beta, seed = (8, 8, 8, 8), 6.0, 1
     with sun_amd.Lattice(ctx, 3, dims, beta, 18, 8) as lat:
         lat.hot_start(seed)
-        U = lat.to_host()
-        obs = lat.generate(seed=seed, n_iter=10, n_or=4, reunit_every=0, use_graph=True)
-        G = lat.to_host()
-    po = []
-    for it in range(10):
-        oracle.heatbath_sweep(U, dims, beta, seed, it)
+        for it in range(10):
+            U = lat.to_host()
+            obs = lat.generate(seed=seed, n_iter=1, n_or=4, reunit_every=0, first_sweep=it)
+            G = lat.to_host()
+
+            def one_iteration(A):
+                oracle.heatbath_sweep(A, dims, beta, seed, it)
+                for _ in range(4):
+                    oracle.overrelax_sweep(A, dims)
+
+            sens = self_sensitivity(one_iteration, U, 2.0 ** -52)
+            one_iteration(U)
+            bound = max(1e-12, 50 * sens)
+            assert np.abs(G - U).max() <= bound, (it, np.abs(G - U).max(), sens)
+            assert np.abs(obs[0, :2] - oracle.plaquette(U, dims)).max() <= max(1e-13, bound)
+            assert abs(complex(obs[0, 2], obs[0, 3]) - oracle.polyakov(U, dims)) <= max(1e-13, bound)
+
+
+@pytest.mark.parametrize("nc,beta,storage", [(3, 6.0, 12), (2, 2.3, 18), (4, 11.0, 18)])
+def test_ensemble_agrees_with_oracle(ctx, oracle, nc, beta, storage):
+    """FP32 GPU chain vs FP64 oracle chain (same seeds, diverging after a few
+    iterations): mean plaquette over iterations 10..39 agrees within 5 sigma."""
+    dims = (6, 6, 6, 6)
+    n_iter, burn = 40, 10
+    with sun_amd.Lattice(ctx, nc, dims, beta, storage, 4) as lat:
+        lat.cold_start()
+        obs = lat.generate(seed=3, n_iter=n_iter, n_or=4, reunit_every=1)
+    g = obs[burn:, :2].mean(axis=1)
+    U = oracle.cold_start(nc, dims)
+    o = []
+    for it in range(n_iter):
+        oracle.heatbath_sweep(U, dims, beta, 3, it)
         for _ in range(4):
             oracle.overrelax_sweep(U, dims)
-        po.append(oracle.plaquette(U, dims))
-    po = np.array(po)
-    assert np.abs(obs[:, :2] - po).max() < 1e-11
-    assert np.abs(G - U).max() < 1e-9
-    # Polyakov loop of the final configuration
-    pl = oracle.polyakov(U, dims)
-    assert abs(complex(obs[-1, 2], obs[-1, 3]) - pl) < 1e-11
+        oracle.reunitarize(U)
+        o.append(oracle.plaquette(U, dims).mean())
+    o = np.array(o[burn:])
+    err = np.sqrt(g.var() / len(g) + o.var() / len(o)) * 2.0  # factor 2: autocorrelation
+    assert abs(g.mean() - o.mean()) < 5 * err + 1e-4, (g.mean(), o.mean(), err)
 
 
 @pytest.mark.parametrize("storage,prec", [(12, 4), (18, 8), (8, 4)])
