@@ -154,6 +154,63 @@ int sun_batch_plaquette(sun_batch *b, void *stream, double *out);
 int sun_batch_generate(sun_batch *b, uint32_t first_sweep, int n_iter, int n_or, int reunit_every,
                        int use_graph, void *stream, double *obs);
 
+/* ---- job server: continuous batching of ensemble-generation jobs ------------------------
+ * Jobs of different N, volume, beta and length are admitted while their link field
+ * and scratch fit the HBM budget (reserved once at creation; no device allocation
+ * afterwards).  Every scheduling step runs one iteration (1 heatbath + n_or
+ * overrelaxation sweeps, reunitarisation when reunit_every == 1, plaquette and
+ * Polyakov loop) of every resident job, jobs of one kind (N, storage, precision,
+ * n_or) together in one launch per (parity, mu), kinds on separate streams.  A job
+ * finishing frees its memory for the next pending one at the step boundary.  A
+ * job's results do not depend on what else is resident (bit-identical alone and in
+ * the mix).  No counterpart in the reference, which runs one lattice per process. */
+typedef struct {
+    int nc, dims[4];
+    double beta;
+    int storage, precision;
+    uint64_t seed;
+    int hot_start;      /* 1: Haar start from seed; 0: cold start                    */
+    int n_iter;         /* iterations (>= 1)                                          */
+    int n_or;           /* overrelaxation sweeps per iteration                        */
+    int reunit_every;   /* 0 or 1                                                      */
+} sun_job_desc;
+
+#define SUN_JOB_PENDING 0
+#define SUN_JOB_RUNNING 1
+#define SUN_JOB_DONE 2
+#define SUN_JOB_CANCELLED 3
+
+typedef struct {
+    uint64_t id;
+    int state, iters_done;
+    double t_submit, t_start, t_end;   /* seconds on the server clock                   */
+    double plaq[2], poly[2];           /* last iteration (done jobs)                     */
+} sun_job_status;
+
+typedef struct {
+    uint64_t jobs_done, link_updates, steps;
+    double busy_s;                     /* wall time spent inside scheduling steps        */
+    size_t budget, bytes_in_use;
+    int running, pending;
+} sun_server_counters;
+
+int sun_server_create(sun_ctx *ctx, size_t budget_bytes, int max_jobs, sun_server **out);
+int sun_server_destroy(sun_server *srv);
+/* SUN_ERR_BUDGET if the job alone exceeds the budget; SUN_ERR_UNSUPPORTED for a cold
+ * 8-real start or reunit_every > 1 */
+int sun_server_submit(sun_server *srv, const sun_job_desc *job, uint64_t *id);
+int sun_server_cancel(sun_server *srv, uint64_t id);
+int sun_server_step(sun_server *srv);                 /* one step on the calling thread  */
+int sun_server_start(sun_server *srv);                /* background scheduling thread    */
+int sun_server_stop(sun_server *srv);
+int sun_server_wait(sun_server *srv, uint64_t id, double timeout_s);  /* SUN_ERR_STATE on timeout */
+int sun_server_status(sun_server *srv, uint64_t id, sun_job_status *st);
+/* rows {plaq_spatial, plaq_temporal, poly_re, poly_im} of every iteration of a done job */
+int sun_server_observables(sun_server *srv, uint64_t id, double *out, int max_rows);
+int sun_server_release(sun_server *srv, uint64_t id);  /* forget a finished job          */
+int sun_server_stats(sun_server *srv, sun_server_counters *st);
+double sun_server_clock(sun_server *srv);
+
 /* ---- streams and events (timing helpers for callers without a HIP binding) ---------- */
 int sun_stream_create(void **stream);
 int sun_stream_destroy(void *stream);

@@ -1,0 +1,238 @@
+/*
+ * sun.hpp -- C++ facade over the C ABI (sun.h) with the class shapes of the
+ * reference's generation API (namespace CULQCD), so a CULQCD program such as
+ *
+ *     initCULQCD(0, SUMMARIZE, TUNE_YES);
+ *     SETPARAMS(false, 6.0, 32, 32, 32, 32, true);
+ *     gauge U(SOA12, Device, PARAMS::Volume * 4, true);
+ *     RNG rng; rng.Init(1234); U.Init(rng);
+ *     HeatBath<float> hb(U, rng); OverRelaxation<float> ovr(U);
+ *     Reunitarize<float> ru(U); PlaquetteCUB<float> plaq(U); OnePolyakovLoop<float> poly(U);
+ *     for (...) { hb.Run(); for (k < 4) ovr.Run(); ru.Run(); plaq.Run(); poly.Run(); }
+ *     U.Release(); EndCULQCD(0);
+ *
+ * compiles against this header after `using namespace sun::culqcd;`.
+ *
+ * Reference interfaces mirrored (file:line in the reference tree):
+ *   initCULQCD / EndCULQCD         include/comm_mpi.h:43-45
+ *   SETPARAMS, PARAMS::            include/constants.h:53-128
+ *   _gauge<Real> (gauge)           include/gaugearray.h:88-465
+ *   RNG                            include/random.h:42-101
+ *   HeatBath<Real>                 include/monte/monte.h:17-75
+ *   OverRelaxation<Real>           include/monte/ovr.h:20
+ *   Reunitarize<Real>              include/reunitarize.h:22-60
+ *   PlaquetteCUB<Real>             include/meas/plaquette.h:120-176
+ *   OnePolyakovLoop<Real>          include/meas/polyakovloop.h:15-73
+ *
+ * Differences, on purpose: errors throw sun::culqcd::Error (a std::runtime_error
+ * carrying sun_last_error()) instead of exiting; SETPARAMS only records the
+ * geometry and beta used by gauge objects created afterwards (a process may hold
+ * lattices of several geometries); N is a run-time property (NCOLORS below, default
+ * 3) instead of a compile-time macro; the RNG is counter-based, so RNG holds only
+ * the seed and HeatBath advances its own sweep counter.
+ */
+#ifndef SUN_HPP
+#define SUN_HPP
+
+#include <complex>
+#include <stdexcept>
+#include <string>
+
+#include "sun.h"
+
+namespace sun {
+namespace culqcd {
+
+class Error : public std::runtime_error {
+   public:
+    explicit Error(const std::string& what) : std::runtime_error(what) {}
+};
+
+inline void check(int rc, const char* what) {
+    if (rc != SUN_OK) throw Error(std::string(what) + ": " + sun_last_error());
+}
+
+typedef enum { SILENT = SUN_SILENT, SUMMARIZE = SUN_SUMMARIZE, VERBOSE = SUN_VERBOSE,
+               DEBUG_VERBOSE = SUN_DEBUG_VERBOSE } Verbosity;
+typedef enum { TUNE_NO = SUN_TUNE_NO, TUNE_YES = SUN_TUNE_YES } TuneMode;
+typedef enum { SOA = 0, SOA12 = 1, SOA12A = 2, SOA8 = 3 } ArrayType;  /* modes.h */
+typedef enum { Host, Device } ReadMode;
+
+/* process-wide context and current geometry (the reference's globals) */
+struct Runtime {
+    sun_ctx* ctx = nullptr;
+    int nc = 3;
+};
+inline Runtime& runtime() {
+    static Runtime r;
+    return r;
+}
+
+namespace PARAMS {
+inline double Beta = 0.0;
+inline int Grid[4] = {0, 0, 0, 0};
+inline int Volume = 0;
+inline int HalfVolume = 0;
+inline int size = 0;
+inline bool UseTex = false;
+}  // namespace PARAMS
+
+/* initCULQCD(gpuid, verbosity, tune)   include/comm_mpi.h:43 */
+inline void initCULQCD(int gpuid, Verbosity v = SILENT, TuneMode t = TUNE_NO) {
+    if (runtime().ctx) return;
+    check(sun_init(gpuid, (int)v, (int)t, &runtime().ctx), "initCULQCD");
+}
+/* EndCULQCD(status): releases the context; does not exit   include/comm_mpi.h:45 */
+inline void EndCULQCD(int /*status*/) {
+    if (runtime().ctx) {
+        sun_finalize(runtime().ctx);
+        runtime().ctx = nullptr;
+    }
+}
+/* number of colours of lattices created from now on (compile-time NCOLORS in the reference) */
+inline void SetNColors(int nc) { runtime().nc = nc; }
+
+/* SETPARAMS(usetex, beta, nx, ny, nz, nt, verbose)   include/constants.h:128 */
+inline void SETPARAMS(bool usetex, double beta, int nx, int ny, int nz, int nt, bool /*verbose*/ = false) {
+    PARAMS::UseTex = usetex;
+    PARAMS::Beta = beta;
+    PARAMS::Grid[0] = nx;
+    PARAMS::Grid[1] = ny;
+    PARAMS::Grid[2] = nz;
+    PARAMS::Grid[3] = nt;
+    PARAMS::Volume = nx * ny * nz * nt;
+    PARAMS::HalfVolume = PARAMS::Volume / 2;
+    PARAMS::size = 4 * PARAMS::Volume;
+}
+
+/* RNG: holds the seed; draws are a function of (seed, sweep, parity, mu, site) */
+class RNG {
+   public:
+    void Init(unsigned int seed) { seed_ = seed; }
+    void Release() {}
+    unsigned int& Seed() { return seed_; }
+
+   private:
+    unsigned int seed_ = 0;
+};
+
+/* _gauge<Real>: a shallow handle of one device link field, released explicitly */
+template <class Real>
+class _gauge {
+   public:
+    _gauge() {}
+    _gauge(ArrayType type, ReadMode mode, int size = 0, bool evenodd = true, bool border = false) {
+        (void)size;
+        (void)border;
+        if (mode != Device) throw Error("_gauge: only Device arrays are supported");
+        if (!evenodd) throw Error("_gauge: only even/odd ordered arrays are supported");
+        if (!runtime().ctx) throw Error("_gauge: call initCULQCD first");
+        sun_lattice_desc d;
+        d.nc = runtime().nc;
+        for (int i = 0; i < 4; i++) d.dims[i] = PARAMS::Grid[i];
+        d.beta = PARAMS::Beta;
+        d.storage = type == SOA12 ? SUN_STORE_12 : type == SOA8 ? SUN_STORE_8 : SUN_STORE_18;
+        if (type == SOA12A) throw Error("_gauge: SOA12A is not supported");
+        d.precision = (int)sizeof(Real);
+        check(sun_lattice_create(runtime().ctx, &d, &lat_), "_gauge::Allocate");
+        type_ = type;
+    }
+    void Release() {
+        if (lat_) sun_lattice_destroy(lat_);
+        lat_ = nullptr;
+    }
+    /* Init(): cold start; Init(RNG): (Haar) hot start    gaugearray.h:765-791 */
+    void Init() { check(sun_cold_start(lat_, nullptr), "_gauge::Init"); }
+    void Init(RNG& r) { check(sun_hot_start(lat_, r.Seed(), nullptr), "_gauge::Init(RNG)"); }
+    void Copy(_gauge<Real>& from) { check(sun_lattice_copy(lat_, from.lat_, nullptr), "_gauge::Copy"); }
+    ArrayType Type() const { return type_; }
+    bool EvenOdd() const { return true; }
+    int Size() const { return PARAMS::size; }
+    sun_lattice* handle() const { return lat_; }
+    /* canonical FP64 host copy: double[4][V][N][N][2] */
+    void ToHost(double* host) const { check(sun_links_to_host(lat_, host, SUN_LAYOUT_CANONICAL), "_gauge::ToHost"); }
+    void FromHost(const double* host) {
+        check(sun_links_from_host(lat_, host, SUN_LAYOUT_CANONICAL), "_gauge::FromHost");
+    }
+
+   private:
+    sun_lattice* lat_ = nullptr;
+    ArrayType type_ = SOA;
+};
+typedef _gauge<float> gauges;
+typedef _gauge<double> gauged;
+
+/* HeatBath<Real>(gauge, RNG).Run(): one sweep, then the sweep counter advances */
+template <class Real>
+class HeatBath {
+   public:
+    HeatBath(_gauge<Real>& U, RNG& r) : U_(U), rng_(r) {}
+    void Run(void* stream = nullptr) { check(sun_heatbath(U_.handle(), rng_.Seed(), sweep_++, stream), "HeatBath"); }
+    uint32_t Sweep() const { return sweep_; }
+    void SetSweep(uint32_t s) { sweep_ = s; }
+
+   private:
+    _gauge<Real>& U_;
+    RNG& rng_;
+    uint32_t sweep_ = 0;
+};
+
+template <class Real>
+class OverRelaxation {
+   public:
+    explicit OverRelaxation(_gauge<Real>& U) : U_(U) {}
+    void Run(void* stream = nullptr) { check(sun_overrelax(U_.handle(), stream), "OverRelaxation"); }
+
+   private:
+    _gauge<Real>& U_;
+};
+
+template <class Real>
+class Reunitarize {
+   public:
+    explicit Reunitarize(_gauge<Real>& U) : U_(U) {}
+    void Run(void* stream = nullptr) { check(sun_reunitarize(U_.handle(), stream), "Reunitarize"); }
+
+   private:
+    _gauge<Real>& U_;
+};
+
+/* PlaquetteCUB<Real>(gauge).Run() -> complex(spatial, temporal) */
+template <class Real>
+class PlaquetteCUB {
+   public:
+    explicit PlaquetteCUB(_gauge<Real>& U) : U_(U) {}
+    std::complex<double> Run(void* stream = nullptr) {
+        double out[2];
+        check(sun_plaquette(U_.handle(), stream, out), "PlaquetteCUB");
+        value_ = std::complex<double>(out[0], out[1]);
+        return value_;
+    }
+    std::complex<double> Value() const { return value_; }
+
+   private:
+    _gauge<Real>& U_;
+    std::complex<double> value_;
+};
+
+template <class Real>
+class OnePolyakovLoop {
+   public:
+    explicit OnePolyakovLoop(_gauge<Real>& U) : U_(U) {}
+    std::complex<double> Run(void* stream = nullptr) {
+        double out[2];
+        check(sun_polyakov(U_.handle(), stream, out), "OnePolyakovLoop");
+        value_ = std::complex<double>(out[0], out[1]);
+        return value_;
+    }
+    std::complex<double> Value() const { return value_; }
+
+   private:
+    _gauge<Real>& U_;
+    std::complex<double> value_;
+};
+
+}  // namespace culqcd
+}  // namespace sun
+
+#endif /* SUN_HPP */

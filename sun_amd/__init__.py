@@ -45,14 +45,24 @@ EXPORTS = [
     "sun_generate",
     "sun_batch_create", "sun_batch_destroy", "sun_batch_heatbath", "sun_batch_overrelax",
     "sun_batch_reunitarize", "sun_batch_plaquette", "sun_batch_generate",
+    "sun_server_create", "sun_server_destroy", "sun_server_submit", "sun_server_cancel",
+    "sun_server_step", "sun_server_start", "sun_server_stop", "sun_server_wait", "sun_server_status",
+    "sun_server_observables", "sun_server_release", "sun_server_stats", "sun_server_clock",
     "sun_stream_create", "sun_stream_destroy", "sun_stream_synchronize",
     "sun_event_create", "sun_event_destroy", "sun_event_record", "sun_event_elapsed_ms",
     "sun_device_name",
 ]
 
 
+ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_BUDGET, ERR_STATE = -1, -2, -3, -4, -5, -6
+
+
 class SunError(RuntimeError):
-    pass
+    """A failed call; `code` is the SUN_ERR_* status it returned."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 class LatticeDesc(ctypes.Structure):
@@ -64,6 +74,28 @@ class GenerateDesc(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("first_sweep", ctypes.c_uint32), ("n_iter", ctypes.c_int),
                 ("n_or", ctypes.c_int), ("reunit_every", ctypes.c_int), ("measure", ctypes.c_int),
                 ("use_graph", ctypes.c_int)]
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("nc", ctypes.c_int), ("dims", ctypes.c_int * 4), ("beta", ctypes.c_double),
+                ("storage", ctypes.c_int), ("precision", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("hot_start", ctypes.c_int), ("n_iter", ctypes.c_int), ("n_or", ctypes.c_int),
+                ("reunit_every", ctypes.c_int)]
+
+
+class JobStatus(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("state", ctypes.c_int), ("iters_done", ctypes.c_int),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
+                ("plaq", ctypes.c_double * 2), ("poly", ctypes.c_double * 2)]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [("jobs_done", ctypes.c_uint64), ("link_updates", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("busy_s", ctypes.c_double), ("budget", ctypes.c_size_t), ("bytes_in_use", ctypes.c_size_t),
+                ("running", ctypes.c_int), ("pending", ctypes.c_int)]
+
+
+JOB_PENDING, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
 
 
 def lib() -> ctypes.CDLL:
@@ -107,6 +139,19 @@ def lib() -> ctypes.CDLL:
         "sun_batch_reunitarize": (ci, [vp, vp]),
         "sun_batch_plaquette": (ci, [vp, vp, dp]),
         "sun_batch_generate": (ci, [vp, u32, ci, ci, ci, ci, vp, dp]),
+        "sun_server_create": (ci, [vp, ctypes.c_size_t, ci, ctypes.POINTER(vp)]),
+        "sun_server_destroy": (ci, [vp]),
+        "sun_server_submit": (ci, [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(u64)]),
+        "sun_server_cancel": (ci, [vp, u64]),
+        "sun_server_step": (ci, [vp]),
+        "sun_server_start": (ci, [vp]),
+        "sun_server_stop": (ci, [vp]),
+        "sun_server_wait": (ci, [vp, u64, cd]),
+        "sun_server_status": (ci, [vp, u64, ctypes.POINTER(JobStatus)]),
+        "sun_server_observables": (ci, [vp, u64, dp, ci]),
+        "sun_server_release": (ci, [vp, u64]),
+        "sun_server_stats": (ci, [vp, ctypes.POINTER(ServerCounters)]),
+        "sun_server_clock": (cd, [vp]),
         "sun_stream_create": (ci, [ctypes.POINTER(vp)]),
         "sun_stream_destroy": (ci, [vp]),
         "sun_stream_synchronize": (ci, [vp]),
@@ -127,7 +172,7 @@ def lib() -> ctypes.CDLL:
 def _check(rc: int, what: str):
     if rc != 0:
         msg = lib().sun_last_error().decode(errors="replace")
-        raise SunError(f"{what} failed ({rc}): {msg}")
+        raise SunError(f"{what} failed ({rc}): {msg}", rc)
 
 
 def _dp(a: np.ndarray):
@@ -291,6 +336,74 @@ class Batch:
         _check(lib().sun_batch_generate(self.h, first_sweep, n_iter, n_or, reunit_every, int(use_graph), stream,
                                         _dp(obs) if measure else None), "sun_batch_generate")
         return obs[:n_iter]
+
+
+class Server:
+    """Continuous-batching job server (sun_server_*): jobs are admitted while their
+    memory fits the HBM budget reserved at creation; each scheduling step runs one
+    iteration of every resident job."""
+
+    def __init__(self, ctx: Context, budget_bytes: int, max_jobs: int = 256):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        _check(lib().sun_server_create(ctx.h, int(budget_bytes), int(max_jobs), ctypes.byref(self.h)),
+               "sun_server_create")
+
+    def close(self):
+        if self.h:
+            _check(lib().sun_server_destroy(self.h), "sun_server_destroy")
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, nc: int, dims, beta: float, seed: int, n_iter: int, n_or: int = 4, reunit_every: int = 1,
+               storage: int = STORE_18, precision: int = F64, hot_start: bool = False) -> int:
+        d = JobDesc(nc, (ctypes.c_int * 4)(*dims), beta, storage, precision, seed, int(hot_start), n_iter, n_or,
+                    reunit_every)
+        jid = ctypes.c_uint64()
+        _check(lib().sun_server_submit(self.h, ctypes.byref(d), ctypes.byref(jid)), "sun_server_submit")
+        return int(jid.value)
+
+    def cancel(self, jid: int):
+        _check(lib().sun_server_cancel(self.h, jid), "sun_server_cancel")
+
+    def step(self):
+        _check(lib().sun_server_step(self.h), "sun_server_step")
+
+    def start(self):
+        _check(lib().sun_server_start(self.h), "sun_server_start")
+
+    def stop(self):
+        _check(lib().sun_server_stop(self.h), "sun_server_stop")
+
+    def wait(self, jid: int, timeout_s: float = 60.0):
+        _check(lib().sun_server_wait(self.h, jid, timeout_s), "sun_server_wait")
+
+    def status(self, jid: int) -> JobStatus:
+        st = JobStatus()
+        _check(lib().sun_server_status(self.h, jid, ctypes.byref(st)), "sun_server_status")
+        return st
+
+    def observables(self, jid: int, n_iter: int) -> np.ndarray:
+        """rows (plaq_spatial, plaq_temporal, poly_re, poly_im), one per iteration"""
+        out = np.zeros((max(n_iter, 1), 4))
+        _check(lib().sun_server_observables(self.h, jid, _dp(out), n_iter), "sun_server_observables")
+        return out[:n_iter]
+
+    def release(self, jid: int):
+        _check(lib().sun_server_release(self.h, jid), "sun_server_release")
+
+    def stats(self) -> ServerCounters:
+        st = ServerCounters()
+        _check(lib().sun_server_stats(self.h, ctypes.byref(st)), "sun_server_stats")
+        return st
+
+    def clock(self) -> float:
+        return float(lib().sun_server_clock(self.h))
 
 
 class Timer:

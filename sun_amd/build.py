@@ -58,9 +58,25 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_facade_demo()
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
     return LIB
+
+
+FACADE_DEMO = os.path.join(ROOT, "_build", "bin", "facade_demo")
+
+
+def build_facade_demo() -> str:
+    """tests/host/facade_demo.cpp: a CULQCD-style program against include/sun.hpp"""
+    os.makedirs(os.path.dirname(FACADE_DEMO), exist_ok=True)
+    src = os.path.join(ROOT, "tests", "host", "facade_demo.cpp")
+    cmd = ["g++", "-std=c++17", "-O2", f"-I{os.path.join(ROOT, 'include')}", src, f"-L{HERE}", "-lsun_hip",
+           "-Wl,-rpath,$ORIGIN/../../sun_amd", "-o", FACADE_DEMO]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade demo build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return FACADE_DEMO
 
 
 if __name__ == "__main__":

@@ -115,6 +115,19 @@ size_t sun::link_bytes(const sun_lattice_desc* d) {
     return (size_t)nreal * (size_t)d->precision;
 }
 
+static size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+static void set_planes(sun_lattice* L) {
+    const size_t nl = 4 * (size_t)L->g.V;
+    const size_t lb = link_bytes(&L->d);
+    const int nfull = (int)(lb / 16);
+    const int tail = (int)(lb % 16);
+    memset(&L->f, 0, sizeof(L->f));
+    char* base = (char*)L->buf;
+    for (int c = 0; c < nfull; c++) L->f.p[c] = base + (size_t)c * nl * 16;
+    if (tail) L->f.p[nfull] = base + (size_t)nfull * nl * 16;
+}
+
 extern "C" size_t sun_lattice_bytes(const sun_lattice_desc* d) {
     if (check_desc(d) != SUN_OK) return 0;
     size_t V = (size_t)d->dims[0] * d->dims[1] * d->dims[2] * d->dims[3];
@@ -145,10 +158,9 @@ int sun::lattice_alloc(sun_ctx* ctx, const sun_lattice_desc* d, sun_lattice** ou
         hipGetLastError();
         return fail(SUN_ERR_NOMEM, "hipMalloc(%zu bytes) for links: %s", nl * lb, hipGetErrorString(e));
     }
-    memset(&L->f, 0, sizeof(L->f));
-    char* base = (char*)L->buf;
-    for (int c = 0; c < nfull; c++) L->f.p[c] = base + (size_t)c * nl * 16;
-    if (tail) L->f.p[nfull] = base + (size_t)nfull * nl * 16;
+    (void)nfull;
+    (void)tail;
+    set_planes(L);
     L->npart = std::max(plaquette_blocks(L->g), polyakov_blocks(L->g));
     e = hipMalloc(&L->partials, sizeof(double2) * (size_t)L->npart);
     if (e == hipSuccess) e = hipMalloc(&L->dres, sizeof(double2) * 2);
@@ -172,8 +184,55 @@ int sun::lattice_alloc(sun_ctx* ctx, const sun_lattice_desc* d, sun_lattice** ou
     return SUN_OK;
 }
 
+size_t sun::lattice_footprint(const sun_lattice_desc* d, int n_hist) {
+    if (check_desc(d) != SUN_OK) return 0;
+    const Geom g = make_geom(d->dims);
+    const size_t field = 4 * (size_t)g.V * link_bytes(d);
+    const size_t part = sizeof(double2) * (size_t)std::max(plaquette_blocks(g), polyakov_blocks(g));
+    return align256(field) + align256(part) + align256(2 * sizeof(double2)) + align256(sizeof(uint32_t)) +
+           align256(sizeof(double2) * 2 * (size_t)(n_hist > 0 ? n_hist : 1));
+}
+
+int sun::lattice_in_memory(sun_ctx* ctx, const sun_lattice_desc* d, char* mem, int n_hist, sun_lattice** out) {
+    int rc = check_desc(d);
+    if (rc) return rc;
+    const OpsTable* ops = find_ops(d->nc, d->storage, d->precision);
+    if (!ops) return fail(SUN_ERR_UNSUPPORTED, "no kernels for nc=%d storage=%d precision=%d", d->nc, d->storage,
+                          d->precision);
+    sun_lattice* L = new (std::nothrow) sun_lattice();
+    if (!L) return fail(SUN_ERR_NOMEM, "host allocation");
+    L->ctx = ctx;
+    L->d = *d;
+    L->ops = ops;
+    L->g = make_geom(d->dims);
+    L->external = true;
+    const size_t field = 4 * (size_t)L->g.V * link_bytes(d);
+    L->bytes = field;
+    L->npart = std::max(plaquette_blocks(L->g), polyakov_blocks(L->g));
+    char* p = mem;
+    L->buf = p;
+    p += align256(field);
+    L->partials = (double2*)p;
+    p += align256(sizeof(double2) * (size_t)L->npart);
+    L->dres = (double2*)p;
+    p += align256(2 * sizeof(double2));
+    L->sweep_dev = (uint32_t*)p;
+    p += align256(sizeof(uint32_t));
+    L->hist = (double2*)p;
+    set_planes(L);
+    L->cfg_hb = ctx->tuner.lookup("heatbath", *d);
+    L->cfg_or = ctx->tuner.lookup("overrelax", *d);
+    L->tuned = true;  // server lattices never tune: tuning needs a scratch copy outside the budget
+    *out = L;
+    return SUN_OK;
+}
+
 void sun::lattice_free(sun_lattice* L) {
     if (!L) return;
+    if (L->external) {
+        delete L;
+        return;
+    }
     if (L->ctx) hipSetDevice(L->ctx->device);
     if (L->buf) hipFree(L->buf);
     if (L->partials) hipFree(L->partials);

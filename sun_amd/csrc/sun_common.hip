@@ -45,12 +45,21 @@ __global__ void __launch_bounds__(kReduceBlock) k_finalize(const double2* partia
     if (threadIdx.x == 0) out[0] = make_double2(v.x * s0, v.y * s1);
 }
 
-__global__ void __launch_bounds__(kReduceBlock) k_batch_finalize(const double2* partials, const int* base,
-                                                                 const int* count, const double2* scales,
-                                                                 double2* out) {
-    const int i = blockIdx.x;
-    double2 v = ordered_sum(partials + base[i], count[i]);
-    if (threadIdx.x == 0) out[i] = make_double2(v.x * scales[i].x, v.y * scales[i].y);
+__global__ void __launch_bounds__(kReduceBlock) k_batch_finalize(const BatchEntry* table, double2* out) {
+    const BatchEntry& E = table[blockIdx.x];
+    double2 v = ordered_sum(E.partials, E.nblocks);
+    if (threadIdx.x == 0) out[blockIdx.x] = make_double2(v.x * E.scale, v.y * E.scale);
+}
+
+__global__ void k_batch_record(BatchEntry* table, int n, const double2* plaq, const double2* poly) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    BatchEntry& E = table[i];
+    if (E.hist) {
+        E.hist[2 * (size_t)E.hist_iter] = plaq[i];
+        E.hist[2 * (size_t)E.hist_iter + 1] = poly ? poly[i] : make_double2(0.0, 0.0);
+    }
+    E.hist_iter += 1u;
 }
 
 __global__ void k_advance_counter(uint32_t* ctr) {
@@ -91,10 +100,16 @@ hipError_t launch_finalize(const double2* partials, int n, double s0, double s1,
     return hipGetLastError();
 }
 
-hipError_t launch_batch_finalize(const double2* partials, const int* base, const int* count, const double2* scales,
-                                 int nlat, double2* out, hipStream_t s) {
+hipError_t launch_batch_finalize(const BatchEntry* table, int nlat, double2* out, hipStream_t s) {
     if (nlat <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_batch_finalize, dim3(nlat), dim3(kReduceBlock), 0, s, partials, base, count, scales, out);
+    hipLaunchKernelGGL(k_batch_finalize, dim3(nlat), dim3(kReduceBlock), 0, s, table, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_record(BatchEntry* table, int nlat, const double2* plaq, const double2* poly,
+                               hipStream_t s) {
+    if (nlat <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_record, dim3((nlat + 255) / 256), dim3(256), 0, s, table, nlat, plaq, poly);
     return hipGetLastError();
 }
 

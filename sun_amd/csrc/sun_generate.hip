@@ -125,9 +125,6 @@ static void batch_free(sun_batch* b) {
     if (b->t_reunit) hipFree(b->t_reunit);
     if (b->t_plaq) hipFree(b->t_plaq);
     if (b->partials) hipFree(b->partials);
-    if (b->d_base) hipFree(b->d_base);
-    if (b->d_count) hipFree(b->d_count);
-    if (b->d_scales) hipFree(b->d_scales);
     if (b->d_out) hipFree(b->d_out);
     if (b->h_out) hipHostFree(b->h_out);
     if (b->obs_dev) hipFree(b->obs_dev);
@@ -153,8 +150,6 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
     b->ops = ops;
     b->lats.assign(lats, lats + n);
     std::vector<BatchEntry> tu(n), tr(n), tp(n);
-    std::vector<int> base(n), count(n);
-    std::vector<double2> scales(n);
     int cu = 0, cr = 0, cp = 0;
     for (int i = 0; i < n; i++) {
         sun_lattice* L = lats[i];
@@ -174,12 +169,8 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
         cr += r.nblocks;
         p.first_block = cp;
         p.nblocks = plaquette_blocks(L->g);
-        p.partial_base = cp;
-        base[i] = cp;
-        count[i] = p.nblocks;
+        p.scale = 1.0 / ((double)(3 * L->d.nc) * (double)L->g.V);  // plaquette_cub.cu:234
         cp += p.nblocks;
-        const double norm = 1.0 / ((double)(3 * L->d.nc) * (double)L->g.V);
-        scales[i] = make_double2(norm, norm);
         tu[i] = a;
         tr[i] = r;
         tp[i] = p;
@@ -193,18 +184,14 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
     if (e == hipSuccess) e = hipMalloc(&b->t_reunit, sizeof(BatchEntry) * n);
     if (e == hipSuccess) e = hipMalloc(&b->t_plaq, sizeof(BatchEntry) * n);
     if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double2) * (size_t)cp);
-    if (e == hipSuccess) e = hipMalloc(&b->d_base, sizeof(int) * n);
-    if (e == hipSuccess) e = hipMalloc(&b->d_count, sizeof(int) * n);
-    if (e == hipSuccess) e = hipMalloc(&b->d_scales, sizeof(double2) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_out, sizeof(double2) * n);
+    if (e == hipSuccess)
+        for (int i = 0; i < n; i++) tp[i].partials = b->partials + tp[i].first_block;
     if (e == hipSuccess) e = hipMalloc(&b->iter_dev, sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(&b->h_out, sizeof(double2) * n, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemcpy(b->t_update, tu.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->t_reunit, tr.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->t_plaq, tp.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->d_base, base.data(), sizeof(int) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->d_count, count.data(), sizeof(int) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->d_scales, scales.data(), sizeof(double2) * n, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         hipGetLastError();
         batch_free(b);
@@ -251,8 +238,8 @@ static int batch_or(sun_batch* b, hipStream_t s) {
 
 static int batch_plaq(sun_batch* b, hipStream_t s) {
     const int n = (int)b->lats.size();
-    HIP_TRY(b->ops->batch_plaquette_partial(b->t_plaq, n, b->blocks_plaq, b->partials, s));
-    HIP_TRY(launch_batch_finalize(b->partials, b->d_base, b->d_count, b->d_scales, n, b->d_out, s));
+    HIP_TRY(b->ops->batch_plaquette_partial(b->t_plaq, n, b->blocks_plaq, s));
+    HIP_TRY(launch_batch_finalize(b->t_plaq, n, b->d_out, s));
     return SUN_OK;
 }
 

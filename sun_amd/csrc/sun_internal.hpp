@@ -95,6 +95,8 @@ struct sun_lattice {
     int obs_cap = 0;
     sun::LaunchCfg cfg_hb, cfg_or;
     bool tuned = false;
+    bool external = false;         // memory belongs to a server arena: never hipFree'd
+    double2* hist = nullptr;       // observable history (server jobs), in the arena
 };
 
 struct sun_batch {
@@ -108,9 +110,6 @@ struct sun_batch {
     int blocks_update = 0, blocks_reunit = 0, blocks_plaq = 0;
     std::vector<sun::BatchEntry> h_update;
     double2* partials = nullptr;
-    int* d_base = nullptr;
-    int* d_count = nullptr;
-    double2* d_scales = nullptr;
     double2* d_out = nullptr;
     double2* h_out = nullptr;
     double2* obs_dev = nullptr;
@@ -122,6 +121,12 @@ namespace sun {
 int check_desc(const sun_lattice_desc* d);
 size_t link_bytes(const sun_lattice_desc* d);
 int lattice_alloc(sun_ctx* ctx, const sun_lattice_desc* d, sun_lattice** out);
+// bytes of a lattice placed in caller memory: link field, reduction scratch,
+// result slots, sweep counter and an n_hist-row observable history
+size_t lattice_footprint(const sun_lattice_desc* d, int n_hist);
+// a lattice whose every device buffer is carved from mem (lattice_footprint bytes,
+// 256-byte aligned); lattice_free then only releases the host structure
+int lattice_in_memory(sun_ctx* ctx, const sun_lattice_desc* d, char* mem, int n_hist, sun_lattice** out);
 void lattice_free(sun_lattice* L);
 hipStream_t pick_stream(sun_lattice* lat, void* stream);
 int heatbath_sweep(sun_lattice* lat, uint64_t seed, uint32_t sweep, const uint32_t* sweep_dev, hipStream_t s);

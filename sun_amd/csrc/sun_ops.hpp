@@ -19,10 +19,29 @@ struct LaunchCfg {
     // flight per XCD, which decides whether neighbour links are still in L2 when
     // the next staple direction needs them
     int lds = 0;
+    // x3 layers per workgroup (1 = linear mapping).  With t3 > 1 the workgroup has
+    // 64*t3 threads; wave w of it takes 64 consecutive sites of x3-layer
+    // (x3 group * t3 + w), so x3 neighbours share the workgroup.
+    int t3 = 1;
 };
 
+// site id (within a parity) of thread t of workgroup b under the mapping of cfg.t3;
+// slab = sites of one parity in one x3 layer
+SUN_HD int tiled_site(int b, int t, int t3, int slab) {
+    const int groups = slab >> 6;
+    const int rg = b % groups, x3g = b / groups;
+    return (x3g * t3 + (t >> 6)) * slab + (rg << 6) + (t & 63);
+}
+
+inline bool t3_valid(const Geom& g, int t3) {
+    const int slab = g.X[0] * g.X[1] * g.X[2] / 2;
+    return t3 >= 1 && t3 <= 16 && (t3 == 1 || ((slab % 64) == 0 && (g.X[3] % t3) == 0));
+}
+
 // One lattice of a batched launch.  `first_block` is the prefix sum of the blocks
-// of the preceding lattices; a block finds its lattice by binary search.
+// of the preceding lattices; a block finds its lattice by binary search.  Each
+// kind of batched launch (update, reunitarisation, plaquette, Polyakov loop) has
+// its own table because the blocks per lattice differ.
 struct BatchEntry {
     Field f;
     Geom g;
@@ -31,7 +50,10 @@ struct BatchEntry {
     uint32_t sweep;      // heatbath sweep counter of this lattice
     int first_block;
     int nblocks;
-    int partial_base;    // first slot of this lattice in the reduction scratch
+    double2* partials;   // this lattice's per-block reduction scratch (nblocks entries)
+    double scale;        // normalisation of the reduced sum
+    double2* hist;       // observable history: {plaquette, polyakov} per iteration
+    uint32_t hist_iter;  // next history row
 };
 
 struct OpsTable {
@@ -53,8 +75,9 @@ struct OpsTable {
     hipError_t (*batch_update)(const BatchEntry* dev_table, int nlat, int total_blocks, int parity,
                                int mu, bool hb, hipStream_t);
     hipError_t (*batch_reunitarize)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
-    hipError_t (*batch_plaquette_partial)(const BatchEntry* dev_table, int nlat, int total_blocks,
-                                          double2* partials, hipStream_t);
+    hipError_t (*batch_plaquette_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
+    hipError_t (*batch_polyakov_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
+    int max_block;   // largest workgroup the update kernels are compiled for
 };
 
 constexpr int kReduceBlock = 256;   // threads of every reduction block
@@ -82,9 +105,12 @@ inline int link_blocks(const Geom& g) { return (4 * g.V + kReduceBlock - 1) / kR
 // sum of partials[0..n) in a fixed order, times `scale`, into out (device)
 hipError_t launch_finalize(const double2* partials, int n, double scale0, double scale1, double2* out,
                            hipStream_t s);
-// finalize for every lattice of a batch: out[i] = sum of its partials * scales[i]
-hipError_t launch_batch_finalize(const double2* partials, const int* base, const int* count,
-                                 const double2* scales, int nlat, double2* out, hipStream_t s);
+// finalize for every lattice of a batch table: out[i] = (sum of entry i's partials)
+// * entry i's scale, in the same order as launch_finalize
+hipError_t launch_batch_finalize(const BatchEntry* table, int nlat, double2* out, hipStream_t s);
+// entry i's history row hist_iter <- {plaq[i], poly[i]}; hist_iter += 1
+hipError_t launch_batch_record(BatchEntry* table, int nlat, const double2* plaq, const double2* poly,
+                               hipStream_t s);
 
 const OpsTable* ops_n2_f18_r4();
 const OpsTable* ops_n2_f18_r8();

@@ -1,0 +1,540 @@
+// sun_server.hip -- continuous-batching server for ensemble-generation jobs.
+//
+// Admission: a job is admitted when its link field and scratch (lattice_footprint)
+// fit in the free part of the HBM budget, which is reserved once at creation and
+// sub-allocated by an Arena, so no hipMalloc/hipFree happens while jobs run.
+// Pending jobs are taken in arrival order; a job that does not fit is passed over
+// for later, smaller ones ("backfill") unless it has already been passed over
+// kMaxPassOver times, in which case admission stops until it fits.
+//
+// Execution: one scheduling step advances every resident job by one iteration.
+// Resident jobs are grouped by kind (kernel variant, n_or, reunit); a group runs its
+// iteration with the batched kernels -- one launch per (parity, mu) for all its
+// lattices -- on its own stream, so groups overlap on the device.  A job's sweep
+// counter and observable history live in its batch-table entries on the device and
+// are advanced by the kernels themselves.  At the end of the step the host retires
+// finished jobs (copies their history out, frees their memory) and admits new ones.
+//
+// A job's arithmetic depends only on its own lattice, seed and sweep number (the
+// per-site update and the per-lattice fixed-order reductions are the same code
+// alone and batched), so its results are bit-identical alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+
+#include "sun_arena.hpp"
+#include "sun_internal.hpp"
+
+using namespace sun;
+
+namespace {
+
+constexpr int kMaxPassOver = 8;
+
+struct Group;
+
+struct Job {
+    uint64_t id = 0;
+    sun_job_desc d{};
+    int state = SUN_JOB_PENDING;
+    int iters_done = 0;
+    int passed_over = 0;
+    bool cancel = false;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    size_t off = SIZE_MAX, bytes = 0;
+    sun_lattice* lat = nullptr;
+    Group* group = nullptr;
+    std::vector<double2> hist;  // host copy of the history once done
+};
+
+struct Group {
+    const OpsTable* ops = nullptr;
+    int n_or = 0, reunit = 0;
+    hipStream_t stream = nullptr;
+    int cap = 0;
+    BatchEntry *t_update = nullptr, *t_reunit = nullptr, *t_plaq = nullptr, *t_poly = nullptr;
+    double2 *d_plaq = nullptr, *d_poly = nullptr;
+    std::vector<Job*> members;
+    bool dirty = true;
+    int blocks_update = 0, blocks_reunit = 0, blocks_plaq = 0, blocks_poly = 0;
+};
+
+double now_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct sun_server {
+    sun_ctx* ctx = nullptr;
+    size_t budget = 0;
+    int max_jobs = 0;
+    char* base = nullptr;
+    Arena arena;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Job*> pending;
+    std::map<uint64_t, Job*> jobs;
+    std::vector<Job*> running;
+    std::vector<Group*> groups;
+    std::thread worker;
+    bool stop = false, worker_on = false;
+    uint64_t next_id = 1;
+    std::chrono::steady_clock::time_point t0;
+    uint64_t jobs_done = 0, link_updates = 0, steps = 0;
+    double busy_s = 0;
+    std::mutex step_mu;  // one scheduling step at a time
+};
+
+namespace {
+
+void free_group(Group* g) {
+    if (g->stream) hipStreamDestroy(g->stream);
+    hipFree(g->t_update);
+    hipFree(g->t_reunit);
+    hipFree(g->t_plaq);
+    hipFree(g->t_poly);
+    hipFree(g->d_plaq);
+    hipFree(g->d_poly);
+    delete g;
+}
+
+int make_group(sun_server* S, const OpsTable* ops, int n_or, int reunit, Group** out) {
+    Group* g = new (std::nothrow) Group();
+    if (!g) return fail(SUN_ERR_NOMEM, "server: host allocation");
+    g->ops = ops;
+    g->n_or = n_or;
+    g->reunit = reunit;
+    g->cap = S->max_jobs;
+    const size_t tb = sizeof(BatchEntry) * (size_t)g->cap;
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&g->t_update, tb);
+    if (e == hipSuccess) e = hipMalloc(&g->t_reunit, tb);
+    if (e == hipSuccess) e = hipMalloc(&g->t_plaq, tb);
+    if (e == hipSuccess) e = hipMalloc(&g->t_poly, tb);
+    if (e == hipSuccess) e = hipMalloc(&g->d_plaq, sizeof(double2) * (size_t)g->cap);
+    if (e == hipSuccess) e = hipMalloc(&g->d_poly, sizeof(double2) * (size_t)g->cap);
+    if (e != hipSuccess) {
+        hipGetLastError();
+        free_group(g);
+        return fail(SUN_ERR_NOMEM, "server: group tables: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return SUN_OK;
+}
+
+// device tables of a group from its members' host state
+int rebuild(Group* g) {
+    const int n = (int)g->members.size();
+    std::vector<BatchEntry> tu(n), tr(n), tp(n), tl(n);
+    int cu = 0, cr = 0, cp = 0, cl = 0;
+    for (int i = 0; i < n; i++) {
+        Job* J = g->members[i];
+        sun_lattice* L = J->lat;
+        BatchEntry e;
+        memset(&e, 0, sizeof(e));
+        e.f = L->f;
+        e.g = L->g;
+        e.seed = J->d.seed;
+        e.bnc = L->d.beta / L->d.nc;
+        e.sweep = (uint32_t)J->iters_done;
+        e.hist = L->hist;
+        e.hist_iter = (uint32_t)J->iters_done;
+        BatchEntry a = e, r = e, p = e, q = e;
+        a.first_block = cu;
+        a.nblocks = update_blocks(L->g, 256);
+        cu += a.nblocks;
+        r.first_block = cr;
+        r.nblocks = link_blocks(L->g);
+        cr += r.nblocks;
+        p.first_block = cp;
+        p.nblocks = plaquette_blocks(L->g);
+        p.partials = L->partials;
+        p.scale = 1.0 / ((double)(3 * L->d.nc) * (double)L->g.V);
+        cp += p.nblocks;
+        q.first_block = cl;
+        q.nblocks = polyakov_blocks(L->g);
+        q.partials = L->partials;  // reused: the Polyakov launch follows the plaquette finalize
+        q.scale = 1.0 / ((double)L->d.nc * (double)L->g.X[0] * L->g.X[1] * L->g.X[2]);
+        cl += q.nblocks;
+        tu[i] = a;
+        tr[i] = r;
+        tp[i] = p;
+        tl[i] = q;
+    }
+    g->blocks_update = cu;
+    g->blocks_reunit = cr;
+    g->blocks_plaq = cp;
+    g->blocks_poly = cl;
+    const size_t b = sizeof(BatchEntry) * (size_t)n;
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(g->t_update, tu.data(), b, hipMemcpyHostToDevice, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->t_reunit, tr.data(), b, hipMemcpyHostToDevice, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->t_plaq, tp.data(), b, hipMemcpyHostToDevice, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->t_poly, tl.data(), b, hipMemcpyHostToDevice, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));  // host vectors go out of scope
+    }
+    g->dirty = false;
+    return SUN_OK;
+}
+
+// one iteration of every member of a group, enqueued on the group's stream
+int group_iteration(Group* g) {
+    const int n = (int)g->members.size();
+    if (!n) return SUN_OK;
+    hipStream_t s = g->stream;
+    for (int parity = 0; parity < 2; parity++)
+        for (int mu = 0; mu < 4; mu++) HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, true, s));
+    for (int k = 0; k < g->n_or; k++)
+        for (int parity = 0; parity < 2; parity++)
+            for (int mu = 0; mu < 4; mu++)
+                HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, false, s));
+    if (g->reunit) HIP_TRY(g->ops->batch_reunitarize(g->t_reunit, n, g->blocks_reunit, s));
+    HIP_TRY(g->ops->batch_plaquette_partial(g->t_plaq, n, g->blocks_plaq, s));
+    HIP_TRY(launch_batch_finalize(g->t_plaq, n, g->d_plaq, s));
+    HIP_TRY(g->ops->batch_polyakov_partial(g->t_poly, n, g->blocks_poly, s));
+    HIP_TRY(launch_batch_finalize(g->t_poly, n, g->d_poly, s));
+    HIP_TRY(launch_batch_record(g->t_plaq, n, g->d_plaq, g->d_poly, s));
+    HIP_TRY(launch_batch_advance(g->t_update, n, s));
+    return SUN_OK;
+}
+
+Group* find_group(sun_server* S, const Job* J) {
+    for (Group* g : S->groups)
+        if (g->ops == J->lat->ops && g->n_or == J->d.n_or && g->reunit == J->d.reunit_every) return g;
+    return nullptr;
+}
+
+// admission, under S->mu
+int admit(sun_server* S) {
+    bool head_blocked = false;
+    for (auto it = S->pending.begin(); it != S->pending.end();) {
+        Job* J = *it;
+        if (J->cancel) {
+            J->state = SUN_JOB_CANCELLED;
+            it = S->pending.erase(it);
+            continue;
+        }
+        if ((int)S->running.size() >= S->max_jobs) break;
+        const size_t off = S->arena.alloc(J->bytes);
+        if (off == SIZE_MAX) {
+            if (++J->passed_over > kMaxPassOver && it == S->pending.begin()) head_blocked = true;
+            if (head_blocked) break;
+            ++it;
+            continue;
+        }
+        sun_lattice* L = nullptr;
+        sun_lattice_desc d = {J->d.nc, {J->d.dims[0], J->d.dims[1], J->d.dims[2], J->d.dims[3]}, J->d.beta,
+                              J->d.storage, J->d.precision};
+        int rc = lattice_in_memory(S->ctx, &d, S->base + off, J->d.n_iter, &L);
+        if (rc != SUN_OK) {
+            S->arena.release(off);
+            return rc;
+        }
+        J->off = off;
+        J->lat = L;
+        Group* g = find_group(S, J);
+        if (!g) {
+            rc = make_group(S, L->ops, J->d.n_or, J->d.reunit_every, &g);
+            if (rc != SUN_OK) {
+                lattice_free(L);
+                S->arena.release(off);
+                return rc;
+            }
+            S->groups.push_back(g);
+        }
+        J->group = g;
+        hipError_t e = J->d.hot_start ? L->ops->hot_start(L->f, L->g, J->d.seed, g->stream)
+                                      : L->ops->cold_start(L->f, 4 * L->g.V, g->stream);
+        if (e != hipSuccess) return fail(SUN_ERR_HIP, "server: start of job %llu: %s", (unsigned long long)J->id,
+                                         hipGetErrorString(e));
+        g->members.push_back(J);
+        g->dirty = true;
+        J->state = SUN_JOB_RUNNING;
+        J->t_start = now_since(S->t0);
+        S->running.push_back(J);
+        it = S->pending.erase(it);
+    }
+    return SUN_OK;
+}
+
+int retire(sun_server* S, Job* J, bool done) {
+    Group* g = J->group;
+    if (done) {
+        J->hist.resize((size_t)2 * J->d.n_iter);
+        HIP_TRY(hipMemcpy(J->hist.data(), J->lat->hist, sizeof(double2) * J->hist.size(), hipMemcpyDeviceToHost));
+    }
+    g->members.erase(std::find(g->members.begin(), g->members.end(), J));
+    g->dirty = true;
+    lattice_free(J->lat);
+    J->lat = nullptr;
+    S->arena.release(J->off);
+    J->off = SIZE_MAX;
+    J->state = done ? SUN_JOB_DONE : SUN_JOB_CANCELLED;
+    J->t_end = now_since(S->t0);
+    return SUN_OK;
+}
+
+int step(sun_server* S) {
+    std::lock_guard<std::mutex> sl(S->step_mu);
+    const double t_begin = now_since(S->t0);
+    std::vector<Group*> active;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        SUN_TRY(admit(S));
+        for (Group* g : S->groups)
+            if (!g->members.empty()) active.push_back(g);
+    }
+    if (active.empty()) return SUN_OK;
+    for (Group* g : active) {
+        if (g->dirty) SUN_TRY(rebuild(g));
+        SUN_TRY(group_iteration(g));
+    }
+    for (Group* g : active) HIP_TRY(hipStreamSynchronize(g->stream));
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (auto it = S->running.begin(); it != S->running.end();) {
+        Job* J = *it;
+        J->iters_done++;
+        const sun_lattice* L = J->lat;
+        S->link_updates += (uint64_t)(1 + J->d.n_or) * 4ull * (uint64_t)L->g.V;
+        const bool done = J->iters_done >= J->d.n_iter;
+        if (done || J->cancel) {
+            SUN_TRY(retire(S, J, done));
+            if (done) S->jobs_done++;
+            it = S->running.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    S->steps++;
+    S->busy_s += now_since(S->t0) - t_begin;
+    S->cv.notify_all();
+    return SUN_OK;
+}
+
+void worker_loop(sun_server* S) {
+    hipSetDevice(S->ctx->device);
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->running.empty(); });
+            if (S->stop) return;
+        }
+        if (step(S) != SUN_OK) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->stop = true;
+            S->cv.notify_all();
+            return;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int sun_server_create(sun_ctx* ctx, size_t budget, int max_jobs, sun_server** out) {
+    if (!ctx || !out || budget < 1024 * 1024 || max_jobs < 1 || max_jobs > 1024)
+        return fail(SUN_ERR_INVALID, "sun_server_create: bad argument (budget >= 1 MiB, 1 <= max_jobs <= 1024)");
+    *out = nullptr;
+    sun_server* S = new (std::nothrow) sun_server();
+    if (!S) return fail(SUN_ERR_NOMEM, "host allocation");
+    S->ctx = ctx;
+    S->budget = budget;
+    S->max_jobs = max_jobs;
+    HIP_TRY_CLEANUP(hipSetDevice(ctx->device), delete S);
+    hipError_t e = hipMalloc(&S->base, budget);
+    if (e != hipSuccess) {
+        hipGetLastError();
+        delete S;
+        return fail(SUN_ERR_NOMEM, "sun_server_create: reserving %zu bytes: %s", budget, hipGetErrorString(e));
+    }
+    S->arena.reset(budget);
+    S->t0 = std::chrono::steady_clock::now();
+    *out = S;
+    return SUN_OK;
+}
+
+extern "C" int sun_server_stop(sun_server* S) {
+    if (!S) return fail(SUN_ERR_INVALID, "sun_server_stop: NULL");
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        S->cv.notify_all();
+    }
+    if (S->worker.joinable()) S->worker.join();
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->worker_on = false;
+    S->stop = false;
+    return SUN_OK;
+}
+
+extern "C" int sun_server_destroy(sun_server* S) {
+    if (!S) return fail(SUN_ERR_INVALID, "sun_server_destroy: NULL");
+    sun_server_stop(S);
+    hipSetDevice(S->ctx->device);
+    for (Group* g : S->groups) {
+        hipStreamSynchronize(g->stream);
+        free_group(g);
+    }
+    for (auto& kv : S->jobs) {
+        if (kv.second->lat) lattice_free(kv.second->lat);
+        delete kv.second;
+    }
+    hipFree(S->base);
+    delete S;
+    return SUN_OK;
+}
+
+extern "C" int sun_server_submit(sun_server* S, const sun_job_desc* jd, uint64_t* id) {
+    if (!S || !jd || !id) return fail(SUN_ERR_INVALID, "sun_server_submit: NULL argument");
+    sun_lattice_desc d = {jd->nc, {jd->dims[0], jd->dims[1], jd->dims[2], jd->dims[3]}, jd->beta, jd->storage,
+                          jd->precision};
+    SUN_TRY(check_desc(&d));
+    if (jd->n_iter < 1 || jd->n_or < 0) return fail(SUN_ERR_INVALID, "job: n_iter >= 1 and n_or >= 0 required");
+    if (jd->reunit_every != 0 && jd->reunit_every != 1)
+        return fail(SUN_ERR_UNSUPPORTED, "job: reunit_every must be 0 or 1 in the server");
+    if (!jd->hot_start && jd->storage == SUN_STORE_8)
+        return fail(SUN_ERR_UNSUPPORTED, "job: 8-real storage cannot hold a cold start (gaugearray.cu:293-297)");
+    const size_t bytes = Arena::round_up(lattice_footprint(&d, jd->n_iter));
+    if (bytes > S->arena.capacity())
+        return fail(SUN_ERR_BUDGET, "job needs %zu bytes, the server budget is %zu", bytes, S->arena.capacity());
+    Job* J = new (std::nothrow) Job();
+    if (!J) return fail(SUN_ERR_NOMEM, "host allocation");
+    J->d = *jd;
+    J->bytes = bytes;
+    std::lock_guard<std::mutex> lk(S->mu);
+    J->id = S->next_id++;
+    J->t_submit = now_since(S->t0);
+    S->jobs[J->id] = J;
+    S->pending.push_back(J);
+    *id = J->id;
+    S->cv.notify_all();
+    return SUN_OK;
+}
+
+extern "C" int sun_server_cancel(sun_server* S, uint64_t id) {
+    if (!S) return fail(SUN_ERR_INVALID, "sun_server_cancel: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(id);
+    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
+    Job* J = it->second;
+    if (J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED) return SUN_OK;
+    J->cancel = true;
+    if (J->state == SUN_JOB_PENDING) {
+        S->pending.erase(std::find(S->pending.begin(), S->pending.end(), J));
+        J->state = SUN_JOB_CANCELLED;
+        J->t_end = now_since(S->t0);
+        S->cv.notify_all();
+    }
+    return SUN_OK;
+}
+
+extern "C" int sun_server_step(sun_server* S) {
+    if (!S) return fail(SUN_ERR_INVALID, "sun_server_step: NULL");
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->worker_on) return fail(SUN_ERR_STATE, "sun_server_step: the scheduling thread is running");
+    }
+    hipSetDevice(S->ctx->device);
+    return step(S);
+}
+
+extern "C" int sun_server_start(sun_server* S) {
+    if (!S) return fail(SUN_ERR_INVALID, "sun_server_start: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->worker_on) return SUN_OK;
+    S->stop = false;
+    S->worker_on = true;
+    S->worker = std::thread(worker_loop, S);
+    return SUN_OK;
+}
+
+extern "C" int sun_server_wait(sun_server* S, uint64_t id, double timeout_s) {
+    if (!S) return fail(SUN_ERR_INVALID, "sun_server_wait: NULL");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(id);
+    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
+    Job* J = it->second;
+    auto finished = [&] { return J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED || S->stop; };
+    if (!S->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), finished))
+        return fail(SUN_ERR_STATE, "job %llu not finished after %.1f s", (unsigned long long)id, timeout_s);
+    if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED)
+        return fail(SUN_ERR_STATE, "the scheduling thread stopped before job %llu finished: %s",
+                    (unsigned long long)id, sun_last_error());
+    return SUN_OK;
+}
+
+extern "C" int sun_server_status(sun_server* S, uint64_t id, sun_job_status* st) {
+    if (!S || !st) return fail(SUN_ERR_INVALID, "sun_server_status: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(id);
+    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
+    const Job* J = it->second;
+    memset(st, 0, sizeof(*st));
+    st->id = J->id;
+    st->state = J->state;
+    st->iters_done = J->iters_done;
+    st->t_submit = J->t_submit;
+    st->t_start = J->t_start;
+    st->t_end = J->t_end;
+    if (J->state == SUN_JOB_DONE && !J->hist.empty()) {
+        const size_t last = J->hist.size() - 2;
+        st->plaq[0] = J->hist[last].x;
+        st->plaq[1] = J->hist[last].y;
+        st->poly[0] = J->hist[last + 1].x;
+        st->poly[1] = J->hist[last + 1].y;
+    }
+    return SUN_OK;
+}
+
+extern "C" int sun_server_observables(sun_server* S, uint64_t id, double* out, int max_rows) {
+    if (!S || !out) return fail(SUN_ERR_INVALID, "sun_server_observables: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(id);
+    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
+    const Job* J = it->second;
+    if (J->state != SUN_JOB_DONE) return fail(SUN_ERR_STATE, "job %llu is not done", (unsigned long long)id);
+    const int rows = std::min(max_rows, J->d.n_iter);
+    for (int r = 0; r < rows; r++) {
+        out[4 * r + 0] = J->hist[2 * r].x;
+        out[4 * r + 1] = J->hist[2 * r].y;
+        out[4 * r + 2] = J->hist[2 * r + 1].x;
+        out[4 * r + 3] = J->hist[2 * r + 1].y;
+    }
+    return SUN_OK;
+}
+
+extern "C" int sun_server_release(sun_server* S, uint64_t id) {
+    if (!S) return fail(SUN_ERR_INVALID, "sun_server_release: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(id);
+    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
+    Job* J = it->second;
+    if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED)
+        return fail(SUN_ERR_STATE, "job %llu is still pending or running", (unsigned long long)id);
+    S->jobs.erase(it);
+    delete J;
+    return SUN_OK;
+}
+
+extern "C" int sun_server_stats(sun_server* S, sun_server_counters* st) {
+    if (!S || !st) return fail(SUN_ERR_INVALID, "sun_server_stats: NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    st->jobs_done = S->jobs_done;
+    st->link_updates = S->link_updates;
+    st->steps = S->steps;
+    st->busy_s = S->busy_s;
+    st->budget = S->arena.capacity();
+    st->bytes_in_use = S->arena.in_use();
+    st->running = (int)S->running.size();
+    st->pending = (int)S->pending.size();
+    return SUN_OK;
+}
+
+extern "C" double sun_server_clock(sun_server* S) { return S ? now_since(S->t0) : 0.0; }

@@ -17,8 +17,12 @@
 
 namespace sun {
 
-static const int kBlocks[] = {64, 128, 256};
-static const int kLds[] = {0, 16384, 24576, 32768, 40960, 54272, 81920};
+// candidate launch shapes: (block, t3) pairs, each with every LDS occupancy cap
+struct Shape {
+    int block, t3;
+};
+static const Shape kShapes[] = {{64, 1}, {128, 1}, {256, 1}, {128, 2}, {256, 4}, {512, 8}, {1024, 16}};
+static const int kLds[] = {0, 16384, 32768, 40960, 54272};
 
 std::string Tuner::key(const char* kernel, const sun_lattice_desc& d) const {
     char buf[256];
@@ -33,7 +37,7 @@ std::string Tuner::path() const {
     return std::string(dir) + "/sun_tunecache.json";
 }
 
-// The cache file is a JSON object {"key": [block, lds, ms], ...}, one entry per line.
+// The cache file is a JSON object {"key": [block, t3, lds, ms], ...}, one entry per line.
 void Tuner::load(int mode) {
     std::lock_guard<std::mutex> lk(mu_);
     mode_ = mode;
@@ -51,9 +55,9 @@ void Tuner::load(int mode) {
         if (q1 == std::string::npos || b0 == std::string::npos) continue;
         LaunchCfg c;
         float ms = 0.f;
-        if (sscanf(line.c_str() + b0, "[%d, %d, %f]", &c.block, &c.lds, &ms) != 3) continue;
+        if (sscanf(line.c_str() + b0, "[%d, %d, %d, %f]", &c.block, &c.t3, &c.lds, &ms) != 4) continue;
         bool ok = false;
-        for (int k : kBlocks) ok |= (k == c.block);
+        for (const Shape& k : kShapes) ok |= (k.block == c.block && k.t3 == c.t3);
         bool okl = false;
         for (int k : kLds) okl |= (k == c.lds);
         if (ok && okl) cache_[line.substr(q0 + 1, q1 - q0 - 1)] = {c, ms};
@@ -68,8 +72,8 @@ void Tuner::save() {
     out << "{\n";
     size_t i = 0;
     for (auto& kv : cache_) {
-        out << "  \"" << kv.first << "\": [" << kv.second.first.block << ", " << kv.second.first.lds << ", "
-            << kv.second.second << "]"
+        out << "  \"" << kv.first << "\": [" << kv.second.first.block << ", " << kv.second.first.t3 << ", "
+            << kv.second.first.lds << ", " << kv.second.second << "]"
             << (++i < cache_.size() ? ",\n" : "\n");
     }
     out << "}\n";
@@ -111,10 +115,13 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
         const bool hb = which == 0;
         LaunchCfg best;
         float best_ms = 1e30f;
-        for (int cand = 0; cand < (int)(sizeof(kBlocks) / sizeof(int)) * (int)(sizeof(kLds) / sizeof(int)); cand++) {
+        const int nshape = (int)(sizeof(kShapes) / sizeof(Shape)), nlds = (int)(sizeof(kLds) / sizeof(int));
+        for (int cand = 0; cand < nshape * nlds; cand++) {
             LaunchCfg c;
-            c.block = kBlocks[cand / (int)(sizeof(kLds) / sizeof(int))];
-            c.lds = kLds[cand % (int)(sizeof(kLds) / sizeof(int))];
+            c.block = kShapes[cand / nlds].block;
+            c.t3 = kShapes[cand / nlds].t3;
+            c.lds = kLds[cand % nlds];
+            if (!t3_valid(lat->g, c.t3) || c.block > lat->ops->max_block) continue;
             // warm-up, then 3 timed launches
             e = lat->ops->update(lat->f, lat->g, 0, 0, hb, 0, nullptr, bnc, 1, c, s);
             if (e == hipSuccess) e = hipEventRecord(e0, s);
@@ -127,8 +134,8 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
             if (e != hipSuccess) break;
             ms /= 3.f;
             if (ctx->verbosity >= SUN_VERBOSE)
-                fprintf(stderr, "[sun tune] %s block=%d lds=%d %.4f ms\n", hb ? "heatbath" : "overrelax", c.block,
-                        c.lds, ms);
+                fprintf(stderr, "[sun tune] %s block=%d t3=%d lds=%d %.4f ms\n", hb ? "heatbath" : "overrelax",
+                        c.block, c.t3, c.lds, ms);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = c;

@@ -179,3 +179,25 @@ def test_errors_are_reported_not_fatal(ctx):
     with sun_amd.Lattice(ctx, 3, (4, 4, 4, 4), 6.0, 18, 8) as lat:
         lat.cold_start()
         assert lat.plaquette()[0] == 1.0
+
+
+def test_cpp_facade_program_matches_c_abi(ctx, tmp_path):
+    """A CULQCD-style C++ program (tests/host/facade_demo.cpp against include/sun.hpp)
+    produces the same bits as the same run through the C ABI from Python."""
+    import os
+    import subprocess
+    from sun_amd.build import FACADE_DEMO
+    assert os.path.exists(FACADE_DEMO), "facade demo not built: run __graft_entry__.build()"
+    out = tmp_path / "links.bin"
+    dims, seed, iters = (8, 8, 8, 8), 1, 3
+    r = subprocess.run([FACADE_DEMO, *map(str, dims), str(seed), str(iters), str(out)], capture_output=True,
+                       text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    rows = np.array([[float(v) for v in line.split()[1:]] for line in r.stdout.strip().splitlines()])
+    links = np.fromfile(out, dtype=np.float64).view(np.complex128).reshape(4, -1, 3, 3)
+    with sun_amd.Lattice(ctx, 3, dims, 6.0, 18, 8) as lat:
+        lat.hot_start(seed)
+        obs = lat.generate(seed=seed, n_iter=iters, n_or=4, reunit_every=0, use_graph=False)
+        G = lat.to_host()
+    assert np.array_equal(rows, obs)
+    assert np.array_equal(links, G)

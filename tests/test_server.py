@@ -1,0 +1,189 @@
+"""Tests of the continuous-batching job server (sun_server_*).
+
+* the arena that carves job memory out of the reserved budget (host check, CPU);
+* every job's per-iteration observables are bit-identical to the same job run
+  alone through sun_generate, whatever else is resident -- jobs of different N,
+  storage, precision, volume, beta and n_or, admitted over time as memory frees up;
+* admission respects the budget (jobs queue, nothing over budget is resident) and
+  rejects jobs that can never fit, with SUN_ERR_BUDGET;
+* cancel of pending and running jobs, unsupported and invalid requests, the
+  background scheduling thread with wait().
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import sun_amd
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_arena_randomised(tmp_path):
+    exe = str(tmp_path / "arena_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(HERE, "host", "arena_check.cpp")],
+                   check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    assert out.strip() == "OK"
+
+
+# (nc, dims, beta, storage, precision, hot, n_iter, n_or, reunit_every, seed)
+JOBS = [
+    (3, (8, 8, 8, 8), 6.0, 12, 4, True, 5, 4, 1, 11),
+    (2, (6, 6, 6, 6), 2.3, 18, 4, False, 7, 4, 1, 12),
+    (4, (4, 4, 4, 8), 11.0, 18, 4, True, 4, 4, 0, 13),
+    (3, (6, 8, 8, 10), 5.8, 8, 4, True, 6, 4, 1, 14),
+    (3, (8, 8, 8, 8), 6.2, 12, 4, False, 3, 4, 1, 15),
+    (3, (4, 6, 6, 8), 6.0, 18, 8, True, 4, 2, 1, 16),
+    (2, (8, 8, 8, 8), 2.5, 18, 8, True, 5, 4, 1, 17),
+    (3, (8, 8, 8, 8), 5.9, 12, 4, True, 8, 4, 1, 18),
+]
+
+
+def alone(ctx, job):
+    nc, dims, beta, storage, prec, hot, n_iter, n_or, reunit, seed = job
+    with sun_amd.Lattice(ctx, nc, dims, beta, storage, prec) as lat:
+        if hot:
+            lat.hot_start(seed)
+        else:
+            lat.cold_start()
+        return lat.generate(seed=seed, n_iter=n_iter, n_or=n_or, reunit_every=reunit, first_sweep=0)
+
+
+def submit(srv, job):
+    nc, dims, beta, storage, prec, hot, n_iter, n_or, reunit, seed = job
+    return srv.submit(nc, dims, beta, seed, n_iter, n_or=n_or, reunit_every=reunit, storage=storage,
+                      precision=prec, hot_start=hot)
+
+
+def job_bytes(job):
+    nc, dims, beta, storage, prec = job[:5]
+    return sun_amd.lattice_bytes(nc, dims, storage, prec, beta)
+
+
+@pytest.mark.gpu
+def test_mix_equals_alone_with_queueing(ctx):
+    """A budget of about 2.5 of the largest jobs (half the total) forces queueing
+    and backfill; each job's history equals its alone run bit for bit."""
+    budget = 2 * max(job_bytes(j) for j in JOBS) + (512 << 10)
+    assert budget < sum(job_bytes(j) for j in JOBS)
+    with sun_amd.Server(ctx, budget, max_jobs=16) as srv:
+        ids = [submit(srv, j) for j in JOBS]
+        peak_running, steps = 0, 0
+        while True:
+            st = srv.stats()
+            if st.jobs_done == len(JOBS):
+                break
+            srv.step()
+            st = srv.stats()
+            peak_running = max(peak_running, st.running)
+            assert st.bytes_in_use <= st.budget
+            steps += 1
+            assert steps < 200
+        assert 1 < peak_running < len(JOBS)
+        assert srv.stats().bytes_in_use == 0
+        for job, jid in zip(JOBS, ids):
+            st = srv.status(jid)
+            assert st.state == sun_amd.JOB_DONE and st.iters_done == job[6]
+            assert st.t_submit <= st.t_start <= st.t_end
+            got = srv.observables(jid, job[6])
+            ref = alone(ctx, job)
+            assert np.array_equal(got, ref), (job, np.abs(got - ref).max())
+            assert got[-1, 0] == st.plaq[0] and got[-1, 3] == st.poly[1]
+            srv.release(jid)
+        # link updates: (1 + n_or) sweeps of 4V links per iteration
+        expect = sum(j[6] * (1 + j[7]) * 4 * int(np.prod(j[1])) for j in JOBS)
+        assert srv.stats().link_updates == expect
+
+
+@pytest.mark.gpu
+def test_arrivals_while_running(ctx):
+    """Jobs submitted while others are mid-run join at the next step and still equal
+    their alone runs."""
+    a, b, c = JOBS[0], JOBS[1], JOBS[2]
+    with sun_amd.Server(ctx, 1 << 30) as srv:
+        ia = submit(srv, a)
+        srv.step()
+        srv.step()
+        ib = submit(srv, b)
+        srv.step()
+        ic = submit(srv, c)
+        while srv.stats().jobs_done < 3:
+            srv.step()
+        for job, jid in ((a, ia), (b, ib), (c, ic)):
+            assert np.array_equal(srv.observables(jid, job[6]), alone(ctx, job))
+
+
+@pytest.mark.gpu
+def test_background_thread_and_wait(ctx):
+    with sun_amd.Server(ctx, 1 << 30) as srv:
+        srv.start()
+        ids = [submit(srv, j) for j in JOBS[:4]]
+        for jid in ids:
+            srv.wait(jid, 60.0)
+        srv.stop()
+        for job, jid in zip(JOBS[:4], ids):
+            assert srv.status(jid).state == sun_amd.JOB_DONE
+            assert np.array_equal(srv.observables(jid, job[6]), alone(ctx, job))
+        # step() is refused while the thread runs, allowed after stop()
+        srv.start()
+        with pytest.raises(sun_amd.SunError) as e:
+            srv.step()
+        assert e.value.code == sun_amd.ERR_STATE
+        srv.stop()
+        srv.step()
+
+
+@pytest.mark.gpu
+def test_budget_rejection_and_bad_requests(ctx):
+    big = (3, (16, 16, 16, 16), 6.0, 18, 8, True, 2, 4, 1, 1)
+    with sun_amd.Server(ctx, job_bytes(big) // 2) as srv:
+        with pytest.raises(sun_amd.SunError) as e:
+            submit(srv, big)
+        assert e.value.code == sun_amd.ERR_BUDGET
+        with pytest.raises(sun_amd.SunError) as e:  # cold start in 8-real storage
+            srv.submit(3, (4, 4, 4, 4), 6.0, 1, 2, storage=8, precision=4, hot_start=False)
+        assert e.value.code == sun_amd.ERR_UNSUPPORTED
+        with pytest.raises(sun_amd.SunError) as e:
+            srv.submit(3, (4, 4, 4, 4), 6.0, 1, 2, reunit_every=2)
+        assert e.value.code == sun_amd.ERR_UNSUPPORTED
+        with pytest.raises(sun_amd.SunError) as e:  # odd extent
+            srv.submit(3, (4, 4, 4, 5), 6.0, 1, 2)
+        assert e.value.code == sun_amd.ERR_INVALID
+        with pytest.raises(sun_amd.SunError) as e:
+            srv.submit(3, (4, 4, 4, 4), 6.0, 1, 0)
+        assert e.value.code == sun_amd.ERR_INVALID
+        with pytest.raises(sun_amd.SunError) as e:
+            srv.status(999)
+        assert e.value.code == sun_amd.ERR_STATE
+        # the server is still usable
+        jid = srv.submit(3, (4, 4, 4, 4), 6.0, 5, 2)
+        while srv.status(jid).state != sun_amd.JOB_DONE:
+            srv.step()
+
+
+@pytest.mark.gpu
+def test_cancel_pending_and_running(ctx):
+    j = JOBS[0]
+    with sun_amd.Server(ctx, job_bytes(j) * 3 // 2 + (64 << 10)) as srv:
+        first = submit(srv, j)
+        second = submit(srv, j)  # does not fit beside the first: stays pending
+        srv.step()
+        assert srv.status(first).state == sun_amd.JOB_RUNNING
+        assert srv.status(second).state == sun_amd.JOB_PENDING
+        srv.cancel(second)
+        assert srv.status(second).state == sun_amd.JOB_CANCELLED
+        srv.cancel(first)
+        srv.step()
+        st = srv.status(first)
+        assert st.state == sun_amd.JOB_CANCELLED and st.iters_done < j[6]
+        assert srv.stats().bytes_in_use == 0 and srv.stats().running == 0
+        with pytest.raises(sun_amd.SunError):
+            srv.observables(first, 1)
+        srv.release(first)
+        srv.release(second)
+        third = submit(srv, j)
+        while srv.status(third).state != sun_amd.JOB_DONE:
+            srv.step()
+        assert np.array_equal(srv.observables(third, j[6]), alone(ctx, j))
