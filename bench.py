@@ -81,6 +81,143 @@ def cpu_baseline(min_seconds: float = 10.0, max_iters: int = 20):
                       f"{dt:.1f} s"}
 
 
+def batch_means(x, nblocks=10):
+    """mean and its standard error from nblocks block means (autocorrelated chain)"""
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x) // nblocks * nblocks
+    b = x[len(x) - n:].reshape(nblocks, -1).mean(axis=1)
+    return float(b.mean()), float(b.std(ddof=1) / np.sqrt(nblocks))
+
+
+def plain_and_precision(sun_amd, ctx, stream, obs32, value, updates_per_step):
+    """The plain single-lattice path (FP64 18-real, one kernel per (parity, mu), eager
+    launches) on the same chain (seed, sweeps, cold start) as the FP32 run: its rate,
+    and the FP32-12 vs FP64-18 plaquette difference per iteration and for the
+    ensemble mean over the second half of the chain."""
+    n = len(obs32)
+    with sun_amd.Lattice(ctx, 3, DIMS, BETA, 18, 8) as pl:
+        pl.cold_start(stream)
+        sun_amd.sync(stream)
+        t0 = time.perf_counter()
+        obs64 = pl.generate(seed=2, n_iter=n, n_or=4, reunit_every=1, use_graph=False, stream=stream)
+        sun_amd.sync(stream)
+        pv = updates_per_step * n / (time.perf_counter() - t0)
+    p32, p64 = obs32[:, :2].mean(axis=1), obs64[:, :2].mean(axis=1)
+    d = np.abs(p32 - p64)
+    over = np.nonzero(d > 1e-6)[0]
+    half = n // 2
+    m32, e32 = batch_means(p32[half:])
+    m64, e64 = batch_means(p64[half:])
+    poly32 = np.hypot(obs32[half:, 2], obs32[half:, 3])
+    poly64 = np.hypot(obs64[half:, 2], obs64[half:, 3])
+    plain = {"value": pv, "unit": "link-updates/s", "iterations": n,
+             "config": "SU(3) 32^4 beta=6.0 cold start, FP64 18-real, one kernel per (parity, mu), eager "
+                       "launches, reunitarize + plaquette + Polyakov every iteration",
+             "gain": value / pv}
+    prec = {"plaquette_absdiff_per_iter_first5": [float(v) for v in d[:5]],
+            "first_iter_absdiff_over_1e-6": int(over[0]) if len(over) else None,
+            "window": [half, n],
+            "plaquette_mean_f32": [m32, e32], "plaquette_mean_f64": [m64, e64],
+            "plaquette_mean_diff": m32 - m64, "plaquette_mean_diff_sigma": float(np.hypot(e32, e64)),
+            "abs_polyakov_mean_f32": batch_means(poly32), "abs_polyakov_mean_f64": batch_means(poly64),
+            "note": "same seed and sweeps; the chains separate after a few iterations (a sweep amplifies "
+                    "differences ~100x), so the ensemble means are compared with batch-means errors"}
+    return plain, prec
+
+
+BATCH_VOLUMES = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 16), (16, 16, 16, 32), (24, 24, 24, 24)]
+
+
+def batched_leg(sun_amd, ctx, stream, n_lat=64, n_iter=30, warm=3):
+    """64 independent SU(3) lattices, volumes drawn from BATCH_VOLUMES, beta uniform in
+    [5.7, 6.3], hot/cold starts 50/50, FP32 12-real; hipGraph-captured iterations of
+    1 heatbath + 4 overrelaxation sweeps (one launch per (parity, mu) for all 64) +
+    reunitarize + per-lattice plaquette."""
+    rng = np.random.default_rng(4)
+    vols = [BATCH_VOLUMES[i] for i in rng.integers(0, len(BATCH_VOLUMES), n_lat)]
+    betas = rng.uniform(5.7, 6.3, n_lat)
+    hot = rng.integers(0, 2, n_lat)
+    lats = [sun_amd.Lattice(ctx, 3, v, float(b), 12, 4) for v, b in zip(vols, betas)]
+    for i, l in enumerate(lats):
+        if hot[i]:
+            l.hot_start(1000 + i, stream)
+        else:
+            l.cold_start(stream)
+    batch = sun_amd.Batch(ctx, lats, [4000 + i for i in range(n_lat)])
+    batch.generate(first_sweep=0, n_iter=warm, n_or=4, reunit_every=1, stream=stream)
+    sun_amd.sync(stream)
+    t0 = time.perf_counter()
+    obs = batch.generate(first_sweep=warm, n_iter=n_iter, n_or=4, reunit_every=1, stream=stream)
+    sun_amd.sync(stream)
+    dt = time.perf_counter() - t0
+    sites = sum(int(np.prod(v)) for v in vols)
+    batch.close()
+    for l in lats:
+        l.close()
+    return {"value": 5 * 4 * sites * n_iter / dt, "unit": "link-updates/s", "lattices": n_lat,
+            "total_sites": sites, "iterations": n_iter, "ms_per_iteration": dt / n_iter * 1e3,
+            "plaquette_range_last": [float(obs[-1, :, :].mean(axis=1).min()),
+                                     float(obs[-1, :, :].mean(axis=1).max())],
+            "config": "64 SU(3) lattices, volumes from {8^4,12^4,16^4,16^3x32,24^4} (numpy seed 4), beta "
+                      "U[5.7,6.3], hot/cold 50/50, FP32 12-real, hipGraph iteration = HB + 4 OR + reunit + "
+                      "per-lattice plaquette"}
+
+
+SERVE_VOLUMES = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 16), (24, 24, 24, 24), (32, 32, 32, 32)]
+SERVE_BETA = {2: (2.2, 2.6), 3: (5.6, 6.2), 4: (10.5, 11.2)}
+
+
+def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
+    """Poisson arrivals of ensemble jobs (SU(2)/SU(3)/SU(4), volumes 8^4..32^4, beta in
+    the usual range per N, 50..500 iterations of 1 HB + 4 OR + reunit + plaquette +
+    Polyakov, FP32 links, FP64 reductions) into the job server's background loop.
+    The arrival rate targets `util` of the device from a bandwidth estimate of each
+    job's work.  Latency = completion - submission."""
+    rng = np.random.default_rng(5)
+    jobs = []
+    for j in range(n_jobs):
+        nc = int(rng.integers(2, 5))
+        dims = SERVE_VOLUMES[int(rng.integers(0, len(SERVE_VOLUMES)))]
+        lo, hi = SERVE_BETA[nc]
+        storage = 12 if nc == 3 else 18
+        jobs.append(dict(nc=nc, dims=dims, beta=float(rng.uniform(lo, hi)), storage=storage, precision=4,
+                         seed=5000 + j, hot_start=bool(rng.integers(0, 2)), n_iter=int(rng.integers(50, 501))))
+    # estimated service time at half of peak bandwidth: 9 link transfers per update
+    est = [j["n_iter"] * 5 * 4 * int(np.prod(j["dims"])) * 9 * link_bytes(j["nc"], j["storage"], 4)
+           / (0.5 * PEAK_HBM_GBS * 1e9) for j in jobs]
+    rate = util / float(np.mean(est))
+    gaps = rng.exponential(1.0 / rate, n_jobs)
+    updates = sum(j["n_iter"] * 5 * 4 * int(np.prod(j["dims"])) for j in jobs)
+    with sun_amd.Server(ctx, budget, max_jobs=256) as srv:
+        srv.start()
+        ids = []
+        t0 = time.perf_counter()
+        t_next = 0.0
+        for j, g in zip(jobs, gaps):
+            t_next += g
+            delay = t_next - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(j["nc"], j["dims"], j["beta"], j["seed"], j["n_iter"], n_or=4, reunit_every=1,
+                                  storage=j["storage"], precision=4, hot_start=j["hot_start"]))
+        for jid in ids:
+            srv.wait(jid, 600.0)
+        srv.stop()
+        st = [srv.status(i) for i in ids]
+        counters = srv.stats()
+    lat = np.array([s.t_end - s.t_submit for s in st])
+    queue = np.array([s.t_start - s.t_submit for s in st])
+    span = max(s.t_end for s in st) - min(s.t_submit for s in st)
+    return {"value": updates / span, "unit": "link-updates/s", "jobs": n_jobs,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "p50_queue_s": float(np.percentile(queue, 50)), "p99_queue_s": float(np.percentile(queue, 99)),
+            "span_s": span, "arrival_rate_per_s": rate, "busy_s": counters.busy_s, "steps": int(counters.steps),
+            "config": "Poisson arrivals (numpy seed 5) at ~75% estimated load; N in {2,3,4}, volumes "
+                      "{8^4,12^4,16^4,24^4,32^4}, beta SU(2) [2.2,2.6], SU(3) [5.6,6.2], SU(4) [10.5,11.2], "
+                      "50..500 iterations of HB + 4 OR + reunit + plaquette + Polyakov, FP32 links (SU(3) "
+                      "12-real), FP64 reductions, hot/cold 50/50, 64 GiB budget"}
+
+
 def kernels_only(sun_amd):
     """For rocprofv3 counter passes: on the 32^4 FP32 12-real field after a few
     iterations, 2 copies of the field (calibration: a plain copy of known size),
@@ -117,6 +254,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-lattice leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson arrivals) leg")
+    ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served leg")
     ap.add_argument("--kernels-only", action="store_true",
                     help="fixed short launch sequence for profiler counter passes (no JSON line)")
     args = ap.parse_args()
@@ -139,13 +279,14 @@ def main():
     sun_amd.sync(stream)
 
     # warmup (tunes launch shapes, builds and replays the graph)
-    lat.generate(seed=2, n_iter=args.warmup, n_or=4, reunit_every=1, first_sweep=0, stream=stream)
+    obs_warm = lat.generate(seed=2, n_iter=args.warmup, n_or=4, reunit_every=1, first_sweep=0, stream=stream)
     sun_amd.sync(stream)
     t0 = time.perf_counter()
     obs = lat.generate(seed=2, n_iter=args.steps, n_or=4, reunit_every=1, first_sweep=args.warmup, stream=stream)
     sun_amd.sync(stream)
     wall = time.perf_counter() - t0
     value = updates_per_step * args.steps / wall
+    obs_all = np.concatenate([obs_warm, obs])
 
     # dominant kernel: per-launch durations on the same stream
     or_ms = time_launches(sun_amd, lat, stream, hb=False, reps=64)
@@ -188,17 +329,12 @@ def main():
     lat.close()
 
     if not args.no_plain:
-        with sun_amd.Lattice(ctx, 3, DIMS, BETA, 18, 8) as pl:
-            pl.cold_start(stream)
-            pl.generate(seed=2, n_iter=2, n_or=4, reunit_every=1, use_graph=False, stream=stream)
-            n = 10
-            t0 = time.perf_counter()
-            pl.generate(seed=2, n_iter=n, n_or=4, reunit_every=1, first_sweep=2, use_graph=False, stream=stream)
-            sun_amd.sync(stream)
-            pv = updates_per_step * n / (time.perf_counter() - t0)
-        out["plain_fp64"] = {"value": pv, "unit": "link-updates/s",
-                             "config": "FP64 18-real, one kernel per (parity, mu), eager launches",
-                             "gain": value / pv}
+        out["plain_fp64"], out["precision"] = plain_and_precision(sun_amd, ctx, stream, obs_all, value,
+                                                                  updates_per_step)
+    if not args.no_batch:
+        out["batched"] = batched_leg(sun_amd, ctx, stream)
+    if not args.no_serve:
+        out["served"] = served_leg(sun_amd, ctx, args.serve_jobs)
     if not args.no_cpu:
         cb = cpu_baseline()
         out["cpu_baseline"] = cb

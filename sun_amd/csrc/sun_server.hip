@@ -7,13 +7,16 @@
 // for later, smaller ones ("backfill") unless it has already been passed over
 // kMaxPassOver times, in which case admission stops until it fits.
 //
-// Execution: one scheduling step advances every resident job by one iteration.
-// Resident jobs are grouped by kind (kernel variant, n_or, reunit); a group runs its
-// iteration with the batched kernels -- one launch per (parity, mu) for all its
-// lattices -- on its own stream, so groups overlap on the device.  A job's sweep
-// counter and observable history live in its batch-table entries on the device and
-// are advanced by the kernels themselves.  At the end of the step the host retires
-// finished jobs (copies their history out, frees their memory) and admits new ones.
+// Execution: resident jobs are grouped by kind (kernel variant, n_or, reunit) and,
+// for large lattices, by volume.  A group runs its iterations with the batched
+// kernels -- one launch per (parity, mu) for all its lattices -- captured once per
+// membership into a hipGraph, on its own stream.  Groups progress independently:
+// the scheduling thread launches a few iterations of an idle group (as many as take
+// about kTargetMs, never more than any member still needs), polls the groups'
+// completion events, and at each completion retires finished jobs (history copied
+// out, memory freed), admits pending ones and re-captures the group.  A job's sweep
+// counter and observable history live in its batch-table entries and are advanced
+// by the kernels themselves.
 //
 // A job's arithmetic depends only on its own lattice, seed and sweep number (the
 // per-site update and the per-lattice fixed-order reductions are the same code
@@ -37,6 +40,9 @@ using namespace sun;
 namespace {
 
 constexpr int kMaxPassOver = 8;
+constexpr double kTargetMs = 2.0;      // work per group launch before the scheduler looks again
+constexpr int kMaxItersPerLaunch = 16;
+constexpr int kLargeVolume = 1 << 17;  // lattices at least this large get a group per volume
 
 struct Group;
 
@@ -56,14 +62,20 @@ struct Job {
 
 struct Group {
     const OpsTable* ops = nullptr;
-    int n_or = 0, reunit = 0;
+    int n_or = 0, reunit = 0, vkey = 0;
     hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
     int cap = 0;
     BatchEntry *t_update = nullptr, *t_reunit = nullptr, *t_plaq = nullptr, *t_poly = nullptr;
     double2 *d_plaq = nullptr, *d_poly = nullptr;
-    std::vector<Job*> members;
-    bool dirty = true;
+    std::vector<Job*> members;   // in the device tables
+    std::vector<Job*> incoming;  // admitted (start kernels queued), join at the next launch
+    bool dirty = true;           // tables and graph need rebuilding
     int blocks_update = 0, blocks_reunit = 0, blocks_plaq = 0, blocks_poly = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int inflight = 0;            // iterations launched and not yet completed
+    double t_launch = 0, iter_ms = 0;
 };
 
 double now_since(std::chrono::steady_clock::time_point t0) {
@@ -77,25 +89,37 @@ struct sun_server {
     size_t budget = 0;
     int max_jobs = 0;
     char* base = nullptr;
-    Arena arena;
-    std::mutex mu;
+    sun::Arena arena;
+    std::mutex mu;  // jobs, pending, running, arena, counters, job states
     std::condition_variable cv;
     std::deque<Job*> pending;
     std::map<uint64_t, Job*> jobs;
     std::vector<Job*> running;
+    // groups are driven by one thread at a time: the scheduling thread, or
+    // sun_server_step's caller while that thread is off (step_mu)
     std::vector<Group*> groups;
     std::thread worker;
-    bool stop = false, worker_on = false;
+    bool stop = false, worker_on = false, admit_needed = false;
+    int worker_rc = SUN_OK;
     uint64_t next_id = 1;
     std::chrono::steady_clock::time_point t0;
     uint64_t jobs_done = 0, link_updates = 0, steps = 0;
     double busy_s = 0;
-    std::mutex step_mu;  // one scheduling step at a time
+    std::mutex step_mu;
 };
 
 namespace {
 
+void drop_graph(Group* g) {
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    g->exec = nullptr;
+    g->graph = nullptr;
+}
+
 void free_group(Group* g) {
+    drop_graph(g);
+    if (g->done) hipEventDestroy(g->done);
     if (g->stream) hipStreamDestroy(g->stream);
     hipFree(g->t_update);
     hipFree(g->t_reunit);
@@ -106,15 +130,19 @@ void free_group(Group* g) {
     delete g;
 }
 
-int make_group(sun_server* S, const OpsTable* ops, int n_or, int reunit, Group** out) {
+int group_vkey(const sun_lattice* L) { return L->g.V >= kLargeVolume ? L->g.V : 0; }
+
+int make_group(sun_server* S, const sun_lattice* L, int n_or, int reunit, Group** out) {
     Group* g = new (std::nothrow) Group();
     if (!g) return fail(SUN_ERR_NOMEM, "server: host allocation");
-    g->ops = ops;
+    g->ops = L->ops;
     g->n_or = n_or;
     g->reunit = reunit;
+    g->vkey = group_vkey(L);
     g->cap = S->max_jobs;
     const size_t tb = sizeof(BatchEntry) * (size_t)g->cap;
     hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&g->t_update, tb);
     if (e == hipSuccess) e = hipMalloc(&g->t_reunit, tb);
     if (e == hipSuccess) e = hipMalloc(&g->t_plaq, tb);
@@ -130,8 +158,8 @@ int make_group(sun_server* S, const OpsTable* ops, int n_or, int reunit, Group**
     return SUN_OK;
 }
 
-// device tables of a group from its members' host state
-int rebuild(Group* g) {
+// device tables of an idle group from its members' host state
+int rebuild_tables(Group* g) {
     const int n = (int)g->members.size();
     std::vector<BatchEntry> tu(n), tr(n), tp(n), tl(n);
     int cu = 0, cr = 0, cp = 0, cl = 0;
@@ -157,12 +185,12 @@ int rebuild(Group* g) {
         p.first_block = cp;
         p.nblocks = plaquette_blocks(L->g);
         p.partials = L->partials;
-        p.scale = 1.0 / ((double)(3 * L->d.nc) * (double)L->g.V);
+        p.scale = 1.0 / ((double)(3 * L->d.nc) * (double)L->g.V);  // plaquette_cub.cu:234
         cp += p.nblocks;
         q.first_block = cl;
         q.nblocks = polyakov_blocks(L->g);
         q.partials = L->partials;  // reused: the Polyakov launch follows the plaquette finalize
-        q.scale = 1.0 / ((double)L->d.nc * (double)L->g.X[0] * L->g.X[1] * L->g.X[2]);
+        q.scale = 1.0 / ((double)L->d.nc * (double)L->g.X[0] * L->g.X[1] * L->g.X[2]);  // polyakovloop.cu:651
         cl += q.nblocks;
         tu[i] = a;
         tr[i] = r;
@@ -179,19 +207,18 @@ int rebuild(Group* g) {
         HIP_TRY(hipMemcpyAsync(g->t_reunit, tr.data(), b, hipMemcpyHostToDevice, g->stream));
         HIP_TRY(hipMemcpyAsync(g->t_plaq, tp.data(), b, hipMemcpyHostToDevice, g->stream));
         HIP_TRY(hipMemcpyAsync(g->t_poly, tl.data(), b, hipMemcpyHostToDevice, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));  // host vectors go out of scope
+        HIP_TRY(hipStreamSynchronize(g->stream));  // the host vectors go out of scope
     }
-    g->dirty = false;
     return SUN_OK;
 }
 
 // one iteration of every member of a group, enqueued on the group's stream
 int group_iteration(Group* g) {
     const int n = (int)g->members.size();
-    if (!n) return SUN_OK;
     hipStream_t s = g->stream;
     for (int parity = 0; parity < 2; parity++)
-        for (int mu = 0; mu < 4; mu++) HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, true, s));
+        for (int mu = 0; mu < 4; mu++)
+            HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, true, s));
     for (int k = 0; k < g->n_or; k++)
         for (int parity = 0; parity < 2; parity++)
             for (int mu = 0; mu < 4; mu++)
@@ -206,22 +233,60 @@ int group_iteration(Group* g) {
     return SUN_OK;
 }
 
+int capture_group(Group* g) {
+    drop_graph(g);
+    HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = group_iteration(g);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc != SUN_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(SUN_ERR_HIP, "server: hipStreamEndCapture: %s", hipGetErrorString(e));
+    g->graph = graph;
+    HIP_TRY(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
+    return SUN_OK;
+}
+
+// idle group: take in the admitted jobs, rebuild if needed, launch up to max_iters
+// iterations (never more than a member still needs) and record the completion event
+int launch_group(sun_server* S, Group* g, int max_iters) {
+    if (!g->incoming.empty()) {
+        for (Job* J : g->incoming) g->members.push_back(J);
+        g->incoming.clear();
+        g->dirty = true;
+    }
+    if (g->members.empty()) return SUN_OK;
+    if (g->dirty) {
+        SUN_TRY(rebuild_tables(g));
+        SUN_TRY(capture_group(g));
+        g->dirty = false;
+    }
+    int k = max_iters;
+    for (const Job* J : g->members) k = std::min(k, J->d.n_iter - J->iters_done);
+    k = std::max(k, 1);
+    for (int i = 0; i < k; i++) HIP_TRY(hipGraphLaunch(g->exec, g->stream));
+    HIP_TRY(hipEventRecord(g->done, g->stream));
+    g->inflight = k;
+    g->t_launch = now_since(S->t0);
+    return SUN_OK;
+}
+
 Group* find_group(sun_server* S, const Job* J) {
+    const int vk = group_vkey(J->lat);
     for (Group* g : S->groups)
-        if (g->ops == J->lat->ops && g->n_or == J->d.n_or && g->reunit == J->d.reunit_every) return g;
+        if (g->ops == J->lat->ops && g->n_or == J->d.n_or && g->reunit == J->d.reunit_every && g->vkey == vk)
+            return g;
     return nullptr;
 }
 
 // admission, under S->mu
 int admit(sun_server* S) {
+    S->admit_needed = false;
     bool head_blocked = false;
     for (auto it = S->pending.begin(); it != S->pending.end();) {
         Job* J = *it;
-        if (J->cancel) {
-            J->state = SUN_JOB_CANCELLED;
-            it = S->pending.erase(it);
-            continue;
-        }
         if ((int)S->running.size() >= S->max_jobs) break;
         const size_t off = S->arena.alloc(J->bytes);
         if (off == SIZE_MAX) {
@@ -242,21 +307,25 @@ int admit(sun_server* S) {
         J->lat = L;
         Group* g = find_group(S, J);
         if (!g) {
-            rc = make_group(S, L->ops, J->d.n_or, J->d.reunit_every, &g);
+            rc = make_group(S, L, J->d.n_or, J->d.reunit_every, &g);
             if (rc != SUN_OK) {
                 lattice_free(L);
+                J->lat = nullptr;
                 S->arena.release(off);
                 return rc;
             }
             S->groups.push_back(g);
         }
         J->group = g;
-        hipError_t e = J->d.hot_start ? L->ops->hot_start(L->f, L->g, J->d.seed, g->stream)
-                                      : L->ops->cold_start(L->f, 4 * L->g.V, g->stream);
-        if (e != hipSuccess) return fail(SUN_ERR_HIP, "server: start of job %llu: %s", (unsigned long long)J->id,
-                                         hipGetErrorString(e));
-        g->members.push_back(J);
-        g->dirty = true;
+        // queued behind whatever the group has in flight; touches only this job's memory
+        const hipError_t e = J->d.hot_start ? L->ops->hot_start(L->f, L->g, J->d.seed, g->stream)
+                                            : L->ops->cold_start(L->f, 4 * L->g.V, g->stream);
+        if (e != hipSuccess) {
+            hipGetLastError();
+            return fail(SUN_ERR_HIP, "server: start of job %llu: %s", (unsigned long long)J->id,
+                        hipGetErrorString(e));
+        }
+        g->incoming.push_back(J);
         J->state = SUN_JOB_RUNNING;
         J->t_start = now_since(S->t0);
         S->running.push_back(J);
@@ -265,6 +334,7 @@ int admit(sun_server* S) {
     return SUN_OK;
 }
 
+// under S->mu; the job's group is idle and the job is one of its members
 int retire(sun_server* S, Job* J, bool done) {
     Group* g = J->group;
     if (done) {
@@ -279,61 +349,137 @@ int retire(sun_server* S, Job* J, bool done) {
     J->off = SIZE_MAX;
     J->state = done ? SUN_JOB_DONE : SUN_JOB_CANCELLED;
     J->t_end = now_since(S->t0);
+    S->running.erase(std::find(S->running.begin(), S->running.end(), J));
+    S->admit_needed = true;
+    if (done) S->jobs_done++;
     return SUN_OK;
 }
 
-int step(sun_server* S) {
-    std::lock_guard<std::mutex> sl(S->step_mu);
-    const double t_begin = now_since(S->t0);
-    std::vector<Group*> active;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        SUN_TRY(admit(S));
-        for (Group* g : S->groups)
-            if (!g->members.empty()) active.push_back(g);
-    }
-    if (active.empty()) return SUN_OK;
-    for (Group* g : active) {
-        if (g->dirty) SUN_TRY(rebuild(g));
-        SUN_TRY(group_iteration(g));
-    }
-    for (Group* g : active) HIP_TRY(hipStreamSynchronize(g->stream));
+// a group's launched iterations have completed (its event has fired)
+int complete_group(sun_server* S, Group* g) {
+    const int k = g->inflight;
+    g->inflight = 0;
+    g->iter_ms = (now_since(S->t0) - g->t_launch) * 1e3 / k;
     std::lock_guard<std::mutex> lk(S->mu);
-    for (auto it = S->running.begin(); it != S->running.end();) {
-        Job* J = *it;
-        J->iters_done++;
-        const sun_lattice* L = J->lat;
-        S->link_updates += (uint64_t)(1 + J->d.n_or) * 4ull * (uint64_t)L->g.V;
+    const std::vector<Job*> members = g->members;
+    for (Job* J : members) {
+        J->iters_done += k;
+        S->link_updates += (uint64_t)k * (uint64_t)(1 + J->d.n_or) * 4ull * (uint64_t)J->lat->g.V;
         const bool done = J->iters_done >= J->d.n_iter;
-        if (done || J->cancel) {
-            SUN_TRY(retire(S, J, done));
-            if (done) S->jobs_done++;
-            it = S->running.erase(it);
-        } else {
-            ++it;
-        }
+        if (done || J->cancel) SUN_TRY(retire(S, J, done));
     }
     S->steps++;
-    S->busy_s += now_since(S->t0) - t_begin;
     S->cv.notify_all();
     return SUN_OK;
 }
 
-void worker_loop(sun_server* S) {
-    hipSetDevice(S->ctx->device);
-    for (;;) {
-        {
-            std::unique_lock<std::mutex> lk(S->mu);
-            S->cv.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->running.empty(); });
-            if (S->stop) return;
-        }
-        if (step(S) != SUN_OK) {
-            std::lock_guard<std::mutex> lk(S->mu);
-            S->stop = true;
-            S->cv.notify_all();
-            return;
+// cancelled jobs that have not joined their (idle) group's tables yet
+int drop_cancelled_incoming(sun_server* S) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (Group* g : S->groups) {
+        if (g->inflight) continue;
+        for (auto it = g->incoming.begin(); it != g->incoming.end();) {
+            Job* J = *it;
+            if (!J->cancel) {
+                ++it;
+                continue;
+            }
+            HIP_TRY(hipStreamSynchronize(g->stream));  // its start kernels
+            it = g->incoming.erase(it);
+            g->members.push_back(J);  // retire() takes it out of the members again
+            SUN_TRY(retire(S, J, false));
         }
     }
+    return SUN_OK;
+}
+
+// synchronous step: one iteration of every resident job
+int step(sun_server* S) {
+    const double t_begin = now_since(S->t0);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        SUN_TRY(admit(S));
+    }
+    SUN_TRY(drop_cancelled_incoming(S));
+    std::vector<Group*> active;
+    for (Group* g : S->groups) {
+        SUN_TRY(launch_group(S, g, 1));
+        if (g->inflight) active.push_back(g);
+    }
+    for (Group* g : active) HIP_TRY(hipEventSynchronize(g->done));
+    for (Group* g : active) SUN_TRY(complete_group(S, g));
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->busy_s += now_since(S->t0) - t_begin;
+    return SUN_OK;
+}
+
+// the scheduling thread: groups progress independently
+int worker_body(sun_server* S) {
+    double busy_since = -1;
+    for (;;) {
+        bool stopping;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            bool any_inflight = false;
+            for (Group* g : S->groups) any_inflight |= g->inflight > 0;
+            if (!any_inflight) {
+                if (busy_since >= 0) S->busy_s += now_since(S->t0) - busy_since;
+                busy_since = -1;
+                S->cv.wait(lk, [&] { return S->stop || S->admit_needed || !S->running.empty(); });
+            }
+            stopping = S->stop;
+            if (!stopping && S->admit_needed) SUN_TRY(admit(S));
+        }
+        if (!stopping) SUN_TRY(drop_cancelled_incoming(S));
+        bool progressed = false, any_inflight = false;
+        for (Group* g : S->groups) {
+            if (g->inflight) {
+                const hipError_t q = hipEventQuery(g->done);
+                if (q == hipErrorNotReady) {
+                    any_inflight = true;
+                    continue;
+                }
+                HIP_TRY(q);
+                SUN_TRY(complete_group(S, g));
+                progressed = true;
+            }
+            if (stopping) continue;
+            const int iters = g->iter_ms > 0 ? (int)(kTargetMs / g->iter_ms) : 1;
+            SUN_TRY(launch_group(S, g, std::min(std::max(iters, 1), kMaxItersPerLaunch)));
+            if (g->inflight) {
+                any_inflight = true;
+                if (busy_since < 0) busy_since = now_since(S->t0);
+            }
+        }
+        if (stopping && !any_inflight) {
+            if (busy_since >= 0) {
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->busy_s += now_since(S->t0) - busy_since;
+            }
+            return SUN_OK;
+        }
+        if (!progressed && any_inflight) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+void worker_loop(sun_server* S) {
+    std::lock_guard<std::mutex> sl(S->step_mu);
+    hipSetDevice(S->ctx->device);
+    const int rc = worker_body(S);
+    if (rc != SUN_OK) {
+        // drain what is in flight so no kernel outlives the failure
+        for (Group* g : S->groups)
+            if (g->inflight) hipEventSynchronize(g->done);
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->worker_rc = rc;
+        S->stop = true;
+        S->cv.notify_all();
+    }
+}
+
+Job* find_job(sun_server* S, uint64_t id) {
+    auto it = S->jobs.find(id);
+    return it == S->jobs.end() ? nullptr : it->second;
 }
 
 }  // namespace
@@ -348,7 +494,7 @@ extern "C" int sun_server_create(sun_ctx* ctx, size_t budget, int max_jobs, sun_
     S->budget = budget;
     S->max_jobs = max_jobs;
     HIP_TRY_CLEANUP(hipSetDevice(ctx->device), delete S);
-    hipError_t e = hipMalloc(&S->base, budget);
+    const hipError_t e = hipMalloc(&S->base, budget);
     if (e != hipSuccess) {
         hipGetLastError();
         delete S;
@@ -371,7 +517,9 @@ extern "C" int sun_server_stop(sun_server* S) {
     std::lock_guard<std::mutex> lk(S->mu);
     S->worker_on = false;
     S->stop = false;
-    return SUN_OK;
+    const int rc = S->worker_rc;
+    S->worker_rc = SUN_OK;
+    return rc == SUN_OK ? SUN_OK : fail(rc, "the scheduling thread had stopped on an error");
 }
 
 extern "C" int sun_server_destroy(sun_server* S) {
@@ -413,6 +561,7 @@ extern "C" int sun_server_submit(sun_server* S, const sun_job_desc* jd, uint64_t
     J->t_submit = now_since(S->t0);
     S->jobs[J->id] = J;
     S->pending.push_back(J);
+    S->admit_needed = true;
     *id = J->id;
     S->cv.notify_all();
     return SUN_OK;
@@ -421,17 +570,17 @@ extern "C" int sun_server_submit(sun_server* S, const sun_job_desc* jd, uint64_t
 extern "C" int sun_server_cancel(sun_server* S, uint64_t id) {
     if (!S) return fail(SUN_ERR_INVALID, "sun_server_cancel: NULL");
     std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->jobs.find(id);
-    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    Job* J = it->second;
+    Job* J = find_job(S, id);
+    if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
     if (J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED) return SUN_OK;
     J->cancel = true;
     if (J->state == SUN_JOB_PENDING) {
         S->pending.erase(std::find(S->pending.begin(), S->pending.end(), J));
         J->state = SUN_JOB_CANCELLED;
         J->t_end = now_since(S->t0);
-        S->cv.notify_all();
+        S->admit_needed = true;
     }
+    S->cv.notify_all();
     return SUN_OK;
 }
 
@@ -441,7 +590,8 @@ extern "C" int sun_server_step(sun_server* S) {
         std::lock_guard<std::mutex> lk(S->mu);
         if (S->worker_on) return fail(SUN_ERR_STATE, "sun_server_step: the scheduling thread is running");
     }
-    hipSetDevice(S->ctx->device);
+    std::lock_guard<std::mutex> sl(S->step_mu);
+    HIP_TRY(hipSetDevice(S->ctx->device));
     return step(S);
 }
 
@@ -450,7 +600,9 @@ extern "C" int sun_server_start(sun_server* S) {
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->worker_on) return SUN_OK;
     S->stop = false;
+    S->worker_rc = SUN_OK;
     S->worker_on = true;
+    S->admit_needed = true;
     S->worker = std::thread(worker_loop, S);
     return SUN_OK;
 }
@@ -458,24 +610,24 @@ extern "C" int sun_server_start(sun_server* S) {
 extern "C" int sun_server_wait(sun_server* S, uint64_t id, double timeout_s) {
     if (!S) return fail(SUN_ERR_INVALID, "sun_server_wait: NULL");
     std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->jobs.find(id);
-    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    Job* J = it->second;
-    auto finished = [&] { return J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED || S->stop; };
+    Job* J = find_job(S, id);
+    if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
+    auto finished = [&] {
+        return J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED || S->worker_rc != SUN_OK;
+    };
     if (!S->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), finished))
         return fail(SUN_ERR_STATE, "job %llu not finished after %.1f s", (unsigned long long)id, timeout_s);
     if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED)
-        return fail(SUN_ERR_STATE, "the scheduling thread stopped before job %llu finished: %s",
-                    (unsigned long long)id, sun_last_error());
+        return fail(SUN_ERR_STATE, "the scheduling thread stopped on an error before job %llu finished",
+                    (unsigned long long)id);
     return SUN_OK;
 }
 
 extern "C" int sun_server_status(sun_server* S, uint64_t id, sun_job_status* st) {
     if (!S || !st) return fail(SUN_ERR_INVALID, "sun_server_status: NULL");
     std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->jobs.find(id);
-    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    const Job* J = it->second;
+    const Job* J = find_job(S, id);
+    if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
     memset(st, 0, sizeof(*st));
     st->id = J->id;
     st->state = J->state;
@@ -496,9 +648,8 @@ extern "C" int sun_server_status(sun_server* S, uint64_t id, sun_job_status* st)
 extern "C" int sun_server_observables(sun_server* S, uint64_t id, double* out, int max_rows) {
     if (!S || !out) return fail(SUN_ERR_INVALID, "sun_server_observables: NULL");
     std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->jobs.find(id);
-    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    const Job* J = it->second;
+    const Job* J = find_job(S, id);
+    if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
     if (J->state != SUN_JOB_DONE) return fail(SUN_ERR_STATE, "job %llu is not done", (unsigned long long)id);
     const int rows = std::min(max_rows, J->d.n_iter);
     for (int r = 0; r < rows; r++) {
@@ -513,12 +664,11 @@ extern "C" int sun_server_observables(sun_server* S, uint64_t id, double* out, i
 extern "C" int sun_server_release(sun_server* S, uint64_t id) {
     if (!S) return fail(SUN_ERR_INVALID, "sun_server_release: NULL");
     std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->jobs.find(id);
-    if (it == S->jobs.end()) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    Job* J = it->second;
+    Job* J = find_job(S, id);
+    if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
     if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED)
         return fail(SUN_ERR_STATE, "job %llu is still pending or running", (unsigned long long)id);
-    S->jobs.erase(it);
+    S->jobs.erase(id);
     delete J;
     return SUN_OK;
 }
