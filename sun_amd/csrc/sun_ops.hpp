@@ -10,8 +10,7 @@
 
 namespace sun {
 
-// Launch shape chosen by the autotuner (threads per block, and sites per thread
-// along the fastest lattice direction for the tiled update kernel).
+// Launch shape chosen by the autotuner.
 struct LaunchCfg {
     int block = 256;
     // dynamic LDS reserved per workgroup and not used by the kernel: caps the
@@ -19,23 +18,39 @@ struct LaunchCfg {
     // flight per XCD, which decides whether neighbour links are still in L2 when
     // the next staple direction needs them
     int lds = 0;
-    // x3 layers per workgroup (1 = linear mapping).  With t3 > 1 the workgroup has
-    // 64*t3 threads; wave w of it takes 64 consecutive sites of x3-layer
-    // (x3 group * t3 + w), so x3 neighbours share the workgroup.
-    int t3 = 1;
+    // brick mapping (brick_site) when b1 >= 0: log2 of the rows per workgroup along
+    // x1, x2, x3; b1 < 0: linear mapping with `block` threads
+    int b1 = -1, b2 = 0, b3 = 0;
 };
 
-// site id (within a parity) of thread t of workgroup b under the mapping of cfg.t3;
-// slab = sites of one parity in one x3 layer
-SUN_HD int tiled_site(int b, int t, int t3, int slab) {
-    const int groups = slab >> 6;
-    const int rg = b % groups, x3g = b / groups;
-    return (x3g * t3 + (t >> 6)) * slab + (rg << 6) + (t & 63);
+// Brick mapping of workgroups to lattice sites.  The sites of one parity form rows
+// of L = X0/2 consecutive ids (fixed x1, x2, x3).  Workgroup b takes a brick of
+// 2^b1 x 2^b2 x 2^b3 rows along (x1, x2, x3) -- L * 2^(b1+b2+b3) threads, row
+// index x1-fastest inside the brick, bricks numbered x1-fastest -- so the sites
+// whose staples share links sit in one workgroup, on one CU, at the same time.
+// Returns the site id within the parity of thread t of workgroup b.
+SUN_HD int brick_site(const Geom& g, int b, int t, int b1, int b2, int b3) {
+    const int L = g.X[0] >> 1;
+    const int row = (int)g.dxh.div((uint32_t)t);
+    const int s = t - row * L;
+    const int i1 = row & ((1 << b1) - 1);
+    const int i2 = (row >> b1) & ((1 << b2) - 1);
+    const int i3 = row >> (b1 + b2);
+    const int n1 = g.X[1] >> b1, n2 = g.X[2] >> b2;
+    const int B23 = b / n1, B1 = b - B23 * n1;
+    const int B3 = B23 / n2, B2 = B23 - B3 * n2;
+    const int x1 = (B1 << b1) + i1, x2 = (B2 << b2) + i2, x3 = (B3 << b3) + i3;
+    return ((x3 * g.X[2] + x2) * g.X[1] + x1) * L + s;
 }
 
-inline bool t3_valid(const Geom& g, int t3) {
-    const int slab = g.X[0] * g.X[1] * g.X[2] / 2;
-    return t3 >= 1 && t3 <= 16 && (t3 == 1 || ((slab % 64) == 0 && (g.X[3] % t3) == 0));
+// threads per workgroup of a brick shape on this geometry, or 0 when the brick does
+// not tile the lattice or the workgroup is not whole wavefronts within max_block
+inline int brick_block(const Geom& g, int b1, int b2, int b3, int max_block) {
+    if (b1 < 0 || b2 < 0 || b3 < 0 || b1 > 6 || b2 > 6 || b3 > 6) return 0;
+    if ((g.X[1] % (1 << b1)) || (g.X[2] % (1 << b2)) || (g.X[3] % (1 << b3))) return 0;
+    const long long block = (long long)(g.X[0] / 2) << (b1 + b2 + b3);
+    if (block < 64 || block > max_block || (block % 64)) return 0;
+    return (int)block;
 }
 
 // One lattice of a batched launch.  `first_block` is the prefix sum of the blocks

@@ -17,12 +17,18 @@
 
 namespace sun {
 
-// candidate launch shapes: (block, t3) pairs, each with every LDS occupancy cap
+// candidate launch shapes, each with every LDS occupancy cap: linear mappings
+// (b1 < 0) of `block` threads, and bricks of 2^b1 x 2^b2 x 2^b3 rows (x1, x2, x3)
+// whose workgroup size follows from X0/2
 struct Shape {
-    int block, t3;
+    int block, b1, b2, b3;
 };
-static const Shape kShapes[] = {{64, 1}, {128, 1}, {256, 1}, {128, 2}, {256, 4}, {512, 8}, {1024, 16}};
-static const int kLds[] = {0, 16384, 32768, 40960, 54272};
+static const Shape kShapes[] = {{64, -1, 0, 0},  {128, -1, 0, 0}, {256, -1, 0, 0}, {512, -1, 0, 0},
+                                {0, 1, 1, 1},    {0, 2, 0, 2},    {0, 2, 1, 1},    {0, 1, 1, 2},
+                                {0, 2, 2, 1},    {0, 2, 1, 2},    {0, 1, 2, 2},    {0, 2, 2, 2},
+                                {0, 3, 1, 2},    {0, 2, 1, 3}};
+// dynamic LDS that no kernel reads: caps the workgroups per CU (160 KiB of LDS)
+static const int kLds[] = {0, 16384, 32768, 40960, 54272, 65536, 81920};
 
 std::string Tuner::key(const char* kernel, const sun_lattice_desc& d) const {
     char buf[256];
@@ -37,7 +43,8 @@ std::string Tuner::path() const {
     return std::string(dir) + "/sun_tunecache.json";
 }
 
-// The cache file is a JSON object {"key": [block, t3, lds, ms], ...}, one entry per line.
+// The cache file is a JSON object {"key": [block, b1, b2, b3, lds, ms], ...}, one
+// entry per line.
 void Tuner::load(int mode) {
     std::lock_guard<std::mutex> lk(mu_);
     mode_ = mode;
@@ -55,9 +62,11 @@ void Tuner::load(int mode) {
         if (q1 == std::string::npos || b0 == std::string::npos) continue;
         LaunchCfg c;
         float ms = 0.f;
-        if (sscanf(line.c_str() + b0, "[%d, %d, %d, %f]", &c.block, &c.t3, &c.lds, &ms) != 4) continue;
+        if (sscanf(line.c_str() + b0, "[%d, %d, %d, %d, %d, %f]", &c.block, &c.b1, &c.b2, &c.b3, &c.lds, &ms) != 6)
+            continue;
         bool ok = false;
-        for (const Shape& k : kShapes) ok |= (k.block == c.block && k.t3 == c.t3);
+        for (const Shape& k : kShapes)
+            ok |= k.b1 < 0 ? (c.b1 < 0 && k.block == c.block) : (k.b1 == c.b1 && k.b2 == c.b2 && k.b3 == c.b3);
         bool okl = false;
         for (int k : kLds) okl |= (k == c.lds);
         if (ok && okl) cache_[line.substr(q0 + 1, q1 - q0 - 1)] = {c, ms};
@@ -72,8 +81,9 @@ void Tuner::save() {
     out << "{\n";
     size_t i = 0;
     for (auto& kv : cache_) {
-        out << "  \"" << kv.first << "\": [" << kv.second.first.block << ", " << kv.second.first.t3 << ", "
-            << kv.second.first.lds << ", " << kv.second.second << "]"
+        const LaunchCfg& c = kv.second.first;
+        out << "  \"" << kv.first << "\": [" << c.block << ", " << c.b1 << ", " << c.b2 << ", " << c.b3 << ", "
+            << c.lds << ", " << kv.second.second << "]"
             << (++i < cache_.size() ? ",\n" : "\n");
     }
     out << "}\n";
@@ -98,8 +108,8 @@ void Tuner::record(const char* kernel, const sun_lattice_desc& d, const LaunchCf
     dirty_ = true;
 }
 
-// Times one (parity 0, mu 0) launch of the heatbath or overrelaxation kernel for
-// each candidate block size and stores the fastest in lat->cfg_hb / cfg_or.
+// Times the four (parity 0, mu) launches of the heatbath or overrelaxation kernel
+// for each candidate launch shape and stores the fastest in lat->cfg_hb / cfg_or.
 int tune_lattice(sun_lattice* lat, hipStream_t s) {
     sun_ctx* ctx = lat->ctx;
     if (!ctx->tuner.enabled() || lat->tuned) return SUN_OK;
@@ -117,25 +127,28 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
         float best_ms = 1e30f;
         const int nshape = (int)(sizeof(kShapes) / sizeof(Shape)), nlds = (int)(sizeof(kLds) / sizeof(int));
         for (int cand = 0; cand < nshape * nlds; cand++) {
+            const Shape& sh = kShapes[cand / nlds];
             LaunchCfg c;
-            c.block = kShapes[cand / nlds].block;
-            c.t3 = kShapes[cand / nlds].t3;
+            c.b1 = sh.b1;
+            c.b2 = sh.b2;
+            c.b3 = sh.b3;
+            c.block = sh.b1 < 0 ? sh.block : brick_block(lat->g, sh.b1, sh.b2, sh.b3, lat->ops->max_block);
             c.lds = kLds[cand % nlds];
-            if (!t3_valid(lat->g, c.t3) || c.block > lat->ops->max_block) continue;
-            // warm-up, then 3 timed launches
+            if (c.block == 0 || c.block > lat->ops->max_block) continue;
+            // warm-up, then 2 timed passes over mu = 0..3
             e = lat->ops->update(lat->f, lat->g, 0, 0, hb, 0, nullptr, bnc, 1, c, s);
             if (e == hipSuccess) e = hipEventRecord(e0, s);
-            for (int r = 0; r < 3 && e == hipSuccess; r++)
-                e = lat->ops->update(lat->f, lat->g, 0, 0, hb, 0, nullptr, bnc, 1, c, s);
+            for (int r = 0; r < 8 && e == hipSuccess; r++)
+                e = lat->ops->update(lat->f, lat->g, 0, r & 3, hb, 0, nullptr, bnc, 1, c, s);
             if (e == hipSuccess) e = hipEventRecord(e1, s);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.f;
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
             if (e != hipSuccess) break;
-            ms /= 3.f;
+            ms /= 8.f;
             if (ctx->verbosity >= SUN_VERBOSE)
-                fprintf(stderr, "[sun tune] %s block=%d t3=%d lds=%d %.4f ms\n", hb ? "heatbath" : "overrelax",
-                        c.block, c.t3, c.lds, ms);
+                fprintf(stderr, "[sun tune] %s block=%d brick=%d,%d,%d lds=%d %.4f ms\n",
+                        hb ? "heatbath" : "overrelax", c.block, c.b1, c.b2, c.b3, c.lds, ms);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = c;

@@ -43,6 +43,36 @@ int main(int argc, char** argv) {
         for (uint32_t n = 0x7fffffffu - 70000; n <= 0x7fffffffu - 1; n++)
             if (f.div(n) != n / d) { printf("FASTDIV FAIL %u %u\n", n, d); return 1; }
     }
+    // every brick shape that brick_block accepts maps (workgroup, thread) onto the
+    // sites of one parity exactly once, and the site's coordinates lie in its brick
+    for (int b1 = 0; b1 <= 4; b1++)
+        for (int b2 = 0; b2 <= 4; b2++)
+            for (int b3 = 0; b3 <= 4; b3++) {
+                const int block = sun::brick_block(g, b1, b2, b3, 1024);
+                if (!block) continue;
+                if (g.half % block) { printf("BRICK FAIL size %d %d %d\n", b1, b2, b3); return 1; }
+                const int grid = g.half / block;
+                std::vector<int> seen(g.half, 0);
+                for (int b = 0; b < grid; b++)
+                    for (int t = 0; t < block; t++) {
+                        const int id = sun::brick_site(g, b, t, b1, b2, b3);
+                        if (id < 0 || id >= g.half || seen[id]++) {
+                            printf("BRICK FAIL %d %d %d b=%d t=%d\n", b1, b2, b3, b, t);
+                            return 1;
+                        }
+                        int x[4];
+                        sun::site_coords(g, id, 0, x);
+                        const int lo1 = sun::brick_site(g, b, 0, b1, b2, b3);
+                        int y[4];
+                        sun::site_coords(g, lo1, 0, y);
+                        if (x[1] - y[1] < 0 || x[1] - y[1] >= (1 << b1) || x[2] - y[2] < 0 ||
+                            x[2] - y[2] >= (1 << b2) || x[3] - y[3] < 0 || x[3] - y[3] >= (1 << b3)) {
+                            printf("BRICK FAIL shape %d %d %d b=%d t=%d\n", b1, b2, b3, b, t);
+                            return 1;
+                        }
+                    }
+                fprintf(stderr, "brick %d %d %d ok (block %d)\n", b1, b2, b3, block);
+            }
     // XCD remap of workgroups is a bijection of [0, n) for every grid size
     for (int n = 1; n <= 5000; n++) {
         std::vector<int> seen(n, 0);
