@@ -45,7 +45,36 @@ struct Geom {
     int V, half;
     int str[4];          // lexicographic strides 1, X0, X0*X1, X0*X1*X2
     FastDiv dxh, dx1, dx2;  // division by X0/2, X1, X2
+    // brick (log2 rows along x1, x2, x3) of the 256-site reduction workgroups, or
+    // rb1 = -1 when no brick of 256 sites tiles the lattice (linear order then)
+    int rb1, rb2, rb3;
 };
+
+// The most cubic brick (least sum of squared log2 extents) of 256 sites of one parity: rows of X0/2 sites, 2^k rows
+// split over x1, x2, x3 so that each extent divides the lattice's.
+inline void reduction_brick(Geom& g) {
+    g.rb1 = -1;
+    g.rb2 = g.rb3 = 0;
+    const int L = g.X[0] / 2;
+    if (L <= 0 || 256 % L) return;
+    const int rows = 256 / L;
+    if (rows & (rows - 1)) return;
+    int k = 0;
+    while ((1 << k) < rows) k++;
+    int best = 99;
+    for (int b1 = k; b1 >= 0; b1--)
+        for (int b2 = k - b1; b2 >= 0; b2--) {
+            const int b3 = k - b1 - b2;
+            if ((g.X[1] % (1 << b1)) || (g.X[2] % (1 << b2)) || (g.X[3] % (1 << b3))) continue;
+            const int m = b1 * b1 + b2 * b2 + b3 * b3;
+            if (m < best) {
+                best = m;
+                g.rb1 = b1;
+                g.rb2 = b2;
+                g.rb3 = b3;
+            }
+        }
+}
 
 inline Geom make_geom(const int dims[4]) {
     Geom g;
@@ -59,6 +88,7 @@ inline Geom make_geom(const int dims[4]) {
     g.dxh = FastDiv::make((uint32_t)(dims[0] / 2));
     g.dx1 = FastDiv::make((uint32_t)dims[1]);
     g.dx2 = FastDiv::make((uint32_t)dims[2]);
+    reduction_brick(g);
     return g;
 }
 

@@ -43,6 +43,19 @@ SUN_HD int brick_site(const Geom& g, int b, int t, int b1, int b2, int b3) {
     return ((x3 * g.X[2] + x2) * g.X[1] + x1) * L + s;
 }
 
+constexpr int kReduceBlock = 256;   // threads of every reduction block
+
+// site (0 <= idd < V, parity-major) of thread t of reduction workgroup lb of a
+// lattice: parity-0 bricks first, then parity 1, each in brick_site order; linear
+// when the geometry has no reduction brick.  The single-lattice and the batched
+// plaquette kernels share it, so their per-workgroup partial sums are identical.
+SUN_HD int reduce_site(const Geom& g, int lb, int t) {
+    if (g.rb1 < 0) return lb * kReduceBlock + t;
+    const int nh = g.half / kReduceBlock;
+    const int parity = lb >= nh ? 1 : 0;
+    return parity * g.half + brick_site(g, lb - parity * nh, t, g.rb1, g.rb2, g.rb3);
+}
+
 // threads per workgroup of a brick shape on this geometry, or 0 when the brick does
 // not tile the lattice or the workgroup is not whole wavefronts within max_block
 inline int brick_block(const Geom& g, int b1, int b2, int b3, int max_block) {
@@ -94,8 +107,6 @@ struct OpsTable {
     hipError_t (*batch_polyakov_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     int max_block;   // largest workgroup the update kernels are compiled for
 };
-
-constexpr int kReduceBlock = 256;   // threads of every reduction block
 
 // gfx950 dispatches workgroup b of a grid to XCD (b mod 8); each XCD has its own
 // 4 MB L2.  xcd_remap turns the dispatch order into a contiguous range of tiles per

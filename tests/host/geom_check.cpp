@@ -73,6 +73,22 @@ int main(int argc, char** argv) {
                     }
                 fprintf(stderr, "brick %d %d %d ok (block %d)\n", b1, b2, b3, block);
             }
+    // the reduction workgroups cover every site once (threads past V in the linear
+    // order are the only ones left over)
+    {
+        const int nb = sun::plaquette_blocks(g);
+        std::vector<int> seen(g.V, 0);
+        int outside = 0;
+        for (int lb = 0; lb < nb; lb++)
+            for (int t = 0; t < sun::kReduceBlock; t++) {
+                const int idd = sun::reduce_site(g, lb, t);
+                if (idd >= g.V && g.rb1 < 0) { outside++; continue; }
+                if (idd < 0 || idd >= g.V || seen[idd]++) { printf("REDUCE FAIL lb=%d t=%d\n", lb, t); return 1; }
+            }
+        for (int i = 0; i < g.V; i++)
+            if (!seen[i]) { printf("REDUCE MISSING %d\n", i); return 1; }
+        fprintf(stderr, "reduce brick %d %d %d ok\n", g.rb1, g.rb2, g.rb3);
+    }
     // XCD remap of workgroups is a bijection of [0, n) for every grid size
     for (int n = 1; n <= 5000; n++) {
         std::vector<int> seen(n, 0);
