@@ -276,6 +276,71 @@ SUN_HD F3 staple_su3(const Field& f, const Geom& g, int id, int parity, const in
     return S;
 }
 
+// staple sum (monte.cu:63-80) for a compile-time direction MU, software-pipelined:
+// the six links of direction nu_{k+1} are loaded into one register set while the
+// other set's direction nu_k is multiplied, so a wavefront keeps ~18 16-byte loads
+// in flight instead of ~4.  Needs ~220 VGPRs: for kernels run at 2 waves per SIMD,
+// the occupancy at which the L2 still holds the neighbour links between their
+// uses.  Same products in the same order as staple_su3 (identical results).
+template <int STORE, int MU>
+SUN_HD F3 staple_su3_piped(const Field& f, const Geom& g, int id, int parity, const int x[4], int lex) {
+    const uint32_t V = (uint32_t)g.V;
+    const int opar = parity ^ 1;
+    const uint32_t self = (uint32_t)(id + parity * g.half);
+    int xm[4] = {x[0], x[1], x[2], x[3]};
+    const uint32_t eo_pmu = (uint32_t)eo_of_lex(g, lex_shift(g, lex, xm, MU, +1), opar);
+    constexpr int N0 = MU == 0 ? 1 : 0;
+    constexpr int N1 = MU <= 1 ? 2 : 1;
+    constexpr int N2 = MU <= 2 ? 3 : 2;
+    constexpr int NUS[3] = {N0, N1, N2};
+    uint32_t eo_pnu[3], eo_mnu[3], eo_mnu_pmu[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int nu = NUS[k];
+        int xn[4] = {x[0], x[1], x[2], x[3]};
+        eo_pnu[k] = (uint32_t)eo_of_lex(g, lex_shift(g, lex, xn, nu, +1), opar);
+        int xd[4] = {x[0], x[1], x[2], x[3]};
+        const int lex_mnu = lex_shift(g, lex, xd, nu, -1);
+        eo_mnu[k] = (uint32_t)eo_of_lex(g, lex_mnu, opar);
+        eo_mnu_pmu[k] = (uint32_t)eo_of_lex(g, lex_shift(g, lex_mnu, xd, MU, +1), parity);
+    }
+    // register sets [slot]: U_mu(x+nu), U_mu(x-nu), U_nu(x), U_nu(x+mu), U_nu(x-nu), U_nu(x-nu+mu)
+    F3 Bu[2], Bd[2], Au[2], Cu[2], Ad[2], Cd[2];
+#define SUN_PIPE_LOAD(k, slot)                                                          \
+    do {                                                                                \
+        const uint32_t nuV = (uint32_t)NUS[k] * V;                                      \
+        Bu[slot] = load_rows<STORE>(f, (uint32_t)MU * V + eo_pnu[k]);                  \
+        Bd[slot] = load_rows<STORE>(f, (uint32_t)MU * V + eo_mnu[k]);                  \
+        Au[slot] = load_rows<STORE>(f, nuV + self);                                     \
+        Cu[slot] = load_rows<STORE>(f, nuV + eo_pmu);                                   \
+        Ad[slot] = load_rows<STORE>(f, nuV + eo_mnu[k]);                                \
+        Cd[slot] = load_rows<STORE>(f, nuV + eo_mnu_pmu[k]);                            \
+    } while (0)
+    SUN_PIPE_LOAD(0, 0);
+    F3 S;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int cur = k & 1;
+        if (k + 1 < 3) SUN_PIPE_LOAD(k + 1, cur ^ 1);
+        F3 W, T;
+        need_row2<STORE>(Cu[cur]);
+        mul_2r_bdag(Bu[cur], Cu[cur], W);
+        complete_row2(W);
+        mul_2r(Au[cur], W, T);
+        complete_row2(T);
+        if (k == 0) S = T; else add_full(S, T);
+        need_row2<STORE>(Cd[cur]);
+        need_row2<STORE>(Ad[cur]);
+        mul_2r(Bd[cur], Cd[cur], W);
+        complete_row2(W);
+        mul_2r_adag(Ad[cur], W, T);
+        complete_row2(T);
+        add_full(S, T);
+    }
+#undef SUN_PIPE_LOAD
+    return S;
+}
+
 // rows p,q of U times columns p,q of F = S^+, projected on SU(2):
 //   a0=(UF)_pp a1=(UF)_pq a2=(UF)_qp a3=(UF)_qq, (UF)_ab = sum_j U_aj conj(S_bj)
 SUN_HD Quat<float> block_proj(const F3& U, const F3& S, int p, int q) {

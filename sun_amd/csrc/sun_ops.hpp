@@ -21,6 +21,8 @@ struct LaunchCfg {
     // brick mapping (brick_site) when b1 >= 0: log2 of the rows per workgroup along
     // x1, x2, x3; b1 < 0: linear mapping with `block` threads
     int b1 = -1, b2 = 0, b3 = 0;
+    // 1: the software-pipelined overrelaxation kernel (where the variant has one)
+    int piped = 0;
 };
 
 // Brick mapping of workgroups to lattice sites.  The sites of one parity form rows
@@ -106,6 +108,8 @@ struct OpsTable {
     hipError_t (*batch_plaquette_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     hipError_t (*batch_polyakov_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     int max_block;   // largest workgroup the update kernels are compiled for
+    int has_piped;   // the variant has the software-pipelined overrelaxation kernel
+    int piped_max_block;
 };
 
 // gfx950 dispatches workgroup b of a grid to XCD (b mod 8); each XCD has its own

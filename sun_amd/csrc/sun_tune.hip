@@ -43,8 +43,8 @@ std::string Tuner::path() const {
     return std::string(dir) + "/sun_tunecache.json";
 }
 
-// The cache file is a JSON object {"key": [block, b1, b2, b3, lds, ms], ...}, one
-// entry per line.
+// The cache file is a JSON object {"key": [block, b1, b2, b3, lds, piped, ms], ...},
+// one entry per line.
 void Tuner::load(int mode) {
     std::lock_guard<std::mutex> lk(mu_);
     mode_ = mode;
@@ -62,7 +62,8 @@ void Tuner::load(int mode) {
         if (q1 == std::string::npos || b0 == std::string::npos) continue;
         LaunchCfg c;
         float ms = 0.f;
-        if (sscanf(line.c_str() + b0, "[%d, %d, %d, %d, %d, %f]", &c.block, &c.b1, &c.b2, &c.b3, &c.lds, &ms) != 6)
+        if (sscanf(line.c_str() + b0, "[%d, %d, %d, %d, %d, %d, %f]", &c.block, &c.b1, &c.b2, &c.b3, &c.lds, &c.piped,
+                   &ms) != 7)
             continue;
         bool ok = false;
         for (const Shape& k : kShapes)
@@ -83,7 +84,7 @@ void Tuner::save() {
     for (auto& kv : cache_) {
         const LaunchCfg& c = kv.second.first;
         out << "  \"" << kv.first << "\": [" << c.block << ", " << c.b1 << ", " << c.b2 << ", " << c.b3 << ", "
-            << c.lds << ", " << kv.second.second << "]"
+            << c.lds << ", " << c.piped << ", " << kv.second.second << "]"
             << (++i < cache_.size() ? ",\n" : "\n");
     }
     out << "}\n";
@@ -121,8 +122,14 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     const double bnc = lat->d.beta / lat->d.nc;
-    for (int which = 0; which < 2 && e == hipSuccess; which++) {
+    // 0: heatbath, 1: overrelaxation, 2: software-pipelined overrelaxation
+    const int kinds = lat->ops->has_piped ? 3 : 2;
+    LaunchCfg best_or;
+    float best_or_ms = 1e30f;
+    for (int which = 0; which < kinds && e == hipSuccess; which++) {
         const bool hb = which == 0;
+        const int piped = which == 2 ? 1 : 0;
+        const int max_block = piped ? lat->ops->piped_max_block : lat->ops->max_block;
         LaunchCfg best;
         float best_ms = 1e30f;
         const int nshape = (int)(sizeof(kShapes) / sizeof(Shape)), nlds = (int)(sizeof(kLds) / sizeof(int));
@@ -132,9 +139,10 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
             c.b1 = sh.b1;
             c.b2 = sh.b2;
             c.b3 = sh.b3;
-            c.block = sh.b1 < 0 ? sh.block : brick_block(lat->g, sh.b1, sh.b2, sh.b3, lat->ops->max_block);
+            c.block = sh.b1 < 0 ? sh.block : brick_block(lat->g, sh.b1, sh.b2, sh.b3, max_block);
             c.lds = kLds[cand % nlds];
-            if (c.block == 0 || c.block > lat->ops->max_block) continue;
+            c.piped = piped;
+            if (c.block == 0 || c.block > max_block) continue;
             // warm-up, then 2 timed passes over mu = 0..3
             e = lat->ops->update(lat->f, lat->g, 0, 0, hb, 0, nullptr, bnc, 1, c, s);
             if (e == hipSuccess) e = hipEventRecord(e0, s);
@@ -148,16 +156,24 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
             ms /= 8.f;
             if (ctx->verbosity >= SUN_VERBOSE)
                 fprintf(stderr, "[sun tune] %s block=%d brick=%d,%d,%d lds=%d %.4f ms\n",
-                        hb ? "heatbath" : "overrelax", c.block, c.b1, c.b2, c.b3, c.lds, ms);
+                        hb ? "heatbath" : piped ? "overrelax-piped" : "overrelax", c.block, c.b1, c.b2, c.b3, c.lds,
+                        ms);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = c;
             }
         }
-        if (e == hipSuccess) {
-            (hb ? lat->cfg_hb : lat->cfg_or) = best;
-            ctx->tuner.record(hb ? "heatbath" : "overrelax", lat->d, best, best_ms);
+        if (e == hipSuccess && hb) {
+            lat->cfg_hb = best;
+            ctx->tuner.record("heatbath", lat->d, best, best_ms);
+        } else if (e == hipSuccess && best_ms < best_or_ms) {
+            best_or = best;
+            best_or_ms = best_ms;
         }
+    }
+    if (e == hipSuccess) {
+        lat->cfg_or = best_or;
+        ctx->tuner.record("overrelax", lat->d, best_or, best_or_ms);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(lat->buf, backup, lat->bytes, hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
