@@ -222,8 +222,11 @@ def kernels_only(sun_amd):
     """For rocprofv3 counter passes: on the 32^4 FP32 12-real field after a few
     iterations, 2 copies of the field (calibration: a plain copy of known size),
     then 8 overrelaxation launches, 8 heatbath launches, reunitarize, plaquette,
-    Polyakov loop."""
-    ctx = sun_amd.Context(0, 0, 0)
+    Polyakov loop.  Launch shapes come from the tune cache in $SUN_RESOURCE_PATH
+    (written by a preceding full bench.py run with the same variable), so the
+    profiled kernels are the ones the benchmark times; without a cache the
+    autotuner runs first and its trial launches appear in the profile too."""
+    ctx = sun_amd.Context(0, 0, 1)
     lat = sun_amd.Lattice(ctx, 3, DIMS, BETA, 12, 4)
     tmp = sun_amd.Lattice(ctx, 3, DIMS, BETA, 12, 4)
     s = sun_amd.new_stream()

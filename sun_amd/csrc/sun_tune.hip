@@ -122,13 +122,13 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     const double bnc = lat->d.beta / lat->d.nc;
-    // 0: heatbath, 1: overrelaxation, 2: software-pipelined overrelaxation
-    const int kinds = lat->ops->has_piped ? 3 : 2;
-    LaunchCfg best_or;
-    float best_or_ms = 1e30f;
-    for (int which = 0; which < kinds && e == hipSuccess; which++) {
-        const bool hb = which == 0;
-        const int piped = which == 2 ? 1 : 0;
+    // (heatbath, overrelaxation) x (plain, software-pipelined where the variant has it)
+    LaunchCfg best_k[2];
+    float best_k_ms[2] = {1e30f, 1e30f};
+    for (int which = 0; which < 4 && e == hipSuccess; which++) {
+        const bool hb = (which & 1) == 0;
+        const int piped = which >> 1;
+        if (piped && !lat->ops->has_piped) continue;
         const int max_block = piped ? lat->ops->piped_max_block : lat->ops->max_block;
         LaunchCfg best;
         float best_ms = 1e30f;
@@ -156,24 +156,23 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
             ms /= 8.f;
             if (ctx->verbosity >= SUN_VERBOSE)
                 fprintf(stderr, "[sun tune] %s block=%d brick=%d,%d,%d lds=%d %.4f ms\n",
-                        hb ? "heatbath" : piped ? "overrelax-piped" : "overrelax", c.block, c.b1, c.b2, c.b3, c.lds,
-                        ms);
+                        hb ? (piped ? "heatbath-piped" : "heatbath") : (piped ? "overrelax-piped" : "overrelax"),
+                        c.block, c.b1, c.b2, c.b3, c.lds, ms);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = c;
             }
         }
-        if (e == hipSuccess && hb) {
-            lat->cfg_hb = best;
-            ctx->tuner.record("heatbath", lat->d, best, best_ms);
-        } else if (e == hipSuccess && best_ms < best_or_ms) {
-            best_or = best;
-            best_or_ms = best_ms;
+        if (e == hipSuccess && best_ms < best_k_ms[hb ? 0 : 1]) {
+            best_k[hb ? 0 : 1] = best;
+            best_k_ms[hb ? 0 : 1] = best_ms;
         }
     }
     if (e == hipSuccess) {
-        lat->cfg_or = best_or;
-        ctx->tuner.record("overrelax", lat->d, best_or, best_or_ms);
+        lat->cfg_hb = best_k[0];
+        lat->cfg_or = best_k[1];
+        ctx->tuner.record("heatbath", lat->d, best_k[0], best_k_ms[0]);
+        ctx->tuner.record("overrelax", lat->d, best_k[1], best_k_ms[1]);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(lat->buf, backup, lat->bytes, hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
