@@ -294,6 +294,7 @@ def main():
     # dominant kernel: per-launch durations on the same stream
     or_ms = time_launches(sun_amd, lat, stream, hb=False, reps=64)
     hb_ms = time_launches(sun_amd, lat, stream, hb=True, reps=32)
+    launch_or, launch_hb = lat.get_launch(1).as_dict(), lat.get_launch(0).as_dict()
     B = link_bytes(3, storage, prec)
     per_launch = 9 * B * (V // 2)
     or_gbs = per_launch / (or_ms * 1e-3) / 1e9
@@ -324,6 +325,7 @@ def main():
                      "kernel": "overrelaxation (parity, mu) launch",
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": or_ms,
                      "heatbath_launch_ms": hb_ms, "heatbath_gbs": hb_gbs,
+                     "launch_shape": launch_or, "heatbath_launch_shape": launch_hb,
                      "iteration_frac": iter_bytes / (step_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "observables": {"plaquette_last": float(obs[-1, :2].mean()), "polyakov_last": [float(obs[-1, 2]),
                                                                                         float(obs[-1, 3])]},

@@ -87,6 +87,21 @@ int sun_lattice_destroy(sun_lattice *lat);
 int sun_lattice_copy(sun_lattice *dst, const sun_lattice *src, void *stream);
 /* device bytes of the link field (gaugearray.h:447-464 Bytes()) */
 size_t sun_lattice_bytes(const sun_lattice_desc *desc);
+
+/* Launch shape of a lattice's heatbath (kernel 0) or overrelaxation (kernel 1)
+ * launches: the autotuner's choice, or one set by the caller.  brick[0] < 0 selects
+ * the linear site order with `block` threads; otherwise a workgroup covers
+ * 2^brick[0] x 2^brick[1] x 2^brick[2] rows of X0/2 sites along (x1, x2, x3).
+ * lds: unused dynamic LDS per workgroup (caps workgroups per CU).  piped: the
+ * software-pipelined kernel (FP32 12-real SU(3) only).  Replaces the reference's
+ * TuneParam override (include/tune.h:36-118); results never depend on it. */
+typedef struct {
+    int block, brick[3], lds, piped;
+} sun_launch;
+int sun_lattice_get_launch(const sun_lattice *lat, int kernel, sun_launch *out);
+/* SUN_ERR_INVALID for a shape the lattice's kernels cannot run; marks the lattice
+ * tuned, so the autotuner keeps the shape */
+int sun_lattice_set_launch(sun_lattice *lat, int kernel, const sun_launch *cfg);
 /* _gauge<Real>::Init() -> ColdStart; refuses 8-real storage as the reference does
  *                                                gaugearray.cu:276-305, gaugearray.h:765-775 */
 int sun_cold_start(sun_lattice *lat, void *stream);

@@ -38,6 +38,7 @@ _lib = None
 EXPORTS = [
     "sun_init", "sun_finalize", "sun_last_error", "sun_version",
     "sun_lattice_create", "sun_lattice_destroy", "sun_lattice_bytes", "sun_lattice_copy",
+    "sun_lattice_get_launch", "sun_lattice_set_launch",
     "sun_cold_start", "sun_hot_start",
     "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
     "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
@@ -74,6 +75,16 @@ class GenerateDesc(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("first_sweep", ctypes.c_uint32), ("n_iter", ctypes.c_int),
                 ("n_or", ctypes.c_int), ("reunit_every", ctypes.c_int), ("measure", ctypes.c_int),
                 ("use_graph", ctypes.c_int)]
+
+
+class Launch(ctypes.Structure):
+    """sun_launch: block, brick (log2 rows along x1, x2, x3; brick[0] < 0 = linear),
+    lds (unused dynamic LDS per workgroup), piped (software-pipelined kernel)"""
+    _fields_ = [("block", ctypes.c_int), ("brick", ctypes.c_int * 3), ("lds", ctypes.c_int),
+                ("piped", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"block": self.block, "brick": list(self.brick), "lds": self.lds, "piped": self.piped}
 
 
 class JobDesc(ctypes.Structure):
@@ -118,6 +129,8 @@ def lib() -> ctypes.CDLL:
         "sun_lattice_destroy": (ci, [vp]),
         "sun_lattice_bytes": (ctypes.c_size_t, [ctypes.POINTER(LatticeDesc)]),
         "sun_lattice_copy": (ci, [vp, vp, vp]),
+        "sun_lattice_get_launch": (ci, [vp, ci, ctypes.POINTER(Launch)]),
+        "sun_lattice_set_launch": (ci, [vp, ci, ctypes.POINTER(Launch)]),
         "sun_cold_start": (ci, [vp, vp]),
         "sun_hot_start": (ci, [vp, u64, vp]),
         "sun_heatbath": (ci, [vp, u64, u32, vp]),
@@ -242,6 +255,16 @@ class Lattice:
         return lattice_bytes(self.nc, self.dims, self.storage, self.precision, self.beta)
 
     # -- initialisation
+    # -- launch shapes (kernel 0: heatbath, 1: overrelaxation)
+    def get_launch(self, kernel: int) -> Launch:
+        out = Launch()
+        _check(lib().sun_lattice_get_launch(self.h, kernel, ctypes.byref(out)), "sun_lattice_get_launch")
+        return out
+
+    def set_launch(self, kernel: int, block: int = 256, brick=(-1, 0, 0), lds: int = 0, piped: bool = False):
+        c = Launch(block, (ctypes.c_int * 3)(*brick), lds, int(piped))
+        _check(lib().sun_lattice_set_launch(self.h, kernel, ctypes.byref(c)), "sun_lattice_set_launch")
+
     def cold_start(self, stream=None):
         _check(lib().sun_cold_start(self.h, stream), "sun_cold_start")
 

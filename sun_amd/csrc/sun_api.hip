@@ -266,6 +266,50 @@ extern "C" int sun_lattice_copy(sun_lattice* dst, const sun_lattice* src, void* 
     return SUN_OK;
 }
 
+extern "C" int sun_lattice_get_launch(const sun_lattice* lat, int kernel, sun_launch* out) {
+    if (!lat || !out || (kernel != 0 && kernel != 1)) return fail(SUN_ERR_INVALID, "sun_lattice_get_launch: bad argument");
+    const LaunchCfg& c = kernel == 0 ? lat->cfg_hb : lat->cfg_or;
+    out->block = c.b1 >= 0 ? brick_block(lat->g, c.b1, c.b2, c.b3, c.piped ? lat->ops->piped_max_block
+                                                                              : lat->ops->max_block)
+                           : c.block;
+    out->brick[0] = c.b1;
+    out->brick[1] = c.b2;
+    out->brick[2] = c.b3;
+    out->lds = c.lds;
+    out->piped = c.piped;
+    return SUN_OK;
+}
+
+extern "C" int sun_lattice_set_launch(sun_lattice* lat, int kernel, const sun_launch* cfg) {
+    if (!lat || !cfg || (kernel != 0 && kernel != 1)) return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: bad argument");
+    if (cfg->piped && !lat->ops->has_piped)
+        return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: no pipelined kernel for this N/storage/precision");
+    const int max_block = cfg->piped ? lat->ops->piped_max_block : lat->ops->max_block;
+    LaunchCfg c;
+    c.piped = cfg->piped ? 1 : 0;
+    c.lds = cfg->lds;
+    if (cfg->lds < 0 || cfg->lds > 65536 * 2) return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: lds %d", cfg->lds);
+    if (cfg->brick[0] >= 0) {
+        const int block = brick_block(lat->g, cfg->brick[0], cfg->brick[1], cfg->brick[2], max_block);
+        if (!block)
+            return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: brick %d,%d,%d does not tile %dx%dx%dx%d",
+                        cfg->brick[0], cfg->brick[1], cfg->brick[2], lat->d.dims[0], lat->d.dims[1], lat->d.dims[2],
+                        lat->d.dims[3]);
+        c.b1 = cfg->brick[0];
+        c.b2 = cfg->brick[1];
+        c.b3 = cfg->brick[2];
+        c.block = block;
+    } else {
+        if (cfg->block < 64 || cfg->block > max_block || (cfg->block & 63))
+            return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: block %d (multiple of 64, <= %d)", cfg->block,
+                        max_block);
+        c.block = cfg->block;
+    }
+    (kernel == 0 ? lat->cfg_hb : lat->cfg_or) = c;
+    lat->tuned = true;
+    return SUN_OK;
+}
+
 hipStream_t sun::pick_stream(sun_lattice* lat, void* stream) {
     return stream ? (hipStream_t)stream : lat->ctx->stream;
 }

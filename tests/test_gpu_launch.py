@@ -1,0 +1,101 @@
+"""Every launch shape the autotuner can pick gives oracle-parity results.
+
+The parity tests elsewhere run the default shape (256-thread linear workgroups).
+Here each (N, storage, precision) variant runs one overrelaxation and one heatbath
+sweep under every shape that `sun_lattice_set_launch` accepts from a list covering
+linear workgroups of 64..512 threads, brick mappings along (x1, x2, x3), LDS
+occupancy caps and the software-pipelined kernels, on two geometries whose X0/2 is
+4 and 16 sites, and compares each with the FP64 oracle at the usual bars.  Shapes a
+variant cannot run must be refused with SUN_ERR_INVALID.
+"""
+import numpy as np
+import pytest
+
+import sun_amd
+from test_gpu_parity import BETAS, IDS, VARIANTS, load_config, per_link_diff, thermalised, tol
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [  # (block, brick, lds, piped)
+    (64, (-1, 0, 0), 0, 0),
+    (128, (-1, 0, 0), 16384, 0),
+    (512, (-1, 0, 0), 0, 0),
+    (0, (1, 1, 1), 0, 0),
+    (0, (2, 1, 1), 0, 0),
+    (0, (2, 0, 2), 54272, 0),
+    (0, (1, 2, 2), 0, 0),
+    (0, (2, 2, 2), 0, 0),
+    (256, (-1, 0, 0), 0, 1),
+    (0, (2, 0, 2), 0, 1),
+    (0, (1, 1, 2), 32768, 1),
+    (0, (2, 1, 2), 0, 1),
+]
+DIMS = [(8, 8, 8, 16), (32, 4, 4, 8)]
+
+
+def accepted_shapes(lat, kernel):
+    out = []
+    for block, brick, lds, piped in SHAPES:
+        try:
+            lat.set_launch(kernel, block, brick, lds, bool(piped))
+        except sun_amd.SunError as e:
+            assert e.code == sun_amd.ERR_INVALID
+            continue
+        got = lat.get_launch(kernel)
+        assert list(got.brick) == list(brick) and got.lds == lds and got.piped == piped
+        if brick[0] < 0:
+            assert got.block == block
+        out.append((block, brick, lds, piped))
+    return out
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+@pytest.mark.parametrize("dims", DIMS)
+def test_every_shape_overrelax_parity(ctx, oracle, nc, storage, prec, dims):
+    U0 = thermalised(oracle, nc, dims, storage)
+    with sun_amd.Lattice(ctx, nc, dims, BETAS[nc], storage, prec) as lat:
+        U = load_config(lat, U0)
+        ref = U.copy()
+        oracle.overrelax_sweep(ref, dims, storage)
+        shapes = accepted_shapes(lat, 1)
+        assert len(shapes) >= 3
+        for block, brick, lds, piped in shapes:
+            lat.set_launch(1, block, brick, lds, bool(piped))
+            lat.from_host(U0)
+            lat.overrelax()
+            G = lat.to_host()
+            assert np.abs(G - ref).max() <= tol(prec), (block, brick, lds, piped)
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+@pytest.mark.parametrize("dims", DIMS)
+def test_every_shape_heatbath_parity(ctx, oracle, nc, storage, prec, dims):
+    beta, seed, sweep = BETAS[nc], 4321, 3
+    U0 = thermalised(oracle, nc, dims, storage)
+    with sun_amd.Lattice(ctx, nc, dims, beta, storage, prec) as lat:
+        U = load_config(lat, U0)
+        ref = U.copy()
+        oracle.heatbath_sweep(ref, dims, beta, seed, sweep, storage)
+        shapes = accepted_shapes(lat, 0)
+        assert len(shapes) >= 3
+        for block, brick, lds, piped in shapes:
+            lat.set_launch(0, block, brick, lds, bool(piped))
+            lat.from_host(U0)
+            lat.heatbath(seed, sweep)
+            d = per_link_diff(lat.to_host(), ref)
+            if prec == 8:
+                assert d.max() <= 1e-12, (block, brick, lds, piped)
+            else:
+                bad = int((d > 5e-5).sum())
+                assert bad <= max(4, d.size // 200), (block, brick, lds, piped, bad)
+
+
+def test_piped_refused_without_kernel(ctx):
+    with sun_amd.Lattice(ctx, 3, (8, 8, 8, 8), 6.0, 18, 8) as lat:
+        with pytest.raises(sun_amd.SunError) as e:
+            lat.set_launch(1, 256, (-1, 0, 0), 0, True)
+        assert e.value.code == sun_amd.ERR_INVALID
+        with pytest.raises(sun_amd.SunError):
+            lat.set_launch(1, 96)  # not whole wavefronts
+        with pytest.raises(sun_amd.SunError):
+            lat.set_launch(1, 0, (3, 3, 3))  # 4 * 2^9 threads
