@@ -282,8 +282,9 @@ SUN_HD F3 staple_su3(const Field& f, const Geom& g, int id, int parity, const in
 // in flight instead of ~4.  Needs ~220 VGPRs: for kernels run at 2 waves per SIMD,
 // the occupancy at which the L2 still holds the neighbour links between their
 // uses.  Same products in the same order as staple_su3 (identical results).
+// The link being updated, U_MU(x), is loaded with the last direction's set into Uown.
 template <int STORE, int MU>
-SUN_HD F3 staple_su3_piped(const Field& f, const Geom& g, int id, int parity, const int x[4], int lex) {
+SUN_HD F3 staple_su3_piped(const Field& f, const Geom& g, int id, int parity, const int x[4], int lex, F3& Uown) {
     const uint32_t V = (uint32_t)g.V;
     const int opar = parity ^ 1;
     const uint32_t self = (uint32_t)(id + parity * g.half);
@@ -322,6 +323,7 @@ SUN_HD F3 staple_su3_piped(const Field& f, const Geom& g, int id, int parity, co
     for (int k = 0; k < 3; k++) {
         const int cur = k & 1;
         if (k + 1 < 3) SUN_PIPE_LOAD(k + 1, cur ^ 1);
+        if (k == 1) Uown = load_rows<STORE>(f, (uint32_t)MU * V + self);
         F3 W, T;
         need_row2<STORE>(Cu[cur]);
         mul_2r_bdag(Bu[cur], Cu[cur], W);
