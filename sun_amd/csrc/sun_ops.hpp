@@ -125,9 +125,13 @@ SUN_HD int xcd_remap(int b, int nblocks) {
 }
 
 inline int plaquette_blocks(const Geom& g) { return (g.V + kReduceBlock - 1) / kReduceBlock; }
+// Polyakov loop workgroups: kReduceBlock threads = kPolySites spatial sites x
+// kPolySegs segments of each time line
+constexpr int kPolySegs = 4;
+constexpr int kPolySites = kReduceBlock / kPolySegs;
 inline int polyakov_blocks(const Geom& g) {
     int ts = g.X[0] * g.X[1] * g.X[2];
-    return (ts + kReduceBlock - 1) / kReduceBlock;
+    return (ts + kPolySites - 1) / kPolySites;
 }
 inline int update_blocks(const Geom& g, int block) { return (g.half + block - 1) / block; }
 inline int link_blocks(const Geom& g) { return (4 * g.V + kReduceBlock - 1) / kReduceBlock; }
