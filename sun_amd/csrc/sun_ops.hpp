@@ -48,14 +48,14 @@ SUN_HD int brick_site(const Geom& g, int b, int t, int b1, int b2, int b3) {
 constexpr int kReduceBlock = 256;   // threads of every reduction block
 
 // site (0 <= idd < V, parity-major) of thread t of reduction workgroup lb of a
-// lattice: parity-0 bricks first, then parity 1, each in brick_site order; linear
-// when the geometry has no reduction brick.  The single-lattice and the batched
-// plaquette kernels share it, so their per-workgroup partial sums are identical.
+// lattice: workgroups 2k and 2k+1 take brick k of parity 0 and of parity 1 (whose
+// links are each other's neighbours, so they run on the same XCD and share its L2);
+// linear when the geometry has no reduction brick.  The single-lattice and the
+// batched plaquette kernels share it, so their per-workgroup partial sums agree.
 SUN_HD int reduce_site(const Geom& g, int lb, int t) {
     if (g.rb1 < 0) return lb * kReduceBlock + t;
-    const int nh = g.half / kReduceBlock;
-    const int parity = lb >= nh ? 1 : 0;
-    return parity * g.half + brick_site(g, lb - parity * nh, t, g.rb1, g.rb2, g.rb3);
+    const int parity = lb & 1;
+    return parity * g.half + brick_site(g, lb >> 1, t, g.rb1, g.rb2, g.rb3);
 }
 
 // threads per workgroup of a brick shape on this geometry, or 0 when the brick does
