@@ -32,6 +32,24 @@ DIMS = (32, 32, 32, 32)
 BETA = 6.0
 
 
+def profile_traffic(shape):
+    """HBM bytes per overrelaxation launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes (calibrated against a device copy) for this launch shape, from the summary
+    committed under profiles/; None when no summary matches the shape."""
+    key = (f"kernel=overrelaxation block={shape['block']} brick={','.join(map(str, shape['brick']))} "
+           f"lds={shape['lds']} piped={shape['piped']} bytes=")
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None, None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".md"):
+            continue
+        for line in open(os.path.join(pdir, name)):
+            if line.startswith("traffic-per-launch: ") and key in line:
+                return int(line.split("bytes=")[1].split()[0]), f"profiles/{name}"
+    return None, None
+
+
 def link_bytes(nc, storage, prec):
     return (2 * nc * nc if storage == 18 else storage) * prec
 
@@ -297,6 +315,7 @@ def main():
     or_ms = time_launches(sun_amd, lat, stream, hb=False, reps=64)
     hb_ms = time_launches(sun_amd, lat, stream, hb=True, reps=32)
     launch_or, launch_hb = lat.get_launch(1).as_dict(), lat.get_launch(0).as_dict()
+    traffic, traffic_src = profile_traffic(launch_or)
     B = link_bytes(3, storage, prec)
     per_launch = 9 * B * (V // 2)
     or_gbs = per_launch / (or_ms * 1e-3) / 1e9
@@ -323,7 +342,7 @@ def main():
                    "lattice": list(DIMS), "beta": BETA, "storage": storage, "precision": "f32",
                    "updates_per_step": updates_per_step},
         "roofline": {"bound": "hbm", "achieved": or_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": or_gbs / PEAK_HBM_GBS, "traffic": None,
+                     "frac": or_gbs / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "overrelaxation (parity, mu) launch",
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": or_ms,
                      "heatbath_launch_ms": hb_ms, "heatbath_gbs": hb_gbs,
