@@ -43,7 +43,7 @@ def lib() -> ctypes.CDLL:
     u64, u32, ci, cd = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_double
     sig = {
         "or_philox4x32_10": (None, [u32p, u32p, u32p]),
-        "or_u01": (cd, [u32, u32]),
+        "or_u01": (cd, [u32]),
         "or_stream_uniforms": (None, [u64, u32, u32, u32, ci, dp]),
         "or_generate_su2_milc": (ci, [cd, dp, ci, dp]),
         "or_sample_a0": (None, [cd, u64, ci, dp]),
@@ -112,8 +112,9 @@ def philox(ctr, key):
     return tuple(int(x) for x in o)
 
 
-def u01(lo: int, hi: int) -> float:
-    return lib().or_u01(lo, hi)
+def u01(w: int) -> float:
+    """uniform in (0,1] from one 32-bit word: (w + 1) * 2^-32"""
+    return lib().or_u01(w)
 
 
 def stream_uniforms(seed: int, c0: int, c2: int, c3: int, n: int) -> np.ndarray:

@@ -151,25 +151,27 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-double or_u01(uint32_t lo, uint32_t hi) {
-    uint64_t m = ((((uint64_t)hi) << 32) | lo) >> 11;
-    return (double)(m + 1) * 0x1p-53;
+/* Uniform double in (0,1] from one 32-bit Philox word: (w + 1) * 2^-32 (32 bits of
+ * randomness per draw, as cuRAND's MRG32k3a uniforms have, random.h:16-22). */
+double or_u01(uint32_t w) {
+    return ((double)w + 1.0) * 0x1p-32;
 }
 
-/* uniform source: a Philox stream, or an explicit list (tests) */
+/* uniform source: a Philox stream, or an explicit list (tests).  Draw k of stream
+ * (seed, c0, c2, c3) is word k%4 of Philox4x32-10((c0, k/4, c2, c3), seed). */
 typedef struct {
     uint32_t key[2];
     uint32_t c0, c2, c3, j;
-    double cache;
-    int has;
+    uint32_t w[4];
+    int nw;
     const double *list;
     int nlist, pos;
 } usrc;
 
 static void usrc_stream(usrc *s, uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3) {
     s->key[0] = (uint32_t)seed; s->key[1] = (uint32_t)(seed >> 32);
-    s->c0 = c0; s->c2 = c2; s->c3 = c3; s->j = 0; s->has = 0; s->list = NULL;
-    s->nlist = 0; s->pos = 0; s->cache = 0.0;
+    s->c0 = c0; s->c2 = c2; s->c3 = c3; s->j = 0; s->nw = 0; s->list = NULL;
+    s->nlist = 0; s->pos = 0;
 }
 static inline double unext(usrc *s) {
     if (s->list) {
@@ -177,14 +179,14 @@ static inline double unext(usrc *s) {
         s->pos++;
         return v;
     }
-    if (s->has) { s->has = 0; s->pos++; return s->cache; }
-    uint32_t ctr[4] = { s->c0, s->j, s->c2, s->c3 }, w[4];
-    or_philox4x32_10(ctr, s->key, w);
-    s->j++;
-    s->cache = or_u01(w[2], w[3]);
-    s->has = 1;
+    if (s->nw == 0) {
+        uint32_t ctr[4] = { s->c0, s->j, s->c2, s->c3 };
+        or_philox4x32_10(ctr, s->key, s->w);
+        s->j++;
+        s->nw = 4;
+    }
     s->pos++;
-    return or_u01(w[0], w[1]);
+    return or_u01(s->w[4 - s->nw--]);
 }
 
 void or_stream_uniforms(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, double *out) {

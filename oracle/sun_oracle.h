@@ -29,8 +29,8 @@ enum { OR_STORE_18 = 18, OR_STORE_12 = 12, OR_STORE_8 = 8 };
 
 /* ---- counter-based RNG (Philox4x32-10, Salmon et al. SC'11) ---------------- */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
-/* uniform double in (0,1] from two 32-bit words: ((hi<<32|lo)>>11)+1) * 2^-53 */
-double or_u01(uint32_t lo, uint32_t hi);
+/* uniform double in (0,1] from one 32-bit Philox word: (w + 1) * 2^-32 */
+double or_u01(uint32_t w);
 /* n uniforms of the per-site stream (seed, c0, c2, c3), draw counter starting at 0 */
 void or_stream_uniforms(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, double *out);
 

@@ -38,35 +38,40 @@ struct Philox {
     }
 };
 
-// Uniform double in (0,1]: ((hi<<32|lo)>>11)+1) * 2^-53
-SUN_HD double u01(uint32_t lo, uint32_t hi) {
-    uint64_t m = ((((uint64_t)hi) << 32) | lo) >> 11;
-    return (double)(m + 1) * 0x1p-53;
-}
+// Uniform in (0,1] from one 32-bit Philox word: (w + 1) * 2^-32 -- 32 bits of
+// randomness per draw, as cuRAND's MRG32k3a uniforms (random.h:16-22).  The double
+// is exact; the float is the double rounded to 24 bits (at most one ulp apart).
+SUN_HD double u01(uint32_t w) { return fma((double)w, 0x1p-32, 0x1p-32); }
+SUN_HD float u01f(uint32_t w) { return fmaf((float)w, 0x1p-32f, 0x1p-32f); }
 
-// Stream of uniforms for one (key, c0, c2, c3); counter word 1 is the draw index.
-// Each Philox call yields two doubles.
+// Stream of uniforms for one (key, c0, c2, c3): draw k is word k%4 of
+// Philox4x32-10((c0, k/4, c2, c3), key), so one Philox call serves four draws.
 struct Stream {
     uint32_t k0, k1, c0, c2, c3, j;
-    double cache;
-    bool has;
+    uint32_t w[4];
+    int nw;
 
     SUN_HD Stream(uint64_t seed, uint32_t c0_, uint32_t c2_, uint32_t c3_)
-        : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), c0(c0_), c2(c2_), c3(c3_), j(0),
-          cache(0.0), has(false) {}
+        : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), c0(c0_), c2(c2_), c3(c3_), j(0), w{0, 0, 0, 0},
+          nw(0) {}
 
-    SUN_HD double next() {
-        if (has) {
-            has = false;
-            return cache;
+    SUN_HD uint32_t word() {
+        if (nw == 0) {
+            uint32_t a = c0, b = j, c = c2, d = c3;
+            Philox::round10(a, b, c, d, k0, k1);
+            w[0] = a;
+            w[1] = b;
+            w[2] = c;
+            w[3] = d;
+            j++;
+            nw = 4;
         }
-        uint32_t a = c0, b = j, c = c2, d = c3;
-        Philox::round10(a, b, c, d, k0, k1);
-        j++;
-        cache = u01(c, d);
-        has = true;
-        return u01(a, b);
+        const int i = 4 - nw;
+        nw--;
+        return i == 0 ? w[0] : i == 1 ? w[1] : i == 2 ? w[2] : w[3];
     }
+    SUN_HD double next() { return u01(word()); }
+    SUN_HD float next_f() { return u01f(word()); }
 };
 
 // SU(2) element a = (a0,a1,a2,a3) with a0 ~ sqrt(1-a0^2) exp(al*a0): the MILC
@@ -131,23 +136,23 @@ SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
     constexpr float kTwoPiF = (float)kTwoPi;
     float xr1, xr2, xr3, xr4, d, r;
     float a0 = 0.f;
-    xr1 = (float)st.next();
+    xr1 = st.next_f();
     xr1 = logf(xr1 + 1.e-10f);
-    xr2 = (float)st.next();
+    xr2 = st.next_f();
     xr2 = logf(xr2 + 1.e-10f);
-    xr3 = (float)st.next();
-    xr4 = (float)st.next();
+    xr3 = st.next_f();
+    xr4 = st.next_f();
     xr3 = cosf(kTwoPiF * xr3);
     d = -(xr2 + xr1 * xr3 * xr3) / al;
     bool accepted = (1.00f - 0.5f * d) > xr4 * xr4;
     if (!accepted && al > 2.f) {
         for (int k = 0; k < 20; k++) {
-            xr1 = (float)st.next();
+            xr1 = st.next_f();
             xr1 = logf(xr1 + 1.e-10f);
-            xr2 = (float)st.next();
+            xr2 = st.next_f();
             xr2 = logf(xr2 + 1.e-10f);
-            xr3 = (float)st.next();
-            xr4 = (float)st.next();
+            xr3 = st.next_f();
+            xr4 = st.next_f();
             xr3 = cosf(kTwoPiF * xr3);
             d = -(xr2 + xr1 * xr3 * xr3) / al;
             if ((1.00f - 0.5f * d) > xr4 * xr4) break;
@@ -157,8 +162,8 @@ SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
         xr3 = expf(-2.f * al);
         xr4 = 1.f - xr3;
         for (int k = 0; k < 20; k++) {
-            xr1 = (float)st.next();
-            xr2 = (float)st.next();
+            xr1 = st.next_f();
+            xr2 = st.next_f();
             r = xr3 + xr4 * xr1;
             a0 = 1.00f + logf(r) / al;
             if ((1.f - a0 * a0) > xr2 * xr2) break;
@@ -169,11 +174,11 @@ SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
     xr3 = 1.f - a[0] * a[0];
     xr3 = fabsf(xr3);
     r = sqrtf(xr3);
-    a[3] = (2.f * (float)st.next() - 1.f) * r;
+    a[3] = (2.f * st.next_f() - 1.f) * r;
     xr1 = xr3 - a[3] * a[3];
     xr1 = fabsf(xr1);
     xr1 = sqrtf(xr1);
-    xr2 = kTwoPiF * (float)st.next();
+    xr2 = kTwoPiF * st.next_f();
     a[1] = xr1 * cosf(xr2);
     a[2] = xr1 * sinf(xr2);
 }
