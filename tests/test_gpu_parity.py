@@ -41,6 +41,19 @@ def store_tol(storage, prec):
     return 2e-5 if storage == 8 else 2e-6
 
 
+def link_bound(lat, ref, storage, prec):
+    """Per-link bar for comparing a GPU sweep with the oracle's result ref.  FP64 and
+    12/18-real FP32: tol(prec).  8-real FP32: the decode divides by |e01|^2+|e02|^2,
+    so FP32 8-real storage cannot hold some links to 5e-5 even when exact; the bar is
+    5e-5 plus four times the FP32 storage round-trip error of ref at that link
+    (measured by loading ref into the lattice and reading it back)."""
+    base = tol(prec)
+    if not (storage == 8 and prec == 4):
+        return base
+    lat.from_host(ref)
+    return base + 4.0 * per_link_diff(lat.to_host(), ref)
+
+
 def maxdiff(A, B):
     return float(np.abs(A - B).max())
 
@@ -135,8 +148,8 @@ def test_overrelax_sweep_parity(ctx, oracle, nc, storage, prec, dims):
         U = load_config(lat, thermalised(oracle, nc, dims, storage))
         lat.overrelax()
         G = lat.to_host()
-    oracle.overrelax_sweep(U, dims, storage)
-    assert maxdiff(G, U) <= tol(prec)
+        oracle.overrelax_sweep(U, dims, storage)
+        assert np.all(per_link_diff(G, U) <= link_bound(lat, U, storage, prec))
 
 
 @pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
