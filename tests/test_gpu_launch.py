@@ -58,7 +58,7 @@ def test_every_shape_overrelax_parity(ctx, oracle, nc, storage, prec, dims):
         U = load_config(lat, U0)
         ref = U.copy()
         oracle.overrelax_sweep(ref, dims, storage)
-        bound = link_bound(lat, ref, storage, prec)
+        bound = link_bound(lat, U, ref, storage, prec, lambda A: oracle.overrelax_sweep(A, dims, storage))
         shapes = accepted_shapes(lat, 1)
         assert len(shapes) >= 3
         for block, brick, lds, piped in shapes:
