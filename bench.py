@@ -143,6 +143,37 @@ def plain_and_precision(sun_amd, ctx, stream, obs32, value, updates_per_step):
     return plain, prec
 
 
+def large_leg(sun_amd, ctx, stream, n_iter=6, warm=2):
+    """SU(3) 64^4 (16.7M sites, 2.15 GB of links), beta=6.2, FP32 8-real links from a
+    Haar start (8 reals cannot hold a cold start); iterations of 1 heatbath + 4
+    overrelaxation sweeps + plaquette + Polyakov loop (FP64 sums), graph replay.
+    Reports link updates/s and the overrelaxation launch's algorithmic GB/s
+    (9 x 32 B per update) from HIP events."""
+    dims = (64, 64, 64, 64)
+    V = int(np.prod(dims))
+    with sun_amd.Lattice(ctx, 3, dims, 6.2, 8, 4) as lat:
+        lat.hot_start(3, stream)
+        lat.generate(seed=3, n_iter=warm, n_or=4, reunit_every=0, stream=stream)
+        sun_amd.sync(stream)
+        t0 = time.perf_counter()
+        obs = lat.generate(seed=3, n_iter=n_iter, n_or=4, reunit_every=0, first_sweep=warm, stream=stream)
+        sun_amd.sync(stream)
+        dt = time.perf_counter() - t0
+        or_ms = time_launches(sun_amd, lat, stream, hb=False, reps=16)
+        hb_ms = time_launches(sun_amd, lat, stream, hb=True, reps=8)
+        per_launch = 9 * 32 * (V // 2)
+        return {"value": 5 * 4 * V * n_iter / dt, "unit": "link-updates/s", "iterations": n_iter,
+                "ms_per_iteration": dt / n_iter * 1e3,
+                "overrelax_launch_ms": or_ms, "overrelax_gbs": per_launch / (or_ms * 1e-3) / 1e9,
+                "overrelax_frac": per_launch / (or_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                "heatbath_launch_ms": hb_ms, "heatbath_gbs": per_launch / (hb_ms * 1e-3) / 1e9,
+                "launch_shape": lat.get_launch(1).as_dict(), "heatbath_launch_shape": lat.get_launch(0).as_dict(),
+                "plaquette_last": [float(obs[-1, 0]), float(obs[-1, 1])],
+                "polyakov_last": [float(obs[-1, 2]), float(obs[-1, 3])],
+                "config": "SU(3) 64^4 beta=6.2 Haar start, FP32 8-real links; iteration = HB + 4 OR + "
+                          "plaquette + spatially averaged Polyakov loop (FP64 sums), hipGraph replay"}
+
+
 BATCH_VOLUMES = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 16), (16, 16, 16, 32), (24, 24, 24, 24)]
 
 
@@ -236,22 +267,27 @@ def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
                       "12-real), FP64 reductions, hot/cold 50/50, 64 GiB budget"}
 
 
-def kernels_only(sun_amd):
-    """For rocprofv3 counter passes: on the 32^4 FP32 12-real field after 100
-    iterations, 2 copies of the field (calibration: a plain copy of known size),
+def kernels_only(sun_amd, large=False):
+    """For rocprofv3 counter passes.  Default: on the 32^4 FP32 12-real field after
+    100 iterations, 2 copies of the field (calibration: a plain copy of known size),
     then 8 overrelaxation launches, 8 heatbath launches, reunitarize, plaquette,
-    Polyakov loop.  Launch shapes come from the tune cache in $SUN_RESOURCE_PATH
-    (written by a preceding full bench.py run with the same variable), so the
-    profiled kernels are the ones the benchmark times; without a cache the
-    autotuner runs first and its trial launches appear in the profile too."""
+    Polyakov loop.  With large: the same on the 64^4 FP32 8-real field (Haar start,
+    beta=6.2) after 20 iterations.  Launch shapes come from the tune cache in
+    $SUN_RESOURCE_PATH (written by a preceding full bench.py run with the same
+    variable), so the profiled kernels are the ones the benchmark times; without a
+    cache the autotuner runs first and its trial launches appear in the profile too."""
     ctx = sun_amd.Context(0, 0, 1)
-    lat = sun_amd.Lattice(ctx, 3, DIMS, BETA, 12, 4)
-    tmp = sun_amd.Lattice(ctx, 3, DIMS, BETA, 12, 4)
+    dims, beta, storage = ((64, 64, 64, 64), 6.2, 8) if large else (DIMS, BETA, 12)
+    lat = sun_amd.Lattice(ctx, 3, dims, beta, storage, 4)
+    tmp = sun_amd.Lattice(ctx, 3, dims, beta, storage, 4)
     s = sun_amd.new_stream()
-    lat.cold_start(s)
-    # ~0.2 s of iterations first, so clocks are at their sustained level when the
-    # launches below run (the profile averages then match the benchmark's timings)
-    lat.generate(seed=2, n_iter=100, n_or=4, reunit_every=1, stream=s)
+    if large:
+        lat.hot_start(3, s)
+    else:
+        lat.cold_start(s)
+    # iterations first, so clocks are at their sustained level when the launches
+    # below run (the profile averages then match the benchmark's timings)
+    lat.generate(seed=2, n_iter=20 if large else 100, n_or=4, reunit_every=0 if large else 1, stream=s)
     for _ in range(2):
         tmp.copy_from(lat, s)
     for parity in (0, 1):
@@ -278,8 +314,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-lattice leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the 64^4 FP32 8-real leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson arrivals) leg")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served leg")
+    ap.add_argument("--large", action="store_true", help="with --kernels-only: the 64^4 FP32 8-real field")
     ap.add_argument("--kernels-only", action="store_true",
                     help="fixed short launch sequence for profiler counter passes (no JSON line)")
     args = ap.parse_args()
@@ -290,7 +328,7 @@ def main():
 
     import sun_amd
     if args.kernels_only:
-        kernels_only(sun_amd)
+        kernels_only(sun_amd, large=args.large)
         return
     ctx = sun_amd.Context(0, 0, 1)
     V = int(np.prod(DIMS))
@@ -357,6 +395,8 @@ def main():
     if not args.no_plain:
         out["plain_fp64"], out["precision"] = plain_and_precision(sun_amd, ctx, stream, obs_all, value,
                                                                   updates_per_step)
+    if not args.no_large:
+        out["large_64x4_f32_8"] = large_leg(sun_amd, ctx, stream)
     if not args.no_batch:
         out["batched"] = batched_leg(sun_amd, ctx, stream)
     if not args.no_serve:
