@@ -276,13 +276,49 @@ SUN_HD F3 staple_su3(const Field& f, const Geom& g, int id, int parity, const in
     return S;
 }
 
+// A link as stored (12, 18 or 8 reals), decoded only when it is multiplied, so a
+// register set of in-flight links costs its stored size: 8 reals for 8-real storage.
+template <int STORE>
+struct RawLink {
+    float r[Layout<float, 3, STORE>::NREAL];
+};
+template <int STORE>
+SUN_HD RawLink<STORE> load_raw(const Field& f, uint32_t l) {
+    RawLink<STORE> x;
+    load_reals<float, 3, STORE>(f, l, x.r);
+    return x;
+}
+// rows 0,1 valid; row 2 valid unless STORE == 12 (need_row2 builds it).  For 8-real
+// links the stored reals pass through an empty asm first: the decode (sqrt, sin,
+// cos, a division) then cannot be hoisted to the load, where the decoded 18 reals
+// of every in-flight link would have to live in registers at once.
+template <int STORE>
+SUN_HD F3 decode(RawLink<STORE> x) {
+    F3 m;
+    if constexpr (STORE == kStore8) {
+        asm volatile("" : "+v"(x.r[0]), "+v"(x.r[1]), "+v"(x.r[2]), "+v"(x.r[3]), "+v"(x.r[4]), "+v"(x.r[5]),
+                     "+v"(x.r[6]), "+v"(x.r[7]));
+    }
+    if constexpr (STORE == kStore12) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            m.e[0][j] = mk<float>(x.r[2 * j], x.r[2 * j + 1]);
+            m.e[1][j] = mk<float>(x.r[6 + 2 * j], x.r[6 + 2 * j + 1]);
+        }
+    } else {
+        m = unpack<float, 3, STORE>(x.r);
+    }
+    return m;
+}
+
 // staple sum (monte.cu:63-80) for a compile-time direction MU, software-pipelined:
 // the six links of direction nu_{k+1} are loaded into one register set while the
 // other set's direction nu_k is multiplied, so a wavefront keeps ~18 16-byte loads
-// in flight instead of ~4.  Needs ~220 VGPRs: for kernels run at 2 waves per SIMD,
-// the occupancy at which the L2 still holds the neighbour links between their
-// uses.  Same products in the same order as staple_su3 (identical results).
-// The link being updated, U_MU(x), is loaded with the last direction's set into Uown.
+// in flight instead of ~4.  ~190 VGPRs for 12-real links (less for 8-real, whose
+// sets hold the 8 stored reals and decode at use): for kernels run at 2 waves per
+// SIMD, the occupancy at which the L2 still holds the neighbour links between their
+// uses.  Same products in the same order as staple_su3.  The link being updated,
+// U_MU(x), is loaded with the last direction's set into Uown.
 template <int STORE, int MU>
 SUN_HD F3 staple_su3_piped(const Field& f, const Geom& g, int id, int parity, const int x[4], int lex, F3& Uown) {
     const uint32_t V = (uint32_t)g.V;
@@ -306,16 +342,16 @@ SUN_HD F3 staple_su3_piped(const Field& f, const Geom& g, int id, int parity, co
         eo_mnu_pmu[k] = (uint32_t)eo_of_lex(g, lex_shift(g, lex_mnu, xd, MU, +1), parity);
     }
     // register sets [slot]: U_mu(x+nu), U_mu(x-nu), U_nu(x), U_nu(x+mu), U_nu(x-nu), U_nu(x-nu+mu)
-    F3 Bu[2], Bd[2], Au[2], Cu[2], Ad[2], Cd[2];
+    RawLink<STORE> Bu[2], Bd[2], Au[2], Cu[2], Ad[2], Cd[2], Uraw;
 #define SUN_PIPE_LOAD(k, slot)                                                          \
     do {                                                                                \
         const uint32_t nuV = (uint32_t)NUS[k] * V;                                      \
-        Bu[slot] = load_rows<STORE>(f, (uint32_t)MU * V + eo_pnu[k]);                  \
-        Bd[slot] = load_rows<STORE>(f, (uint32_t)MU * V + eo_mnu[k]);                  \
-        Au[slot] = load_rows<STORE>(f, nuV + self);                                     \
-        Cu[slot] = load_rows<STORE>(f, nuV + eo_pmu);                                   \
-        Ad[slot] = load_rows<STORE>(f, nuV + eo_mnu[k]);                                \
-        Cd[slot] = load_rows<STORE>(f, nuV + eo_mnu_pmu[k]);                            \
+        Bu[slot] = load_raw<STORE>(f, (uint32_t)MU * V + eo_pnu[k]);                   \
+        Bd[slot] = load_raw<STORE>(f, (uint32_t)MU * V + eo_mnu[k]);                   \
+        Au[slot] = load_raw<STORE>(f, nuV + self);                                      \
+        Cu[slot] = load_raw<STORE>(f, nuV + eo_pmu);                                    \
+        Ad[slot] = load_raw<STORE>(f, nuV + eo_mnu[k]);                                 \
+        Cd[slot] = load_raw<STORE>(f, nuV + eo_mnu_pmu[k]);                             \
     } while (0)
     SUN_PIPE_LOAD(0, 0);
     F3 S;
@@ -323,23 +359,30 @@ SUN_HD F3 staple_su3_piped(const Field& f, const Geom& g, int id, int parity, co
     for (int k = 0; k < 3; k++) {
         const int cur = k & 1;
         if (k + 1 < 3) SUN_PIPE_LOAD(k + 1, cur ^ 1);
-        if (k == 1) Uown = load_rows<STORE>(f, (uint32_t)MU * V + self);
+        if (k == 1) Uraw = load_raw<STORE>(f, (uint32_t)MU * V + self);
         F3 W, T;
-        need_row2<STORE>(Cu[cur]);
-        mul_2r_bdag(Bu[cur], Cu[cur], W);
-        complete_row2(W);
-        mul_2r(Au[cur], W, T);
-        complete_row2(T);
+        {
+            F3 B = decode(Bu[cur]), Cc = decode(Cu[cur]), A = decode(Au[cur]);
+            need_row2<STORE>(Cc);
+            mul_2r_bdag(B, Cc, W);
+            complete_row2(W);
+            mul_2r(A, W, T);
+            complete_row2(T);
+        }
         if (k == 0) S = T; else add_full(S, T);
-        need_row2<STORE>(Cd[cur]);
-        need_row2<STORE>(Ad[cur]);
-        mul_2r(Bd[cur], Cd[cur], W);
-        complete_row2(W);
-        mul_2r_adag(Ad[cur], W, T);
-        complete_row2(T);
+        {
+            F3 B = decode(Bd[cur]), Cc = decode(Cd[cur]), A = decode(Ad[cur]);
+            need_row2<STORE>(Cc);
+            need_row2<STORE>(A);
+            mul_2r(B, Cc, W);
+            complete_row2(W);
+            mul_2r_adag(A, W, T);
+            complete_row2(T);
+        }
         add_full(S, T);
     }
 #undef SUN_PIPE_LOAD
+    Uown = decode(Uraw);
     return S;
 }
 

@@ -189,6 +189,27 @@ SUN_HD void store_reals(const Field& f, uint32_t l, const R* r) {
     }
 }
 
+// sin and cos of a stored phase th in (-pi, pi] (atan2 output).  On the device:
+// sincospi of th/pi, which needs no large-argument reduction (the general sin/cos
+// carry a Payne-Hanek path that dominates the registers of a kernel decoding 19
+// links).
+SUN_HD void phase(double th, double& s, double& c) {
+#ifdef __HIP_DEVICE_COMPILE__
+    sincospi(th * 0.318309886183790671538, &s, &c);
+#else
+    s = sin(th);
+    c = cos(th);
+#endif
+}
+SUN_HD void phase(float th, float& s, float& c) {
+#ifdef __HIP_DEVICE_COMPILE__
+    sincospif(th * 0.318309886183790671538f, &s, &c);
+#else
+    s = sinf(th);
+    c = cosf(th);
+#endif
+}
+
 // reconstruct_12p_8p.h:38-82 (8-real SU(3) decode)
 template <typename R>
 SUN_HD void reconstruct8(Mat<R, 3>& m, R th0, R th1) {
@@ -198,12 +219,15 @@ SUN_HD void reconstruct8(Mat<R, 3>& m, R th0, R th1) {
     p2 = R(1) / p1;
     diff = R(1) - p1;
     p1 = sqrt(diff >= R(0) ? diff : R(0));
-    m.e[0][0].re = p1 * cos(th0);
-    m.e[0][0].im = p1 * sin(th0);
+    R c0, s0, c1, s1;
+    phase(th0, s0, c0);
+    phase(th1, s1, c1);
+    m.e[0][0].re = p1 * c0;
+    m.e[0][0].im = p1 * s0;
     diff = R(1) - abs2(m.e[0][0]) - abs2(m.e[1][0]);
     p1 = sqrt(diff >= R(0) ? diff : R(0));
-    m.e[2][0].re = p1 * cos(th1);
-    m.e[2][0].im = p1 * sin(th1);
+    m.e[2][0].re = p1 * c1;
+    m.e[2][0].im = p1 * s1;
     m.e[1][1] = conj(m.e[2][0] * m.e[0][2]);
     m.e[2][2] = conj(m.e[0][0]) * m.e[1][0];
     m.e[1][1] = m.e[1][1] + m.e[2][2] * m.e[0][1];

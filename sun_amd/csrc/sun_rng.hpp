@@ -74,6 +74,40 @@ struct Stream {
     SUN_HD float next_f() { return u01f(word()); }
 };
 
+// cos(2 pi u) and (sin, cos)(2 pi u) for u in (0, 1].  On the device they are
+// cospi/sincospi of the exact argument 2u: no large-argument reduction (the general
+// cos/sin inline a Payne-Hanek path that costs far more registers than the sampler).
+SUN_HD double cos2pi(double u) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return cospi(2.0 * u);
+#else
+    return cos(kTwoPi * u);
+#endif
+}
+SUN_HD void sincos2pi(double u, double& s, double& c) {
+#ifdef __HIP_DEVICE_COMPILE__
+    sincospi(2.0 * u, &s, &c);
+#else
+    s = sin(kTwoPi * u);
+    c = cos(kTwoPi * u);
+#endif
+}
+SUN_HD float cos2pi(float u) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return cospif(2.f * u);
+#else
+    return cosf((float)kTwoPi * u);
+#endif
+}
+SUN_HD void sincos2pi(float u, float& s, float& c) {
+#ifdef __HIP_DEVICE_COMPILE__
+    sincospif(2.f * u, &s, &c);
+#else
+    s = sinf((float)kTwoPi * u);
+    c = cosf((float)kTwoPi * u);
+#endif
+}
+
 // SU(2) element a = (a0,a1,a2,a3) with a0 ~ sqrt(1-a0^2) exp(al*a0): the MILC
 // Kennedy-Pendleton generator with the Creutz fallback for al <= 2, restating
 // device_PHB_OVR.h:162-223 (including the 1e-10 guard and the 20-try caps).  The
@@ -88,7 +122,7 @@ SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
     xr2 = log(xr2 + 1.e-10);
     xr3 = st.next();
     xr4 = st.next();
-    xr3 = cos(kTwoPi * xr3);
+    xr3 = cos2pi(xr3);
     d = -(xr2 + xr1 * xr3 * xr3) / al;
     bool accepted = (1.00 - 0.5 * d) > xr4 * xr4;
     if (!accepted && al > 2.0) {
@@ -99,7 +133,7 @@ SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
             xr2 = log(xr2 + 1.e-10);
             xr3 = st.next();
             xr4 = st.next();
-            xr3 = cos(kTwoPi * xr3);
+            xr3 = cos2pi(xr3);
             d = -(xr2 + xr1 * xr3 * xr3) / al;
             if ((1.00 - 0.5 * d) > xr4 * xr4) break;
         }
@@ -124,16 +158,16 @@ SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
     xr1 = xr3 - a[3] * a[3];
     xr1 = fabs(xr1);
     xr1 = sqrt(xr1);
-    xr2 = kTwoPi * st.next();
-    a[1] = xr1 * cos(xr2);
-    a[2] = xr1 * sin(xr2);
+    double sn, cs;
+    sincos2pi(st.next(), sn, cs);
+    a[1] = xr1 * cs;
+    a[2] = xr1 * sn;
 }
 
 // The same generator evaluated in FP32 (uniforms rounded to float), used by the
 // FP32-storage engine: the link it updates is FP32, so the sampler's precision
 // matches it, and FP32 log/cos/sin/exp/sqrt cost a fraction of the FP64 ones.
 SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
-    constexpr float kTwoPiF = (float)kTwoPi;
     float xr1, xr2, xr3, xr4, d, r;
     float a0 = 0.f;
     xr1 = st.next_f();
@@ -142,7 +176,7 @@ SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
     xr2 = logf(xr2 + 1.e-10f);
     xr3 = st.next_f();
     xr4 = st.next_f();
-    xr3 = cosf(kTwoPiF * xr3);
+    xr3 = cos2pi(xr3);
     d = -(xr2 + xr1 * xr3 * xr3) / al;
     bool accepted = (1.00f - 0.5f * d) > xr4 * xr4;
     if (!accepted && al > 2.f) {
@@ -153,7 +187,7 @@ SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
             xr2 = logf(xr2 + 1.e-10f);
             xr3 = st.next_f();
             xr4 = st.next_f();
-            xr3 = cosf(kTwoPiF * xr3);
+            xr3 = cos2pi(xr3);
             d = -(xr2 + xr1 * xr3 * xr3) / al;
             if ((1.00f - 0.5f * d) > xr4 * xr4) break;
         }
@@ -178,9 +212,10 @@ SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
     xr1 = xr3 - a[3] * a[3];
     xr1 = fabsf(xr1);
     xr1 = sqrtf(xr1);
-    xr2 = kTwoPiF * st.next_f();
-    a[1] = xr1 * cosf(xr2);
-    a[2] = xr1 * sinf(xr2);
+    float sn, cs;
+    sincos2pi(st.next_f(), sn, cs);
+    a[1] = xr1 * cs;
+    a[2] = xr1 * sn;
 }
 
 }  // namespace sun
