@@ -189,23 +189,6 @@ static inline double unext(usrc *s) {
     return or_u01(s->w[4 - s->nw--]);
 }
 
-/* One "packet": the four uniforms of one Philox call, (w_i + 1) * 2^-32.  The SU(2)
- * sampler draws one packet per Kennedy-Pendleton attempt, one per Creutz attempt
- * (first two used) and one for the final direction (first two used), so every
- * attempt costs exactly one Philox call.  List mode: the next four list entries. */
-static inline void upacket(usrc *s, double u[4]) {
-    if (s->list) {
-        for (int i = 0; i < 4; i++) u[i] = (s->pos + i < s->nlist) ? s->list[s->pos + i] : 1.0;
-        s->pos += 4;
-        return;
-    }
-    uint32_t ctr[4] = { s->c0, s->j, s->c2, s->c3 }, w[4];
-    or_philox4x32_10(ctr, s->key, w);
-    s->j++;
-    s->pos += 4;
-    for (int i = 0; i < 4; i++) u[i] = or_u01(w[i]);
-}
-
 void or_stream_uniforms(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, double *out) {
     usrc s;
     usrc_stream(&s, seed, c0, c2, c3);
@@ -230,28 +213,30 @@ static inline void q_mul_uv_dagger(const double v[4], const double u[4], double 
     b[3] = v[3] * u[0] - v[0] * u[3] + v[1] * u[2] - v[2] * u[1];
 }
 
-/* device_PHB_OVR.h:162-223 (MILC Kennedy-Pendleton / Creutz generator); uniforms
- * drawn by packets (upacket): attempt k of either branch uses packet k, the
- * direction uses the next packet. */
+/* device_PHB_OVR.h:162-223 (MILC Kennedy-Pendleton / Creutz generator) */
 static void gen_su2_milc(double al, usrc *st, double a[4]) {
-    double xr1, xr2, xr3, xr4, d, r, p[4];
+    double xr1, xr2, xr3, xr4, d, r;
     int k;
     double a0 = 0.0;
-    upacket(st, p);
-    xr1 = log(p[0] + 1.e-10);
-    xr2 = log(p[1] + 1.e-10);
-    xr3 = cos(PII * p[2]);
-    xr4 = p[3];
+    xr1 = unext(st);
+    xr1 = log(xr1 + 1.e-10);
+    xr2 = unext(st);
+    xr2 = log(xr2 + 1.e-10);
+    xr3 = unext(st);
+    xr4 = unext(st);
+    xr3 = cos(PII * xr3);
     d = -(xr2 + xr1 * xr3 * xr3) / al;
     int nacd = 0;
     if ((1.00 - 0.5 * d) > xr4 * xr4) nacd = 1;
     if (nacd == 0 && al > 2.0) {
         for (k = 0; k < 20; k++) {
-            upacket(st, p);
-            xr1 = log(p[0] + 1.e-10);
-            xr2 = log(p[1] + 1.e-10);
-            xr3 = cos(PII * p[2]);
-            xr4 = p[3];
+            xr1 = unext(st);
+            xr1 = log(xr1 + 1.e-10);
+            xr2 = unext(st);
+            xr2 = log(xr2 + 1.e-10);
+            xr3 = unext(st);
+            xr4 = unext(st);
+            xr3 = cos(PII * xr3);
             d = -(xr2 + xr1 * xr3 * xr3) / al;
             if ((1.00 - 0.5 * d) > xr4 * xr4) break;
         }
@@ -260,9 +245,8 @@ static void gen_su2_milc(double al, usrc *st, double a[4]) {
         xr3 = exp(-2.0 * al);
         xr4 = 1.0 - xr3;
         for (k = 0; k < 20; k++) {
-            upacket(st, p);
-            xr1 = p[0];
-            xr2 = p[1];
+            xr1 = unext(st);
+            xr2 = unext(st);
             r = xr3 + xr4 * xr1;
             a0 = 1.00 + log(r) / al;
             if ((1.0 - a0 * a0) > xr2 * xr2) break;
@@ -273,12 +257,11 @@ static void gen_su2_milc(double al, usrc *st, double a[4]) {
     xr3 = 1.0 - a[0] * a[0];
     xr3 = fabs(xr3);
     r = sqrt(xr3);
-    upacket(st, p);
-    a[3] = (2.0 * p[0] - 1.0) * r;
+    a[3] = (2.0 * unext(st) - 1.0) * r;
     xr1 = xr3 - a[3] * a[3];
     xr1 = fabs(xr1);
     xr1 = sqrt(xr1);
-    xr2 = PII * p[1];
+    xr2 = PII * unext(st);
     a[1] = xr1 * cos(xr2);
     a[2] = xr1 * sin(xr2);
 }

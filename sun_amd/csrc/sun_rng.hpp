@@ -72,28 +72,6 @@ struct Stream {
     }
     SUN_HD double next() { return u01(word()); }
     SUN_HD float next_f() { return u01f(word()); }
-
-    // one packet: the four uniforms of the next Philox call (the SU(2) sampler
-    // draws one per attempt, so the draw pattern is the same on every lane)
-    SUN_HD void packet_words(uint32_t v[4]) {
-        uint32_t a = c0, b = j, c = c2, d = c3;
-        Philox::round10(a, b, c, d, k0, k1);
-        j++;
-        v[0] = a;
-        v[1] = b;
-        v[2] = c;
-        v[3] = d;
-    }
-    SUN_HD void packet(double u[4]) {
-        uint32_t v[4];
-        packet_words(v);
-        for (int i = 0; i < 4; i++) u[i] = u01(v[i]);
-    }
-    SUN_HD void packet_f(float u[4]) {
-        uint32_t v[4];
-        packet_words(v);
-        for (int i = 0; i < 4; i++) u[i] = u01f(v[i]);
-    }
 };
 
 // cos(2 pi u) and (sin, cos)(2 pi u) for u in (0, 1].  On the device they are
@@ -133,25 +111,29 @@ SUN_HD void sincos2pi(float u, float& s, float& c) {
 // SU(2) element a = (a0,a1,a2,a3) with a0 ~ sqrt(1-a0^2) exp(al*a0): the MILC
 // Kennedy-Pendleton generator with the Creutz fallback for al <= 2, restating
 // device_PHB_OVR.h:162-223 (including the 1e-10 guard and the 20-try caps).  The
-// uniforms come by packets (Stream::packet): attempt k of either branch uses packet
-// k, the direction the next one.
+// reference's double literals make this arithmetic double even for Real=float;
+// here it is double for every storage precision.
 SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
-    double xr1, xr2, xr3, xr4, d, r, p[4];
+    double xr1, xr2, xr3, xr4, d, r;
     double a0 = 0.0;
-    st.packet(p);
-    xr1 = log(p[0] + 1.e-10);
-    xr2 = log(p[1] + 1.e-10);
-    xr3 = cos2pi(p[2]);
-    xr4 = p[3];
+    xr1 = st.next();
+    xr1 = log(xr1 + 1.e-10);
+    xr2 = st.next();
+    xr2 = log(xr2 + 1.e-10);
+    xr3 = st.next();
+    xr4 = st.next();
+    xr3 = cos2pi(xr3);
     d = -(xr2 + xr1 * xr3 * xr3) / al;
     bool accepted = (1.00 - 0.5 * d) > xr4 * xr4;
     if (!accepted && al > 2.0) {
         for (int k = 0; k < 20; k++) {
-            st.packet(p);
-            xr1 = log(p[0] + 1.e-10);
-            xr2 = log(p[1] + 1.e-10);
-            xr3 = cos2pi(p[2]);
-            xr4 = p[3];
+            xr1 = st.next();
+            xr1 = log(xr1 + 1.e-10);
+            xr2 = st.next();
+            xr2 = log(xr2 + 1.e-10);
+            xr3 = st.next();
+            xr4 = st.next();
+            xr3 = cos2pi(xr3);
             d = -(xr2 + xr1 * xr3 * xr3) / al;
             if ((1.00 - 0.5 * d) > xr4 * xr4) break;
         }
@@ -160,10 +142,11 @@ SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
         xr3 = exp(-2.0 * al);
         xr4 = 1.0 - xr3;
         for (int k = 0; k < 20; k++) {
-            st.packet(p);
-            r = xr3 + xr4 * p[0];
+            xr1 = st.next();
+            xr2 = st.next();
+            r = xr3 + xr4 * xr1;
             a0 = 1.00 + log(r) / al;
-            if ((1.0 - a0 * a0) > p[1] * p[1]) break;
+            if ((1.0 - a0 * a0) > xr2 * xr2) break;
         }
         d = 1.0 - a0;
     }
@@ -171,13 +154,12 @@ SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
     xr3 = 1.0 - a[0] * a[0];
     xr3 = fabs(xr3);
     r = sqrt(xr3);
-    st.packet(p);
-    a[3] = (2.0 * p[0] - 1.0) * r;
+    a[3] = (2.0 * st.next() - 1.0) * r;
     xr1 = xr3 - a[3] * a[3];
     xr1 = fabs(xr1);
     xr1 = sqrt(xr1);
     double sn, cs;
-    sincos2pi(p[1], sn, cs);
+    sincos2pi(st.next(), sn, cs);
     a[1] = xr1 * cs;
     a[2] = xr1 * sn;
 }
@@ -186,22 +168,26 @@ SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
 // FP32-storage engine: the link it updates is FP32, so the sampler's precision
 // matches it, and FP32 log/cos/sin/exp/sqrt cost a fraction of the FP64 ones.
 SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
-    float xr1, xr2, xr3, xr4, d, r, p[4];
+    float xr1, xr2, xr3, xr4, d, r;
     float a0 = 0.f;
-    st.packet_f(p);
-    xr1 = logf(p[0] + 1.e-10f);
-    xr2 = logf(p[1] + 1.e-10f);
-    xr3 = cos2pi(p[2]);
-    xr4 = p[3];
+    xr1 = st.next_f();
+    xr1 = logf(xr1 + 1.e-10f);
+    xr2 = st.next_f();
+    xr2 = logf(xr2 + 1.e-10f);
+    xr3 = st.next_f();
+    xr4 = st.next_f();
+    xr3 = cos2pi(xr3);
     d = -(xr2 + xr1 * xr3 * xr3) / al;
     bool accepted = (1.00f - 0.5f * d) > xr4 * xr4;
     if (!accepted && al > 2.f) {
         for (int k = 0; k < 20; k++) {
-            st.packet_f(p);
-            xr1 = logf(p[0] + 1.e-10f);
-            xr2 = logf(p[1] + 1.e-10f);
-            xr3 = cos2pi(p[2]);
-            xr4 = p[3];
+            xr1 = st.next_f();
+            xr1 = logf(xr1 + 1.e-10f);
+            xr2 = st.next_f();
+            xr2 = logf(xr2 + 1.e-10f);
+            xr3 = st.next_f();
+            xr4 = st.next_f();
+            xr3 = cos2pi(xr3);
             d = -(xr2 + xr1 * xr3 * xr3) / al;
             if ((1.00f - 0.5f * d) > xr4 * xr4) break;
         }
@@ -210,10 +196,11 @@ SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
         xr3 = expf(-2.f * al);
         xr4 = 1.f - xr3;
         for (int k = 0; k < 20; k++) {
-            st.packet_f(p);
-            r = xr3 + xr4 * p[0];
+            xr1 = st.next_f();
+            xr2 = st.next_f();
+            r = xr3 + xr4 * xr1;
             a0 = 1.00f + logf(r) / al;
-            if ((1.f - a0 * a0) > p[1] * p[1]) break;
+            if ((1.f - a0 * a0) > xr2 * xr2) break;
         }
         d = 1.f - a0;
     }
@@ -221,13 +208,12 @@ SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
     xr3 = 1.f - a[0] * a[0];
     xr3 = fabsf(xr3);
     r = sqrtf(xr3);
-    st.packet_f(p);
-    a[3] = (2.f * p[0] - 1.f) * r;
+    a[3] = (2.f * st.next_f() - 1.f) * r;
     xr1 = xr3 - a[3] * a[3];
     xr1 = fabsf(xr1);
     xr1 = sqrtf(xr1);
     float sn, cs;
-    sincos2pi(p[1], sn, cs);
+    sincos2pi(st.next_f(), sn, cs);
     a[1] = xr1 * cs;
     a[2] = xr1 * sn;
 }

