@@ -93,7 +93,8 @@ size_t sun_lattice_bytes(const sun_lattice_desc *desc);
  * the linear site order with `block` threads; otherwise a workgroup covers
  * 2^brick[0] x 2^brick[1] x 2^brick[2] rows of X0/2 sites along (x1, x2, x3).
  * lds: unused dynamic LDS per workgroup (caps workgroups per CU).  piped: the
- * software-pipelined kernel (FP32 12-real SU(3) only).  Replaces the reference's
+ * software-pipelined kernels (FP32 12- and 8-real SU(3)): 1 pipelined by
+ * directions (2 waves per SIMD), 2 by half-directions (3 waves).  Replaces the reference's
  * TuneParam override (include/tune.h:36-118); results never depend on it. */
 typedef struct {
     int block, brick[3], lds, piped;

@@ -79,7 +79,8 @@ class GenerateDesc(ctypes.Structure):
 
 class Launch(ctypes.Structure):
     """sun_launch: block, brick (log2 rows along x1, x2, x3; brick[0] < 0 = linear),
-    lds (unused dynamic LDS per workgroup), piped (software-pipelined kernel)"""
+    lds (unused dynamic LDS per workgroup), piped (0 plain, 1/2 software-pipelined by
+    directions / half-directions)"""
     _fields_ = [("block", ctypes.c_int), ("brick", ctypes.c_int * 3), ("lds", ctypes.c_int),
                 ("piped", ctypes.c_int)]
 
@@ -261,7 +262,8 @@ class Lattice:
         _check(lib().sun_lattice_get_launch(self.h, kernel, ctypes.byref(out)), "sun_lattice_get_launch")
         return out
 
-    def set_launch(self, kernel: int, block: int = 256, brick=(-1, 0, 0), lds: int = 0, piped: bool = False):
+    def set_launch(self, kernel: int, block: int = 256, brick=(-1, 0, 0), lds: int = 0, piped: int = 0):
+        """piped: 0 plain kernel, 1 pipelined by directions, 2 by half-directions"""
         c = Launch(block, (ctypes.c_int * 3)(*brick), lds, int(piped))
         _check(lib().sun_lattice_set_launch(self.h, kernel, ctypes.byref(c)), "sun_lattice_set_launch")
 

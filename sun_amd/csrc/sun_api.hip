@@ -285,8 +285,10 @@ extern "C" int sun_lattice_set_launch(sun_lattice* lat, int kernel, const sun_la
     if (cfg->piped && !lat->ops->has_piped)
         return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: no pipelined kernel for this N/storage/precision");
     const int max_block = cfg->piped ? lat->ops->piped_max_block : lat->ops->max_block;
+    if (cfg->piped < 0 || cfg->piped > 2)
+        return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: piped %d (0, 1 or 2)", cfg->piped);
     LaunchCfg c;
-    c.piped = cfg->piped ? 1 : 0;
+    c.piped = cfg->piped;
     c.lds = cfg->lds;
     if (cfg->lds < 0 || cfg->lds > 65536 * 2) return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: lds %d", cfg->lds);
     if (cfg->brick[0] >= 0) {

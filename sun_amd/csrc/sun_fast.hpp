@@ -386,6 +386,80 @@ SUN_HD F3 staple_su3_piped(const Field& f, const Geom& g, int id, int parity, co
     return S;
 }
 
+// The same staple pipelined by half-directions: stage h = 2k (up term of nu_k) or
+// 2k+1 (down term) needs three links; stage h+1's three are in flight while stage h
+// is multiplied.  ~150 VGPRs for 12-real links, so 3 waves per SIMD fit (more
+// waves, each with half the loads in flight of staple_su3_piped).  Same products in
+// the same order as staple_su3.
+template <int STORE, int MU>
+SUN_HD F3 staple_su3_piped_half(const Field& f, const Geom& g, int id, int parity, const int x[4], int lex,
+                                F3& Uown) {
+    const uint32_t V = (uint32_t)g.V;
+    const int opar = parity ^ 1;
+    const uint32_t self = (uint32_t)(id + parity * g.half);
+    int xm[4] = {x[0], x[1], x[2], x[3]};
+    const uint32_t eo_pmu = (uint32_t)eo_of_lex(g, lex_shift(g, lex, xm, MU, +1), opar);
+    constexpr int N0 = MU == 0 ? 1 : 0;
+    constexpr int N1 = MU <= 1 ? 2 : 1;
+    constexpr int N2 = MU <= 2 ? 3 : 2;
+    constexpr int NUS[3] = {N0, N1, N2};
+    uint32_t eo_pnu[3], eo_mnu[3], eo_mnu_pmu[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int nu = NUS[k];
+        int xn[4] = {x[0], x[1], x[2], x[3]};
+        eo_pnu[k] = (uint32_t)eo_of_lex(g, lex_shift(g, lex, xn, nu, +1), opar);
+        int xd[4] = {x[0], x[1], x[2], x[3]};
+        const int lex_mnu = lex_shift(g, lex, xd, nu, -1);
+        eo_mnu[k] = (uint32_t)eo_of_lex(g, lex_mnu, opar);
+        eo_mnu_pmu[k] = (uint32_t)eo_of_lex(g, lex_shift(g, lex_mnu, xd, MU, +1), parity);
+    }
+    // slot: B = U_mu(x +- nu), A = U_nu(x) | U_nu(x-nu), C = U_nu(x+mu) | U_nu(x-nu+mu)
+    RawLink<STORE> B[2], A[2], C[2], Uraw;
+#define SUN_HALF_LOAD(h, slot)                                                             \
+    do {                                                                                   \
+        const int k_ = (h) >> 1;                                                           \
+        const uint32_t nuV = (uint32_t)NUS[k_] * V;                                        \
+        if (((h) & 1) == 0) {                                                              \
+            B[slot] = load_raw<STORE>(f, (uint32_t)MU * V + eo_pnu[k_]);                  \
+            A[slot] = load_raw<STORE>(f, nuV + self);                                      \
+            C[slot] = load_raw<STORE>(f, nuV + eo_pmu);                                    \
+        } else {                                                                           \
+            B[slot] = load_raw<STORE>(f, (uint32_t)MU * V + eo_mnu[k_]);                  \
+            A[slot] = load_raw<STORE>(f, nuV + eo_mnu[k_]);                                \
+            C[slot] = load_raw<STORE>(f, nuV + eo_mnu_pmu[k_]);                            \
+        }                                                                                  \
+    } while (0)
+    SUN_HALF_LOAD(0, 0);
+    F3 S;
+#pragma unroll
+    for (int h = 0; h < 6; h++) {
+        const int cur = h & 1;
+        if (h + 1 < 6) SUN_HALF_LOAD(h + 1, cur ^ 1);
+        if (h == 4) Uraw = load_raw<STORE>(f, (uint32_t)MU * V + self);
+        F3 W, T;
+        F3 Bm = decode(B[cur]), Cm = decode(C[cur]), Am = decode(A[cur]);
+        need_row2<STORE>(Cm);
+        if ((h & 1) == 0) {
+            mul_2r_bdag(Bm, Cm, W);
+            complete_row2(W);
+            mul_2r(Am, W, T);
+            complete_row2(T);
+            if (h == 0) S = T; else add_full(S, T);
+        } else {
+            need_row2<STORE>(Am);
+            mul_2r(Bm, Cm, W);
+            complete_row2(W);
+            mul_2r_adag(Am, W, T);
+            complete_row2(T);
+            add_full(S, T);
+        }
+    }
+#undef SUN_HALF_LOAD
+    Uown = decode(Uraw);
+    return S;
+}
+
 // rows p,q of U times columns p,q of F = S^+, projected on SU(2):
 //   a0=(UF)_pp a1=(UF)_pq a2=(UF)_qp a3=(UF)_qq, (UF)_ab = sum_j U_aj conj(S_bj)
 SUN_HD Quat<float> block_proj(const F3& U, const F3& S, int p, int q) {

@@ -125,7 +125,8 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
     // (heatbath, overrelaxation) x (plain, software-pipelined where the variant has it)
     LaunchCfg best_k[2];
     float best_k_ms[2] = {1e30f, 1e30f};
-    for (int which = 0; which < 4 && e == hipSuccess; which++) {
+    // piped: 0 plain kernel, 1 pipelined by directions, 2 pipelined by half-directions
+    for (int which = 0; which < 6 && e == hipSuccess; which++) {
         const bool hb = (which & 1) == 0;
         const int piped = which >> 1;
         if (piped && !lat->ops->has_piped) continue;
@@ -156,7 +157,8 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
             ms /= 8.f;
             if (ctx->verbosity >= SUN_VERBOSE)
                 fprintf(stderr, "[sun tune] %s block=%d brick=%d,%d,%d lds=%d %.4f ms\n",
-                        hb ? (piped ? "heatbath-piped" : "heatbath") : (piped ? "overrelax-piped" : "overrelax"),
+                        hb ? (piped == 2 ? "heatbath-piped2" : piped ? "heatbath-piped" : "heatbath")
+                           : (piped == 2 ? "overrelax-piped2" : piped ? "overrelax-piped" : "overrelax"),
                         c.block, c.b1, c.b2, c.b3, c.lds, ms);
             // candidates come in increasing LDS cap and shape order; a later one must
             // win by more than 1% (timing noise), so the choice is stable run to run

@@ -30,6 +30,9 @@ SHAPES = [  # (block, brick, lds, piped)
     (0, (2, 0, 2), 0, 1),
     (0, (1, 1, 2), 32768, 1),
     (0, (2, 1, 2), 0, 1),
+    (256, (-1, 0, 0), 0, 2),
+    (0, (2, 0, 2), 16384, 2),
+    (0, (1, 1, 2), 0, 2),
 ]
 DIMS = [(8, 8, 8, 16), (32, 4, 4, 8)]
 
@@ -38,7 +41,7 @@ def accepted_shapes(lat, kernel):
     out = []
     for block, brick, lds, piped in SHAPES:
         try:
-            lat.set_launch(kernel, block, brick, lds, bool(piped))
+            lat.set_launch(kernel, block, brick, lds, piped)
         except sun_amd.SunError as e:
             assert e.code == sun_amd.ERR_INVALID
             continue
@@ -62,7 +65,7 @@ def test_every_shape_overrelax_parity(ctx, oracle, nc, storage, prec, dims):
         shapes = accepted_shapes(lat, 1)
         assert len(shapes) >= 3
         for block, brick, lds, piped in shapes:
-            lat.set_launch(1, block, brick, lds, bool(piped))
+            lat.set_launch(1, block, brick, lds, piped)
             lat.from_host(U0)
             lat.overrelax()
             d = per_link_diff(lat.to_host(), ref)
@@ -81,7 +84,7 @@ def test_every_shape_heatbath_parity(ctx, oracle, nc, storage, prec, dims):
         shapes = accepted_shapes(lat, 0)
         assert len(shapes) >= 3
         for block, brick, lds, piped in shapes:
-            lat.set_launch(0, block, brick, lds, bool(piped))
+            lat.set_launch(0, block, brick, lds, piped)
             lat.from_host(U0)
             lat.heatbath(seed, sweep)
             d = per_link_diff(lat.to_host(), ref)
@@ -95,7 +98,7 @@ def test_every_shape_heatbath_parity(ctx, oracle, nc, storage, prec, dims):
 def test_piped_refused_without_kernel(ctx):
     with sun_amd.Lattice(ctx, 3, (8, 8, 8, 8), 6.0, 18, 8) as lat:
         with pytest.raises(sun_amd.SunError) as e:
-            lat.set_launch(1, 256, (-1, 0, 0), 0, True)
+            lat.set_launch(1, 256, (-1, 0, 0), 0, 1)
         assert e.value.code == sun_amd.ERR_INVALID
         with pytest.raises(sun_amd.SunError):
             lat.set_launch(1, 96)  # not whole wavefronts
