@@ -139,6 +139,26 @@ int sun_polyakov_async(sun_lattice *lat, void *stream, double *dev_out);
 int sun_links_to_host(sun_lattice *lat, double *host, int layout);
 int sun_links_from_host(sun_lattice *lat, const double *host, int layout);
 
+/* ---- checkpoints ------------------------------------------------------------------------
+ * SUN_FILE_NATIVE: the lattice's device bytes exactly, an 80-byte header (N, extents,
+ * storage, precision, beta, seed, next sweep number, CRC32 of the payload); loading
+ * restores the field bit for bit and returns (seed, next sweep), which is the whole RNG
+ * state, so a chain resumes exactly.  SUN_FILE_REFERENCE[_NOHEADER]: the layout of the
+ * reference's SaveBin_Gauge / ReadBin_Gauge (io_gauge.cpp:429-583, :605-700): [header
+ * grid[4], beta, bytes per real] then every site in lexicographic order (x fastest) with
+ * its four full N x N complex matrices as float (save_prec 4) or double (8); reading
+ * checks grid and beta (io_gauge.cpp:585-593) and projects onto the lattice's storage.
+ * Both synchronise the device first.  seed / next_sweep are ignored (save) or left
+ * unchanged (load) for the reference formats; save_prec is read from the header when
+ * there is one. */
+#define SUN_FILE_NATIVE 0
+#define SUN_FILE_REFERENCE 1
+#define SUN_FILE_REFERENCE_NOHEADER 2
+int sun_lattice_save(sun_lattice *lat, const char *path, int format, int save_prec, uint64_t seed,
+                     uint32_t next_sweep);
+int sun_lattice_load(sun_lattice *lat, const char *path, int format, int save_prec, uint64_t *seed,
+                     uint32_t *next_sweep);
+
 /* ---- generation driver ------------------------------------------------------------------
  * The loop the reference's API implies but does not ship (no HB/OR driver exists in
  * test.cpp): for it in [0, n_iter): heatbath (sweep = first_sweep + it), n_or

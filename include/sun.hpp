@@ -23,6 +23,7 @@
  *   Reunitarize<Real>              include/reunitarize.h:22-60
  *   PlaquetteCUB<Real>             include/meas/plaquette.h:120-176
  *   OnePolyakovLoop<Real>          include/meas/polyakovloop.h:15-73
+ *   SaveBin_Gauge / ReadBin_Gauge  include/io_gauge.h, src/io_gauge.cpp:429-700
  *
  * Differences, on purpose: errors throw sun::culqcd::Error (a std::runtime_error
  * carrying sun_last_error()) instead of exiting; SETPARAMS only records the
@@ -231,6 +232,22 @@ class OnePolyakovLoop {
     _gauge<Real>& U_;
     std::complex<double> value_;
 };
+
+/* SaveBin_Gauge<Real, RealSaveConf>(gauge, filename, withheader) / ReadBin_Gauge:
+ * the reference's binary configuration layout (io_gauge.cpp:429-583, :605-700);
+ * RealSaveConf is the precision of the reals in the file. */
+template <class RealSaveConf, class Real>
+void SaveBin_Gauge(_gauge<Real>& U, const std::string& filename, bool withheader = true) {
+    check(sun_lattice_save(U.handle(), filename.c_str(), withheader ? SUN_FILE_REFERENCE : SUN_FILE_REFERENCE_NOHEADER,
+                           (int)sizeof(RealSaveConf), 0, 0),
+          "SaveBin_Gauge");
+}
+template <class RealSaveConf, class Real>
+void ReadBin_Gauge(_gauge<Real>& U, const std::string& filename, bool withheader = true) {
+    check(sun_lattice_load(U.handle(), filename.c_str(), withheader ? SUN_FILE_REFERENCE : SUN_FILE_REFERENCE_NOHEADER,
+                           (int)sizeof(RealSaveConf), nullptr, nullptr),
+          "ReadBin_Gauge");
+}
 
 }  // namespace culqcd
 }  // namespace sun

@@ -32,13 +32,14 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsun_hip.so")
 
 STORE_18, STORE_12, STORE_8 = 18, 12, 8
+FILE_NATIVE, FILE_REFERENCE, FILE_REFERENCE_NOHEADER = 0, 1, 2
 F32, F64 = 4, 8
 _lib = None
 
 EXPORTS = [
     "sun_init", "sun_finalize", "sun_last_error", "sun_version",
     "sun_lattice_create", "sun_lattice_destroy", "sun_lattice_bytes", "sun_lattice_copy",
-    "sun_lattice_get_launch", "sun_lattice_set_launch",
+    "sun_lattice_get_launch", "sun_lattice_set_launch", "sun_lattice_save", "sun_lattice_load",
     "sun_cold_start", "sun_hot_start",
     "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
     "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
@@ -132,6 +133,8 @@ def lib() -> ctypes.CDLL:
         "sun_lattice_copy": (ci, [vp, vp, vp]),
         "sun_lattice_get_launch": (ci, [vp, ci, ctypes.POINTER(Launch)]),
         "sun_lattice_set_launch": (ci, [vp, ci, ctypes.POINTER(Launch)]),
+        "sun_lattice_save": (ci, [vp, ctypes.c_char_p, ci, ci, u64, u32]),
+        "sun_lattice_load": (ci, [vp, ctypes.c_char_p, ci, ci, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
         "sun_cold_start": (ci, [vp, vp]),
         "sun_hot_start": (ci, [vp, u64, vp]),
         "sun_heatbath": (ci, [vp, u64, u32, vp]),
@@ -256,6 +259,19 @@ class Lattice:
         return lattice_bytes(self.nc, self.dims, self.storage, self.precision, self.beta)
 
     # -- initialisation
+    # -- checkpoints
+    def save(self, path: str, fmt: int = FILE_NATIVE, save_prec: int = 8, seed: int = 0, next_sweep: int = 0):
+        """native: exact bytes + (seed, next_sweep); reference: SaveBin_Gauge layout"""
+        _check(lib().sun_lattice_save(self.h, os.fsencode(path), fmt, save_prec, seed, next_sweep),
+               "sun_lattice_save")
+
+    def load(self, path: str, fmt: int = FILE_NATIVE, save_prec: int = 8):
+        """returns (seed, next_sweep) for a native checkpoint, (None, None) otherwise"""
+        seed, sweep = ctypes.c_uint64(0), ctypes.c_uint32(0)
+        _check(lib().sun_lattice_load(self.h, os.fsencode(path), fmt, save_prec, ctypes.byref(seed),
+                                      ctypes.byref(sweep)), "sun_lattice_load")
+        return (int(seed.value), int(sweep.value)) if fmt == FILE_NATIVE else (None, None)
+
     # -- launch shapes (kernel 0: heatbath, 1: overrelaxation)
     def get_launch(self, kernel: int) -> Launch:
         out = Launch()
