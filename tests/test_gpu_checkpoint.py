@@ -42,7 +42,7 @@ def test_native_roundtrip_bit_exact(ctx, tmp_path, nc, storage, prec):
         assert np.array_equal(a.to_host(), b.to_host())
         raw = open(path, "rb").read()
         assert len(raw) == HEADER + a.bytes
-        crc = int.from_bytes(raw[68:72], "little")
+        crc = int.from_bytes(raw[72:76], "little")  # header offset of payload_crc32
         assert crc == zlib.crc32(raw[HEADER:])
 
 
