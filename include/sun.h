@@ -215,6 +215,7 @@ typedef struct {
 #define SUN_JOB_RUNNING 1
 #define SUN_JOB_DONE 2
 #define SUN_JOB_CANCELLED 3
+#define SUN_JOB_SUSPENDED 4
 
 typedef struct {
     uint64_t id;
@@ -244,6 +245,15 @@ int sun_server_status(sun_server *srv, uint64_t id, sun_job_status *st);
 /* rows {plaq_spatial, plaq_temporal, poly_re, poly_im} of every iteration of a done job */
 int sun_server_observables(sun_server *srv, uint64_t id, double *out, int max_rows);
 int sun_server_release(sun_server *srv, uint64_t id);  /* forget a finished job          */
+/* Takes a job off the device into a file: a pending job at once, a running one at its
+ * next iteration boundary (wait for state SUN_JOB_SUSPENDED).  The file holds the job
+ * description, the iterations done, their observable rows and a native checkpoint of
+ * the lattice (SUN_FILE_NATIVE layout).  No counterpart in the reference; its
+ * checkpoint format is io_gauge.cpp:429-700. */
+int sun_server_suspend(sun_server *srv, uint64_t id, const char *path);
+/* Submits a suspended job again, on this or another server: it continues from where it
+ * stopped and its results equal those of the job never suspended. */
+int sun_server_resume(sun_server *srv, const char *path, uint64_t *id);
 int sun_server_stats(sun_server *srv, sun_server_counters *st);
 double sun_server_clock(sun_server *srv);
 

@@ -50,6 +50,7 @@ EXPORTS = [
     "sun_server_create", "sun_server_destroy", "sun_server_submit", "sun_server_cancel",
     "sun_server_step", "sun_server_start", "sun_server_stop", "sun_server_wait", "sun_server_status",
     "sun_server_observables", "sun_server_release", "sun_server_stats", "sun_server_clock",
+    "sun_server_suspend", "sun_server_resume",
     "sun_stream_create", "sun_stream_destroy", "sun_stream_synchronize",
     "sun_event_create", "sun_event_destroy", "sun_event_record", "sun_event_elapsed_ms",
     "sun_device_name",
@@ -108,7 +109,7 @@ class ServerCounters(ctypes.Structure):
                 ("running", ctypes.c_int), ("pending", ctypes.c_int)]
 
 
-JOB_PENDING, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
+JOB_PENDING, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_SUSPENDED = 0, 1, 2, 3, 4
 
 
 def lib() -> ctypes.CDLL:
@@ -169,6 +170,8 @@ def lib() -> ctypes.CDLL:
         "sun_server_release": (ci, [vp, u64]),
         "sun_server_stats": (ci, [vp, ctypes.POINTER(ServerCounters)]),
         "sun_server_clock": (cd, [vp]),
+        "sun_server_suspend": (ci, [vp, u64, ctypes.c_char_p]),
+        "sun_server_resume": (ci, [vp, ctypes.c_char_p, ctypes.POINTER(u64)]),
         "sun_stream_create": (ci, [ctypes.POINTER(vp)]),
         "sun_stream_destroy": (ci, [vp]),
         "sun_stream_synchronize": (ci, [vp]),
@@ -445,6 +448,17 @@ class Server:
 
     def clock(self) -> float:
         return float(lib().sun_server_clock(self.h))
+
+    def suspend(self, jid: int, path: str):
+        """take the job off the device into a file (a running job: at its next
+        iteration boundary; it then reaches JOB_SUSPENDED)"""
+        _check(lib().sun_server_suspend(self.h, jid, os.fsencode(path)), "sun_server_suspend")
+
+    def resume(self, path: str) -> int:
+        """submit a suspended job again; returns its new id"""
+        jid = ctypes.c_uint64()
+        _check(lib().sun_server_resume(self.h, os.fsencode(path), ctypes.byref(jid)), "sun_server_resume")
+        return int(jid.value)
 
 
 class Timer:

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <map>
 #include <mutex>
 #include <string>
@@ -139,4 +140,10 @@ hipError_t launch_advance_counter(uint32_t* ctr, hipStream_t s);
 hipError_t launch_record_obs(const double2* src, int n, double2* history, const uint32_t* ctr, uint32_t first,
                              hipStream_t s);
 hipError_t launch_batch_advance(BatchEntry* table, int n, hipStream_t s);
+// native checkpoint of a lattice on an open file (sun_io.hip)
+int write_native(FILE* f, sun_lattice* L, uint64_t seed, uint32_t next_sweep, const char* path);
+int read_native(FILE* f, const sun_lattice_desc& d, size_t bytes, std::vector<unsigned char>& payload,
+                uint64_t* seed, uint32_t* next_sweep, const char* path);
+bool write_native_bytes(FILE* f, const sun_job_desc& d, const std::vector<unsigned char>& payload, uint64_t seed,
+                        uint32_t next_sweep);
 }  // namespace sun

@@ -90,7 +90,10 @@ size_t eo_of(const sun_lattice* L, size_t lex) {
     return (lex >> 1) + (size_t)parity * (size_t)(L->g.half);
 }
 
-int save_native(sun_lattice* L, const char* path, uint64_t seed, uint32_t next_sweep) {
+}  // namespace
+
+// native checkpoint of lat appended to an open file (header + payload)
+int sun::write_native(FILE* f, sun_lattice* L, uint64_t seed, uint32_t next_sweep, const char* path) {
     std::vector<unsigned char> payload(L->bytes);
     HIP_TRY(hipMemcpy(payload.data(), L->buf, L->bytes, hipMemcpyDeviceToHost));
     NativeHeader h;
@@ -106,41 +109,73 @@ int save_native(sun_lattice* L, const char* path, uint64_t seed, uint32_t next_s
     h.next_sweep = next_sweep;
     h.payload_bytes = L->bytes;
     h.payload_crc32 = crc32(payload.data(), payload.size());
+    if (fwrite(&h, sizeof(h), 1, f) != 1 || fwrite(payload.data(), 1, payload.size(), f) != payload.size())
+        return fail(SUN_ERR_INVALID, "writing %s failed", path);
+    return SUN_OK;
+}
+
+// native checkpoint of field bytes already on the host (a job not on the device)
+bool sun::write_native_bytes(FILE* f, const sun_job_desc& d, const std::vector<unsigned char>& payload, uint64_t seed,
+                             uint32_t next_sweep) {
+    NativeHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, kMagic, 8);
+    h.version = kVersion;
+    h.nc = d.nc;
+    for (int i = 0; i < 4; i++) h.dims[i] = d.dims[i];
+    h.storage = d.storage;
+    h.precision = d.precision;
+    h.beta = d.beta;
+    h.seed = seed;
+    h.next_sweep = next_sweep;
+    h.payload_bytes = payload.size();
+    h.payload_crc32 = crc32(payload.data(), payload.size());
+    return fwrite(&h, sizeof(h), 1, f) == 1 && fwrite(payload.data(), 1, payload.size(), f) == payload.size();
+}
+
+// reads a native checkpoint from an open file, checks it against the lattice
+// description d and returns its payload (the field bytes) on the host
+int sun::read_native(FILE* f, const sun_lattice_desc& d, size_t bytes, std::vector<unsigned char>& payload,
+                     uint64_t* seed, uint32_t* next_sweep, const char* path) {
+    NativeHeader h;
+    if (fread(&h, sizeof(h), 1, f) != 1) return fail(SUN_ERR_INVALID, "%s: short header", path);
+    if (memcmp(h.magic, kMagic, 8) != 0 || h.version != kVersion)
+        return fail(SUN_ERR_INVALID, "%s: not a native checkpoint (magic/version)", path);
+    if (h.nc != d.nc || h.storage != d.storage || h.precision != d.precision ||
+        memcmp(h.dims, d.dims, sizeof(h.dims)) != 0)
+        return fail(SUN_ERR_INVALID,
+                    "%s: checkpoint of N=%d %dx%dx%dx%d storage %d precision %d does not match the lattice", path,
+                    h.nc, h.dims[0], h.dims[1], h.dims[2], h.dims[3], h.storage, h.precision);
+    if (h.payload_bytes != bytes) return fail(SUN_ERR_INVALID, "%s: payload size mismatch", path);
+    payload.resize(bytes);
+    if (fread(payload.data(), 1, payload.size(), f) != payload.size())
+        return fail(SUN_ERR_INVALID, "%s: truncated payload", path);
+    if (crc32(payload.data(), payload.size()) != h.payload_crc32)
+        return fail(SUN_ERR_INVALID, "%s: payload checksum mismatch", path);
+    if (seed) *seed = h.seed;
+    if (next_sweep) *next_sweep = h.next_sweep;
+    return SUN_OK;
+}
+
+namespace {
+
+int save_native(sun_lattice* L, const char* path, uint64_t seed, uint32_t next_sweep) {
     File fo;
     fo.f = fopen(path, "wb");
     if (!fo.f) return fail(SUN_ERR_INVALID, "cannot open %s for writing", path);
-    if (fwrite(&h, sizeof(h), 1, fo.f) != 1 || fwrite(payload.data(), 1, payload.size(), fo.f) != payload.size())
-        return fail(SUN_ERR_INVALID, "writing %s failed", path);
-    if (fclose(fo.f) != 0) {
-        fo.f = nullptr;
-        return fail(SUN_ERR_INVALID, "writing %s failed", path);
-    }
+    SUN_TRY(write_native(fo.f, L, seed, next_sweep, path));
+    const int rc = fclose(fo.f);
     fo.f = nullptr;
-    return SUN_OK;
+    return rc == 0 ? SUN_OK : fail(SUN_ERR_INVALID, "writing %s failed", path);
 }
 
 int load_native(sun_lattice* L, const char* path, uint64_t* seed, uint32_t* next_sweep) {
     File fi;
     fi.f = fopen(path, "rb");
     if (!fi.f) return fail(SUN_ERR_INVALID, "cannot open %s", path);
-    NativeHeader h;
-    if (fread(&h, sizeof(h), 1, fi.f) != 1) return fail(SUN_ERR_INVALID, "%s: short header", path);
-    if (memcmp(h.magic, kMagic, 8) != 0 || h.version != kVersion)
-        return fail(SUN_ERR_INVALID, "%s: not a native checkpoint (magic/version)", path);
-    if (h.nc != L->d.nc || h.storage != L->d.storage || h.precision != L->d.precision ||
-        memcmp(h.dims, L->d.dims, sizeof(h.dims)) != 0)
-        return fail(SUN_ERR_INVALID,
-                    "%s: checkpoint of N=%d %dx%dx%dx%d storage %d precision %d does not match the lattice", path,
-                    h.nc, h.dims[0], h.dims[1], h.dims[2], h.dims[3], h.storage, h.precision);
-    if (h.payload_bytes != L->bytes) return fail(SUN_ERR_INVALID, "%s: payload size mismatch", path);
-    std::vector<unsigned char> payload(L->bytes);
-    if (fread(payload.data(), 1, payload.size(), fi.f) != payload.size())
-        return fail(SUN_ERR_INVALID, "%s: truncated payload", path);
-    if (crc32(payload.data(), payload.size()) != h.payload_crc32)
-        return fail(SUN_ERR_INVALID, "%s: payload checksum mismatch", path);
+    std::vector<unsigned char> payload;
+    SUN_TRY(read_native(fi.f, L->d, L->bytes, payload, seed, next_sweep, path));
     HIP_TRY(hipMemcpy(L->buf, payload.data(), L->bytes, hipMemcpyHostToDevice));
-    if (seed) *seed = h.seed;
-    if (next_sweep) *next_sweep = h.next_sweep;
     return SUN_OK;
 }
 

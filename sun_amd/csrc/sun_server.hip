@@ -29,6 +29,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -58,6 +59,13 @@ struct Job {
     sun_lattice* lat = nullptr;
     Group* group = nullptr;
     std::vector<double2> hist;  // host copy of the history once done
+    // suspension requested (running jobs: honoured at the next iteration boundary)
+    bool suspend = false;
+    std::string suspend_path;
+    // a resumed job: field bytes and the observable rows of its first iters_done
+    // iterations, copied to its memory at admission (kept until it retires)
+    std::vector<unsigned char> resume_field;
+    std::vector<double2> resume_hist;
 };
 
 struct Group {
@@ -318,8 +326,17 @@ int admit(sun_server* S) {
         }
         J->group = g;
         // queued behind whatever the group has in flight; touches only this job's memory
-        const hipError_t e = J->d.hot_start ? L->ops->hot_start(L->f, L->g, J->d.seed, g->stream)
-                                            : L->ops->cold_start(L->f, 4 * L->g.V, g->stream);
+        hipError_t e;
+        if (!J->resume_field.empty()) {
+            e = hipMemcpyAsync(L->buf, J->resume_field.data(), J->resume_field.size(), hipMemcpyHostToDevice,
+                               g->stream);
+            if (e == hipSuccess && !J->resume_hist.empty())
+                e = hipMemcpyAsync(L->hist, J->resume_hist.data(), sizeof(double2) * J->resume_hist.size(),
+                                   hipMemcpyHostToDevice, g->stream);
+        } else {
+            e = J->d.hot_start ? L->ops->hot_start(L->f, L->g, J->d.seed, g->stream)
+                               : L->ops->cold_start(L->f, 4 * L->g.V, g->stream);
+        }
         if (e != hipSuccess) {
             hipGetLastError();
             return fail(SUN_ERR_HIP, "server: start of job %llu: %s", (unsigned long long)J->id,
@@ -334,12 +351,56 @@ int admit(sun_server* S) {
     return SUN_OK;
 }
 
+// Job file: "SUNJOB01", u32 version (1), u32 has_lattice, the job description field
+// by field (i32 nc, i32 dims[4], f64 beta, i32 storage, i32 precision, u64 seed,
+// i32 hot_start, i32 n_iter, i32 n_or, i32 reunit_every), i32 iters_done, then
+// iters_done observable rows (4 f64: plaquette spatial/temporal, Polyakov re/im) and,
+// with has_lattice, a native lattice checkpoint (seed, next sweep = iters_done).
+constexpr char kJobMagic[8] = {'S', 'U', 'N', 'J', 'O', 'B', '0', '1'};
+
+template <class T>
+bool put(FILE* f, const T& v) {
+    return fwrite(&v, sizeof(T), 1, f) == 1;
+}
+template <class T>
+bool get(FILE* f, T& v) {
+    return fread(&v, sizeof(T), 1, f) == 1;
+}
+
+// the lattice comes from the device (a job on it) or from the host bytes of a resumed
+// job not yet admitted again
+int write_job_file(const Job* J, const char* path, const std::vector<double2>& hist) {
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(SUN_ERR_INVALID, "cannot open %s for writing", path);
+    const sun_job_desc& d = J->d;
+    const uint32_t version = 1, has_lattice = (J->lat || !J->resume_field.empty()) ? 1u : 0u;
+    bool ok = fwrite(kJobMagic, 1, 8, f) == 8 && put(f, version) && put(f, has_lattice) && put(f, d.nc);
+    for (int i = 0; i < 4; i++) ok = ok && put(f, d.dims[i]);
+    ok = ok && put(f, d.beta) && put(f, d.storage) && put(f, d.precision) && put(f, d.seed) && put(f, d.hot_start) &&
+         put(f, d.n_iter) && put(f, d.n_or) && put(f, d.reunit_every) && put(f, J->iters_done);
+    ok = ok && fwrite(hist.data(), sizeof(double2), hist.size(), f) == hist.size();
+    int rc = ok ? SUN_OK : fail(SUN_ERR_INVALID, "writing %s failed", path);
+    if (rc == SUN_OK && J->lat) {
+        rc = write_native(f, J->lat, d.seed, (uint32_t)J->iters_done, path);
+    } else if (rc == SUN_OK && has_lattice) {
+        if (!write_native_bytes(f, d, J->resume_field, d.seed, (uint32_t)J->iters_done))
+            rc = fail(SUN_ERR_INVALID, "writing %s failed", path);
+    }
+    if (fclose(f) != 0 && rc == SUN_OK) rc = fail(SUN_ERR_INVALID, "writing %s failed", path);
+    return rc;
+}
+
 // under S->mu; the job's group is idle and the job is one of its members
 int retire(sun_server* S, Job* J, bool done) {
     Group* g = J->group;
     if (done) {
         J->hist.resize((size_t)2 * J->d.n_iter);
         HIP_TRY(hipMemcpy(J->hist.data(), J->lat->hist, sizeof(double2) * J->hist.size(), hipMemcpyDeviceToHost));
+    } else if (J->suspend) {
+        std::vector<double2> rows((size_t)2 * J->iters_done);
+        if (!rows.empty())
+            HIP_TRY(hipMemcpy(rows.data(), J->lat->hist, sizeof(double2) * rows.size(), hipMemcpyDeviceToHost));
+        SUN_TRY(write_job_file(J, J->suspend_path.c_str(), rows));
     }
     g->members.erase(std::find(g->members.begin(), g->members.end(), J));
     g->dirty = true;
@@ -347,8 +408,10 @@ int retire(sun_server* S, Job* J, bool done) {
     J->lat = nullptr;
     S->arena.release(J->off);
     J->off = SIZE_MAX;
-    J->state = done ? SUN_JOB_DONE : SUN_JOB_CANCELLED;
+    J->state = done ? SUN_JOB_DONE : J->suspend ? SUN_JOB_SUSPENDED : SUN_JOB_CANCELLED;
     J->t_end = now_since(S->t0);
+    J->resume_field.clear();
+    J->resume_field.shrink_to_fit();
     S->running.erase(std::find(S->running.begin(), S->running.end(), J));
     S->admit_needed = true;
     if (done) S->jobs_done++;
@@ -366,7 +429,7 @@ int complete_group(sun_server* S, Group* g) {
         J->iters_done += k;
         S->link_updates += (uint64_t)k * (uint64_t)(1 + J->d.n_or) * 4ull * (uint64_t)J->lat->g.V;
         const bool done = J->iters_done >= J->d.n_iter;
-        if (done || J->cancel) SUN_TRY(retire(S, J, done));
+        if (done || J->cancel || J->suspend) SUN_TRY(retire(S, J, done));
     }
     S->steps++;
     S->cv.notify_all();
@@ -572,7 +635,7 @@ extern "C" int sun_server_cancel(sun_server* S, uint64_t id) {
     std::lock_guard<std::mutex> lk(S->mu);
     Job* J = find_job(S, id);
     if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    if (J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED) return SUN_OK;
+    if (J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED || J->state == SUN_JOB_SUSPENDED) return SUN_OK;
     J->cancel = true;
     if (J->state == SUN_JOB_PENDING) {
         S->pending.erase(std::find(S->pending.begin(), S->pending.end(), J));
@@ -613,11 +676,12 @@ extern "C" int sun_server_wait(sun_server* S, uint64_t id, double timeout_s) {
     Job* J = find_job(S, id);
     if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
     auto finished = [&] {
-        return J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED || S->worker_rc != SUN_OK;
+        return J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED || J->state == SUN_JOB_SUSPENDED ||
+               S->worker_rc != SUN_OK;
     };
     if (!S->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), finished))
         return fail(SUN_ERR_STATE, "job %llu not finished after %.1f s", (unsigned long long)id, timeout_s);
-    if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED)
+    if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED && J->state != SUN_JOB_SUSPENDED)
         return fail(SUN_ERR_STATE, "the scheduling thread stopped on an error before job %llu finished",
                     (unsigned long long)id);
     return SUN_OK;
@@ -666,7 +730,7 @@ extern "C" int sun_server_release(sun_server* S, uint64_t id) {
     std::lock_guard<std::mutex> lk(S->mu);
     Job* J = find_job(S, id);
     if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED)
+    if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED && J->state != SUN_JOB_SUSPENDED)
         return fail(SUN_ERR_STATE, "job %llu is still pending or running", (unsigned long long)id);
     S->jobs.erase(id);
     delete J;
@@ -688,3 +752,87 @@ extern "C" int sun_server_stats(sun_server* S, sun_server_counters* st) {
 }
 
 extern "C" double sun_server_clock(sun_server* S) { return S ? now_since(S->t0) : 0.0; }
+
+extern "C" int sun_server_suspend(sun_server* S, uint64_t id, const char* path) {
+    if (!S || !path) return fail(SUN_ERR_INVALID, "sun_server_suspend: NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    Job* J = find_job(S, id);
+    if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
+    if (J->state != SUN_JOB_PENDING && J->state != SUN_JOB_RUNNING)
+        return fail(SUN_ERR_STATE, "job %llu is not pending or running", (unsigned long long)id);
+    if (J->suspend) return fail(SUN_ERR_STATE, "job %llu is already being suspended", (unsigned long long)id);
+    J->suspend_path = path;
+    if (J->state == SUN_JOB_PENDING) {
+        // not on the device: the file holds the description and any resumed state
+        SUN_TRY(write_job_file(J, path, J->resume_hist));
+        S->pending.erase(std::find(S->pending.begin(), S->pending.end(), J));
+        J->suspend = true;
+        J->state = SUN_JOB_SUSPENDED;
+        J->t_end = now_since(S->t0);
+        J->resume_field.clear();
+        S->admit_needed = true;
+        S->cv.notify_all();
+        return SUN_OK;
+    }
+    J->suspend = true;  // retired, with its file written, at the group's next completion
+    S->cv.notify_all();
+    return SUN_OK;
+}
+
+extern "C" int sun_server_resume(sun_server* S, const char* path, uint64_t* id) {
+    if (!S || !path || !id) return fail(SUN_ERR_INVALID, "sun_server_resume: NULL argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(SUN_ERR_INVALID, "cannot open %s", path);
+    struct Closer {
+        FILE* f;
+        ~Closer() { fclose(f); }
+    } closer{f};
+    char magic[8];
+    uint32_t version = 0, has_lattice = 0;
+    sun_job_desc d;
+    memset(&d, 0, sizeof(d));
+    int iters_done = 0;
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kJobMagic, 8) == 0 && get(f, version) && version == 1 &&
+              get(f, has_lattice) && get(f, d.nc);
+    for (int i = 0; i < 4; i++) ok = ok && get(f, d.dims[i]);
+    ok = ok && get(f, d.beta) && get(f, d.storage) && get(f, d.precision) && get(f, d.seed) && get(f, d.hot_start) &&
+         get(f, d.n_iter) && get(f, d.n_or) && get(f, d.reunit_every) && get(f, iters_done);
+    if (!ok) return fail(SUN_ERR_INVALID, "%s: not a job file", path);
+    sun_lattice_desc ld = {d.nc, {d.dims[0], d.dims[1], d.dims[2], d.dims[3]}, d.beta, d.storage, d.precision};
+    SUN_TRY(check_desc(&ld));
+    if (iters_done < 0 || iters_done > d.n_iter || (iters_done > 0 && !has_lattice))
+        return fail(SUN_ERR_INVALID, "%s: inconsistent job state", path);
+    Job* J = new (std::nothrow) Job();
+    if (!J) return fail(SUN_ERR_NOMEM, "host allocation");
+    J->d = d;
+    J->iters_done = iters_done;
+    J->resume_hist.resize((size_t)2 * iters_done);
+    if (fread(J->resume_hist.data(), sizeof(double2), J->resume_hist.size(), f) != J->resume_hist.size()) {
+        delete J;
+        return fail(SUN_ERR_INVALID, "%s: truncated observables", path);
+    }
+    if (has_lattice) {
+        uint64_t seed = 0;
+        uint32_t next = 0;
+        const int rc = read_native(f, ld, 4 * (size_t)make_geom(ld.dims).V * link_bytes(&ld), J->resume_field, &seed,
+                                   &next, path);
+        if (rc != SUN_OK || seed != d.seed || (int)next != iters_done) {
+            delete J;
+            return rc != SUN_OK ? rc : fail(SUN_ERR_INVALID, "%s: lattice checkpoint does not match the job", path);
+        }
+    }
+    J->bytes = Arena::round_up(lattice_footprint(&ld, d.n_iter));
+    if (J->bytes > S->arena.capacity()) {
+        delete J;
+        return fail(SUN_ERR_BUDGET, "job needs %zu bytes, the server budget is %zu", J->bytes, S->arena.capacity());
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    J->id = S->next_id++;
+    J->t_submit = now_since(S->t0);
+    S->jobs[J->id] = J;
+    S->pending.push_back(J);
+    S->admit_needed = true;
+    *id = J->id;
+    S->cv.notify_all();
+    return SUN_OK;
+}

@@ -7,7 +7,10 @@
 * admission respects the budget (jobs queue, nothing over budget is resident) and
   rejects jobs that can never fit, with SUN_ERR_BUDGET;
 * cancel of pending and running jobs, unsupported and invalid requests, the
-  background scheduling thread with wait().
+  background scheduling thread with wait();
+* suspend/resume: a job taken off the device into a file (running: at its next
+  iteration boundary; pending: at once) and resumed on another server finishes with
+  observables bit-identical to the job never suspended.
 """
 import os
 import subprocess
@@ -187,3 +190,90 @@ def test_cancel_pending_and_running(ctx):
         while srv.status(third).state != sun_amd.JOB_DONE:
             srv.step()
         assert np.array_equal(srv.observables(third, j[6]), alone(ctx, j))
+
+
+def run_until(srv, jid, state, max_steps=200):
+    for _ in range(max_steps):
+        if srv.status(jid).state == state:
+            return
+        srv.step()
+    raise AssertionError(f"job {jid} did not reach state {state}")
+
+
+@pytest.mark.gpu
+def test_suspend_running_and_resume_elsewhere(ctx, tmp_path):
+    job = JOBS[7]  # 8 iterations
+    path = str(tmp_path / "job.bin")
+    with sun_amd.Server(ctx, 1 << 30) as srv:
+        other = submit(srv, JOBS[1])
+        jid = submit(srv, job)
+        srv.step()
+        srv.step()
+        srv.suspend(jid, path)
+        srv.step()  # honoured at this iteration boundary
+        st = srv.status(jid)
+        assert st.state == sun_amd.JOB_SUSPENDED and st.iters_done == 3
+        with pytest.raises(sun_amd.SunError):
+            srv.observables(jid, 1)
+        run_until(srv, other, sun_amd.JOB_DONE)
+        assert np.array_equal(srv.observables(other, JOBS[1][6]), alone(ctx, JOBS[1]))
+        assert srv.stats().bytes_in_use == 0
+    with sun_amd.Server(ctx, 1 << 30) as srv2:
+        j2 = srv2.resume(path)
+        assert srv2.status(j2).iters_done == 3
+        run_until(srv2, j2, sun_amd.JOB_DONE)
+        assert np.array_equal(srv2.observables(j2, job[6]), alone(ctx, job))
+
+
+@pytest.mark.gpu
+def test_suspend_pending_and_resumed_pending(ctx, tmp_path):
+    j = JOBS[0]
+    p1, p2, p3 = (str(tmp_path / f"j{i}.bin") for i in range(3))
+    with sun_amd.Server(ctx, job_bytes(j) * 3 // 2 + (64 << 10)) as srv:
+        first = submit(srv, j)
+        second = submit(srv, j)  # no room beside the first: pending
+        srv.step()
+        srv.suspend(second, p1)  # pending: written at once, no lattice
+        assert srv.status(second).state == sun_amd.JOB_SUSPENDED
+        srv.step()
+        srv.suspend(first, p2)
+        srv.step()
+        assert srv.status(first).state == sun_amd.JOB_SUSPENDED
+        # both resumed; the second one waits for memory, and is suspended again while
+        # pending (its lattice travels from the first file to the third)
+        a = srv.resume(p2)
+        b = srv.resume(p1)
+        srv.step()
+        c = srv.resume(p2)
+        assert srv.status(c).state == sun_amd.JOB_PENDING
+        srv.suspend(c, p3)
+        run_until(srv, a, sun_amd.JOB_DONE)
+        run_until(srv, b, sun_amd.JOB_DONE)
+        ref = alone(ctx, j)
+        assert np.array_equal(srv.observables(a, j[6]), ref)
+        assert np.array_equal(srv.observables(b, j[6]), ref)
+        d = srv.resume(p3)
+        run_until(srv, d, sun_amd.JOB_DONE)
+        assert np.array_equal(srv.observables(d, j[6]), ref)
+        with pytest.raises(sun_amd.SunError):
+            srv.resume(str(tmp_path / "missing.bin"))
+        open(p1, "r+b").write(b"XXXXXXXX")
+        with pytest.raises(sun_amd.SunError, match="job file"):
+            srv.resume(p1)
+
+
+@pytest.mark.gpu
+def test_suspend_with_background_thread(ctx, tmp_path):
+    # long enough that the suspend request lands before the job finishes
+    job = (3, (8, 8, 8, 8), 6.0, 12, 4, True, 60, 4, 1, 19)
+    path = str(tmp_path / "bg.bin")
+    with sun_amd.Server(ctx, 1 << 30) as srv:
+        srv.start()
+        jid = submit(srv, job)
+        srv.suspend(jid, path)
+        srv.wait(jid, 60.0)
+        assert srv.status(jid).state == sun_amd.JOB_SUSPENDED
+        j2 = srv.resume(path)
+        srv.wait(j2, 60.0)
+        srv.stop()
+        assert np.array_equal(srv.observables(j2, job[6]), alone(ctx, job))
