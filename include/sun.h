@@ -135,6 +135,25 @@ int sun_polyakov(sun_lattice *lat, void *stream, double out[2]);
 int sun_plaquette_async(sun_lattice *lat, void *stream, double *dev_out);
 int sun_polyakov_async(sun_lattice *lat, void *stream, double *dev_out);
 
+/* ---- link monitors (meas/linktrsum.h:24-90, meas/linkdetsum.h:23-100) ------------------- */
+/* One pass over the field; FP64 arithmetic on the stored links (12/8-real rows
+ * reconstructed as the update kernels read them), deterministic sums.
+ *   out[0], out[1]  Re, Im of sum_{x,mu} Tr U_mu(x) / (4 N V)   GaugeTrace[CUB]<Real>::Run
+ *                   (linktrsum.cu:130-150)
+ *   out[2], out[3]  Re, Im of sum_{x,mu} det U_mu(x) / (4 V)    GaugeDet[CUB]<Real>::Run
+ *                   (linkdetsum.cu:84-101)
+ *   out[4]          max over links of max_ij |(U U^+)_ij - delta_ij|
+ *   out[5]          max over links of |det U - 1|
+ * out[4] and out[5] have no reference counterpart: they measure how far FP32 links
+ * drift from SU(N) between reunitarisations. */
+int sun_link_monitor(sun_lattice *lat, void *stream, double out[6]);
+/* asynchronous form: writes double[6] to DEVICE memory dev_out, no host sync */
+int sun_link_monitor_async(sun_lattice *lat, void *stream, double *dev_out);
+/* GaugeTrace<Real>::Run (linktrsum.cu:130-150): out = mean Tr U / N */
+int sun_link_trace(sun_lattice *lat, void *stream, double out[2]);
+/* GaugeDet<Real>::Run (linkdetsum.cu:84-101): out = mean det U */
+int sun_link_det(sun_lattice *lat, void *stream, double out[2]);
+
 /* ---- host exchange (gaugearray.h:521-596 Get/Set, canonical FP64 layout) ---------------- */
 int sun_links_to_host(sun_lattice *lat, double *host, int layout);
 int sun_links_from_host(sun_lattice *lat, const double *host, int layout);

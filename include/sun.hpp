@@ -36,6 +36,7 @@
 #define SUN_HPP
 
 #include <complex>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 
@@ -231,6 +232,57 @@ class OnePolyakovLoop {
    private:
     _gauge<Real>& U_;
     std::complex<double> value_;
+};
+
+/* GaugeTraceCUB<Real>(gauge).Run() -> mean Tr U / N (linktrsum.h:24-55);
+   GaugeTrace<Real>(gauge, sum) takes the reference's device scratch array and ignores it */
+template <class Real>
+class GaugeTraceCUB {
+   public:
+    explicit GaugeTraceCUB(_gauge<Real>& U) : U_(U) {}
+    std::complex<double> Run(void* stream = nullptr) {
+        double out[2];
+        check(sun_link_trace(U_.handle(), stream, out), "GaugeTrace");
+        value_ = std::complex<double>(out[0], out[1]);
+        return value_;
+    }
+    std::complex<double> Value() const { return value_; }
+    void printValue() const { std::printf("Link Trace Sum: < %.12e : %.12e >\n", value_.real(), value_.imag()); }
+
+   private:
+    _gauge<Real>& U_;
+    std::complex<double> value_;
+};
+template <class Real>
+class GaugeTrace : public GaugeTraceCUB<Real> {
+   public:
+    GaugeTrace(_gauge<Real>& U, void* /*sum*/) : GaugeTraceCUB<Real>(U) {}
+};
+
+/* GaugeDetCUB<Real>(gauge).Run() -> mean det U (linkdetsum.h:23-66); GaugeDet alike */
+template <class Real>
+class GaugeDetCUB {
+   public:
+    explicit GaugeDetCUB(_gauge<Real>& U) : U_(U) {}
+    std::complex<double> Run(void* stream = nullptr) {
+        double out[2];
+        check(sun_link_det(U_.handle(), stream, out), "GaugeDet");
+        value_ = std::complex<double>(out[0], out[1]);
+        return value_;
+    }
+    std::complex<double> Value() const { return value_; }
+    void printValue() const {
+        std::printf("Link Determinant Sum: < %.12e : %.12e >\n", value_.real(), value_.imag());
+    }
+
+   private:
+    _gauge<Real>& U_;
+    std::complex<double> value_;
+};
+template <class Real>
+class GaugeDet : public GaugeDetCUB<Real> {
+   public:
+    explicit GaugeDet(_gauge<Real>& U) : GaugeDetCUB<Real>(U) {}
 };
 
 /* SaveBin_Gauge<Real, RealSaveConf>(gauge, filename, withheader) / ReadBin_Gauge:

@@ -68,6 +68,7 @@ def lib() -> ctypes.CDLL:
         "or_reunitarize": (None, [ci, ctypes.c_long, ci, dp]),
         "or_plaquette": (None, [ci, ip, dp, dp]),
         "or_polyakov": (None, [ci, ip, dp, dp]),
+        "or_link_monitor": (None, [ci, ctypes.c_long, dp, dp]),
         "or_matmul": (None, [ci, dp, dp, dp]),
         "or_det": (None, [ci, dp, dp]),
         "or_num_threads": (ci, []),
@@ -266,6 +267,14 @@ def polyakov(U, dims):
     out = np.zeros(2)
     lib().or_polyakov(U.shape[-1], _dims(dims), _dp(U), _dp(out))
     return complex(out[0], out[1])
+
+
+def link_monitor(U) -> np.ndarray:
+    """[mean Tr U / N re, im, mean det U re, im, max |U U^+ - 1|_ij, max |det U - 1|]"""
+    nc = U.shape[-1]
+    out = np.zeros(6)
+    lib().or_link_monitor(nc, U.size // (4 * nc * nc), _dp(np.ascontiguousarray(U)), _dp(out))
+    return out
 
 
 def matmul(A, B):

@@ -826,6 +826,52 @@ void or_plaquette(int nc, const int X[4], const double *U, double out[2]) {
     out[1] = t / norm;
 }
 
+/* Link monitors: linktrsum.cu:26-55,130-150 (sum of Tr U over all links / (4 N V))
+ * and linkdetsum.cu:31-60,84-101 (sum of det U / (4 V)); out[4] = max over links of
+ * max_ij |(U U^+)_ij - delta_ij|, out[5] = max over links of |det U - 1|. */
+void or_link_monitor(int nc, long V, const double *U, double out[6]) {
+    long nl = 4 * V;
+    long nchunk = (nl + CHUNK - 1) / CHUNK;
+    double *part = (double *)calloc((size_t)nchunk * 6, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nchunk; c++) {
+        double *p = part + 6 * c;
+        long end = (c + 1) * CHUNK < nl ? (c + 1) * CHUNK : nl;
+        for (long l = c * CHUNK; l < end; l++) {
+            mat a;
+            mload(nc, U + (size_t)l * nc * nc * 2, &a);
+            for (int i = 0; i < nc; i++) {
+                p[0] += a.e[i][i].re;
+                p[1] += a.e[i][i].im;
+            }
+            cx d = mdet(nc, &a);
+            p[2] += d.re;
+            p[3] += d.im;
+            double dd = cabs2(cmk(d.re - 1.0, d.im));
+            if (dd > p[5]) p[5] = dd;
+            for (int i = 0; i < nc; i++)
+                for (int j = 0; j < nc; j++) {
+                    cx e = cmk(i == j ? -1.0 : 0.0, 0.0);
+                    for (int k = 0; k < nc; k++) e = cadd(e, cmul(a.e[i][k], cconj(a.e[j][k])));
+                    double ee = cabs2(e);
+                    if (ee > p[4]) p[4] = ee;
+                }
+        }
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (long c = 0; c < nchunk; c++) {
+        for (int k = 0; k < 4; k++) acc[k] += part[6 * c + k];
+        for (int k = 4; k < 6; k++) if (part[6 * c + k] > acc[k]) acc[k] = part[6 * c + k];
+    }
+    free(part);
+    out[0] = acc[0] / (4.0 * nc * (double)V);
+    out[1] = acc[1] / (4.0 * nc * (double)V);
+    out[2] = acc[2] / (4.0 * (double)V);
+    out[3] = acc[3] / (4.0 * (double)V);
+    out[4] = sqrt(acc[4]);
+    out[5] = sqrt(acc[5]);
+}
+
 void or_polyakov(int nc, const int X[4], const double *U, double out[2]) {
     long V = volume_of(X);
     long half = V / 2;

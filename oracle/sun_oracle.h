@@ -86,6 +86,11 @@ void or_plaquette(int nc, const int dims[4], const double *U, double out[2]);
 /* polyakovloop.cu:53-99,651: complex mean trace / (N*Nx*Ny*Nz) */
 void or_polyakov(int nc, const int dims[4], const double *U, double out[2]);
 
+/* linktrsum.cu:130-150, linkdetsum.cu:84-101: out = {mean Tr U / N (re, im),
+ * mean det U (re, im), max_links max_ij |(U U^+ - 1)_ij|, max_links |det U - 1|};
+ * U holds nlinks = 4V links in the canonical layout */
+void or_link_monitor(int nc, long V, const double *U, double out[6]);
+
 /* ---- helpers ---------------------------------------------------------------- */
 void or_matmul(int nc, const double *A, const double *B, double *C);     /* C = A*B */
 void or_det(int nc, const double *A, double out[2]);

@@ -43,6 +43,7 @@ EXPORTS = [
     "sun_cold_start", "sun_hot_start",
     "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
     "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
+    "sun_link_monitor", "sun_link_monitor_async", "sun_link_trace", "sun_link_det",
     "sun_links_to_host", "sun_links_from_host",
     "sun_generate",
     "sun_batch_create", "sun_batch_destroy", "sun_batch_heatbath", "sun_batch_overrelax",
@@ -147,6 +148,10 @@ def lib() -> ctypes.CDLL:
         "sun_polyakov": (ci, [vp, vp, dp]),
         "sun_plaquette_async": (ci, [vp, vp, vp]),
         "sun_polyakov_async": (ci, [vp, vp, vp]),
+        "sun_link_monitor": (ci, [vp, vp, dp]),
+        "sun_link_monitor_async": (ci, [vp, vp, vp]),
+        "sun_link_trace": (ci, [vp, vp, dp]),
+        "sun_link_det": (ci, [vp, vp, dp]),
         "sun_links_to_host": (ci, [vp, dp, ci]),
         "sun_links_from_host": (ci, [vp, dp, ci]),
         "sun_generate": (ci, [vp, ctypes.POINTER(GenerateDesc), vp, dp]),
@@ -317,6 +322,22 @@ class Lattice:
     def polyakov(self, stream=None) -> complex:
         out = np.zeros(2)
         _check(lib().sun_polyakov(self.h, stream, _dp(out)), "sun_polyakov")
+        return complex(out[0], out[1])
+
+    def link_monitor(self, stream=None) -> np.ndarray:
+        """[mean Tr U / N (re, im), mean det U (re, im), max |U U^+ - 1|_ij, max |det U - 1|]"""
+        out = np.zeros(6)
+        _check(lib().sun_link_monitor(self.h, stream, _dp(out)), "sun_link_monitor")
+        return out
+
+    def link_trace(self, stream=None) -> complex:
+        out = np.zeros(2)
+        _check(lib().sun_link_trace(self.h, stream, _dp(out)), "sun_link_trace")
+        return complex(out[0], out[1])
+
+    def link_det(self, stream=None) -> complex:
+        out = np.zeros(2)
+        _check(lib().sun_link_det(self.h, stream, _dp(out)), "sun_link_det")
         return complex(out[0], out[1])
 
     # -- host exchange (canonical layout, complex128 [4, V, N, N])

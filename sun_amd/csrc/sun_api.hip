@@ -161,11 +161,11 @@ int sun::lattice_alloc(sun_ctx* ctx, const sun_lattice_desc* d, sun_lattice** ou
     (void)nfull;
     (void)tail;
     set_planes(L);
-    L->npart = std::max(plaquette_blocks(L->g), polyakov_blocks(L->g));
+    L->npart = partial_slots(L->g);
     e = hipMalloc(&L->partials, sizeof(double2) * (size_t)L->npart);
-    if (e == hipSuccess) e = hipMalloc(&L->dres, sizeof(double2) * 2);
+    if (e == hipSuccess) e = hipMalloc(&L->dres, sizeof(double2) * kResSlots);
     if (e == hipSuccess) e = hipMalloc(&L->sweep_dev, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipHostMalloc(&L->hres, sizeof(double2) * 2, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&L->hres, sizeof(double2) * kResSlots, hipHostMallocDefault);
     if (e != hipSuccess) {
         hipGetLastError();
         lattice_free(L);
@@ -188,8 +188,8 @@ size_t sun::lattice_footprint(const sun_lattice_desc* d, int n_hist) {
     if (check_desc(d) != SUN_OK) return 0;
     const Geom g = make_geom(d->dims);
     const size_t field = 4 * (size_t)g.V * link_bytes(d);
-    const size_t part = sizeof(double2) * (size_t)std::max(plaquette_blocks(g), polyakov_blocks(g));
-    return align256(field) + align256(part) + align256(2 * sizeof(double2)) + align256(sizeof(uint32_t)) +
+    const size_t part = sizeof(double2) * (size_t)partial_slots(g);
+    return align256(field) + align256(part) + align256(kResSlots * sizeof(double2)) + align256(sizeof(uint32_t)) +
            align256(sizeof(double2) * 2 * (size_t)(n_hist > 0 ? n_hist : 1));
 }
 
@@ -208,14 +208,14 @@ int sun::lattice_in_memory(sun_ctx* ctx, const sun_lattice_desc* d, char* mem, i
     L->external = true;
     const size_t field = 4 * (size_t)L->g.V * link_bytes(d);
     L->bytes = field;
-    L->npart = std::max(plaquette_blocks(L->g), polyakov_blocks(L->g));
+    L->npart = partial_slots(L->g);
     char* p = mem;
     L->buf = p;
     p += align256(field);
     L->partials = (double2*)p;
     p += align256(sizeof(double2) * (size_t)L->npart);
     L->dres = (double2*)p;
-    p += align256(2 * sizeof(double2));
+    p += align256(kResSlots * sizeof(double2));
     L->sweep_dev = (uint32_t*)p;
     p += align256(sizeof(uint32_t));
     L->hist = (double2*)p;
@@ -434,6 +434,53 @@ extern "C" int sun_plaquette_async(sun_lattice* lat, void* stream, double* dev_o
 extern "C" int sun_polyakov_async(sun_lattice* lat, void* stream, double* dev_out) {
     if (!lat || !dev_out) return fail(SUN_ERR_INVALID, "sun_polyakov_async: NULL argument");
     return polyakov_to(lat, pick_stream(lat, stream), (double2*)dev_out);
+}
+
+// link monitors: trace and determinant means, unitarity deviations
+int sun::link_monitor_to(sun_lattice* lat, hipStream_t s, double2* dev_out) {
+    const double V = (double)lat->g.V;
+    HIP_TRY(lat->ops->monitor_partial(lat->f, lat->g, lat->partials, s));
+    // linktrsum.cu:137 (4 N V), linkdetsum.cu:93 (4 V)
+    HIP_TRY(launch_monitor_finalize(lat->partials, monitor_blocks(lat->g), 1.0 / (4.0 * lat->d.nc * V),
+                                    1.0 / (4.0 * V), dev_out, s));
+    return SUN_OK;
+}
+
+extern "C" int sun_link_monitor(sun_lattice* lat, void* stream, double out[6]) {
+    if (!lat || !out) return fail(SUN_ERR_INVALID, "sun_link_monitor: NULL argument");
+    if (!lat->hres) return fail(SUN_ERR_STATE, "sun_link_monitor: lattice has no host result buffer");
+    hipStream_t s = pick_stream(lat, stream);
+    SUN_TRY(link_monitor_to(lat, s, lat->dres));
+    HIP_TRY(hipMemcpyAsync(lat->hres, lat->dres, 3 * sizeof(double2), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < 3; i++) {
+        out[2 * i] = lat->hres[i].x;
+        out[2 * i + 1] = lat->hres[i].y;
+    }
+    return SUN_OK;
+}
+
+extern "C" int sun_link_monitor_async(sun_lattice* lat, void* stream, double* dev_out) {
+    if (!lat || !dev_out) return fail(SUN_ERR_INVALID, "sun_link_monitor_async: NULL argument");
+    return link_monitor_to(lat, pick_stream(lat, stream), (double2*)dev_out);
+}
+
+extern "C" int sun_link_trace(sun_lattice* lat, void* stream, double out[2]) {
+    if (!out) return fail(SUN_ERR_INVALID, "sun_link_trace: NULL argument");
+    double m[6];
+    SUN_TRY(sun_link_monitor(lat, stream, m));
+    out[0] = m[0];
+    out[1] = m[1];
+    return SUN_OK;
+}
+
+extern "C" int sun_link_det(sun_lattice* lat, void* stream, double out[2]) {
+    if (!out) return fail(SUN_ERR_INVALID, "sun_link_det: NULL argument");
+    double m[6];
+    SUN_TRY(sun_link_monitor(lat, stream, m));
+    out[0] = m[2];
+    out[1] = m[3];
+    return SUN_OK;
 }
 
 // ---------------------------------------------------------------------------------

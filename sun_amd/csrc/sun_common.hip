@@ -45,6 +45,38 @@ __global__ void __launch_bounds__(kReduceBlock) k_finalize(const double2* partia
     if (threadIdx.x == 0) out[0] = make_double2(v.x * s0, v.y * s1);
 }
 
+// trace and determinant segments summed in the order of ordered_sum, maxima of the
+// deviation segment; square roots of the maxima at the end
+__global__ void __launch_bounds__(kReduceBlock) k_monitor_finalize(const double2* partials, int n, double s_tr,
+                                                                   double s_det, double2* out) {
+    const double2 tr = ordered_sum(partials, n);
+    __syncthreads();
+    const double2 dt = ordered_sum(partials + n, n);
+    double2 m = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < n; i += kReduceBlock) {
+        m.x = fmax(m.x, partials[2 * n + i].x);
+        m.y = fmax(m.y, partials[2 * n + i].y);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        m.x = fmax(m.x, __shfl_down(m.x, off, 64));
+        m.y = fmax(m.y, __shfl_down(m.y, off, 64));
+    }
+    __shared__ double2 wm[kReduceBlock / 64];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 1; i < kReduceBlock / 64; i++) {
+            m.x = fmax(m.x, wm[i].x);
+            m.y = fmax(m.y, wm[i].y);
+        }
+        out[0] = make_double2(tr.x * s_tr, tr.y * s_tr);
+        out[1] = make_double2(dt.x * s_det, dt.y * s_det);
+        out[2] = make_double2(sqrt(m.x), sqrt(m.y));
+    }
+}
+
 __global__ void __launch_bounds__(kReduceBlock) k_batch_finalize(const BatchEntry* table, double2* out) {
     const BatchEntry& E = table[blockIdx.x];
     double2 v = ordered_sum(E.partials, E.nblocks);
@@ -97,6 +129,12 @@ hipError_t launch_batch_advance(BatchEntry* table, int n, hipStream_t s) {
 
 hipError_t launch_finalize(const double2* partials, int n, double s0, double s1, double2* out, hipStream_t s) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kReduceBlock), 0, s, partials, n, s0, s1, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_monitor_finalize(const double2* partials, int n, double s_tr, double s_det, double2* out,
+                                   hipStream_t s) {
+    hipLaunchKernelGGL(k_monitor_finalize, dim3(1), dim3(kReduceBlock), 0, s, partials, n, s_tr, s_det, out);
     return hipGetLastError();
 }
 

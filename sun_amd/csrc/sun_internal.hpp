@@ -79,6 +79,12 @@ struct sun_ctx {
     sun::Tuner tuner;
 };
 
+namespace sun {
+// device result slots of a lattice: plaquette and Polyakov of sun_generate, or the
+// three of the link monitor
+constexpr int kResSlots = 3;
+}  // namespace sun
+
 struct sun_lattice {
     sun_ctx* ctx = nullptr;
     sun_lattice_desc d{};
@@ -89,8 +95,8 @@ struct sun_lattice {
     size_t bytes = 0;
     double2* partials = nullptr;   // per-block reduction scratch
     int npart = 0;
-    double2* dres = nullptr;       // device results: [0] plaquette/polyakov scratch
-    double2* hres = nullptr;       // pinned host copy
+    double2* dres = nullptr;       // device results (kResSlots)
+    double2* hres = nullptr;       // pinned host copy (kResSlots), null for server lattices
     uint32_t* sweep_dev = nullptr; // device sweep counter for graph replay
     double2* obs_dev = nullptr;    // device observable history of sun_generate
     int obs_cap = 0;
@@ -134,6 +140,7 @@ int heatbath_sweep(sun_lattice* lat, uint64_t seed, uint32_t sweep, const uint32
 int overrelax_sweep(sun_lattice* lat, hipStream_t s);
 int plaquette_to(sun_lattice* lat, hipStream_t s, double2* dev_out);
 int polyakov_to(sun_lattice* lat, hipStream_t s, double2* dev_out);
+int link_monitor_to(sun_lattice* lat, hipStream_t s, double2* dev_out);
 int tune_lattice(sun_lattice* lat, hipStream_t s);
 // small helper kernels (sun_common.hip)
 hipError_t launch_advance_counter(uint32_t* ctr, hipStream_t s);

@@ -107,6 +107,8 @@ struct OpsTable {
     hipError_t (*batch_reunitarize)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     hipError_t (*batch_plaquette_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     hipError_t (*batch_polyakov_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
+    // link trace / determinant sums and unitarity deviations, 3 * monitor_blocks(geom) partials
+    hipError_t (*monitor_partial)(const Field&, const Geom&, double2* partials, hipStream_t);
     int max_block;   // largest workgroup the update kernels are compiled for
     int has_piped;   // the variant has the software-pipelined overrelaxation kernel
     int piped_max_block;
@@ -133,12 +135,23 @@ inline int polyakov_blocks(const Geom& g) {
     int ts = g.X[0] * g.X[1] * g.X[2];
     return (ts + kPolySites - 1) / kPolySites;
 }
+// link monitor workgroups (one site per thread) and the partials every reduction of
+// a lattice needs: the monitor writes three segments of monitor_blocks entries
+inline int monitor_blocks(const Geom& g) { return (g.V + kReduceBlock - 1) / kReduceBlock; }
+inline int partial_slots(const Geom& g) {
+    const int a = 3 * monitor_blocks(g), b = plaquette_blocks(g), c = polyakov_blocks(g);
+    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
 inline int update_blocks(const Geom& g, int block) { return (g.half + block - 1) / block; }
 inline int link_blocks(const Geom& g) { return (4 * g.V + kReduceBlock - 1) / kReduceBlock; }
 
 // sum of partials[0..n) in a fixed order, times `scale`, into out (device)
 hipError_t launch_finalize(const double2* partials, int n, double scale0, double scale1, double2* out,
                            hipStream_t s);
+// link monitor: out[0] = trace sum * scale_tr, out[1] = determinant sum * scale_det,
+// out[2] = (max deviation, max |det - 1|) from the three segments of n partials
+hipError_t launch_monitor_finalize(const double2* partials, int n, double scale_tr, double scale_det, double2* out,
+                                   hipStream_t s);
 // finalize for every lattice of a batch table: out[i] = (sum of entry i's partials)
 // * entry i's scale, in the same order as launch_finalize
 hipError_t launch_batch_finalize(const BatchEntry* table, int nlat, double2* out, hipStream_t s);

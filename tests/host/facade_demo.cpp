@@ -39,6 +39,10 @@ int main(int argc, char** argv) {
             std::complex<double> l = poly.Run();
             printf("%d %.17g %.17g %.17g %.17g\n", it, p.real(), p.imag(), l.real(), l.imag());
         }
+        GaugeTraceCUB<double> trace(U);
+        GaugeDet<double> det(U);
+        std::complex<double> t = trace.Run(), d = det.Run();
+        printf("monitor %.17g %.17g %.17g %.17g\n", t.real(), t.imag(), d.real(), d.imag());
         std::vector<double> links((size_t)4 * PARAMS::Volume * 18);
         U.ToHost(links.data());
         FILE* f = fopen(argv[7], "wb");

@@ -193,11 +193,16 @@ def test_cpp_facade_program_matches_c_abi(ctx, tmp_path):
     r = subprocess.run([FACADE_DEMO, *map(str, dims), str(seed), str(iters), str(out)], capture_output=True,
                        text=True, timeout=100)
     assert r.returncode == 0, r.stderr
-    rows = np.array([[float(v) for v in line.split()[1:]] for line in r.stdout.strip().splitlines()])
+    lines = r.stdout.strip().splitlines()
+    assert lines[-1].startswith("monitor ")
+    monitor = [float(v) for v in lines[-1].split()[1:]]
+    rows = np.array([[float(v) for v in line.split()[1:]] for line in lines[:-1]])
     links = np.fromfile(out, dtype=np.float64).view(np.complex128).reshape(4, -1, 3, 3)
     with sun_amd.Lattice(ctx, 3, dims, 6.0, 18, 8) as lat:
         lat.hot_start(seed)
         obs = lat.generate(seed=seed, n_iter=iters, n_or=4, reunit_every=0, use_graph=False)
         G = lat.to_host()
+        m = lat.link_monitor()
     assert np.array_equal(rows, obs)
+    assert monitor == list(m[:4])
     assert np.array_equal(links, G)
