@@ -848,20 +848,21 @@ void or_link_monitor(int nc, long V, const double *U, double out[6]) {
             p[2] += d.re;
             p[3] += d.im;
             double dd = cabs2(cmk(d.re - 1.0, d.im));
-            if (dd > p[5]) p[5] = dd;
+            if (dd > p[5] || dd != dd) p[5] = dd;  /* a NaN stays */
             for (int i = 0; i < nc; i++)
                 for (int j = 0; j < nc; j++) {
                     cx e = cmk(i == j ? -1.0 : 0.0, 0.0);
                     for (int k = 0; k < nc; k++) e = cadd(e, cmul(a.e[i][k], cconj(a.e[j][k])));
                     double ee = cabs2(e);
-                    if (ee > p[4]) p[4] = ee;
+                    if (ee > p[4] || ee != ee) p[4] = ee;
                 }
         }
     }
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (long c = 0; c < nchunk; c++) {
         for (int k = 0; k < 4; k++) acc[k] += part[6 * c + k];
-        for (int k = 4; k < 6; k++) if (part[6 * c + k] > acc[k]) acc[k] = part[6 * c + k];
+        for (int k = 4; k < 6; k++)
+            if (part[6 * c + k] > acc[k] || part[6 * c + k] != part[6 * c + k]) acc[k] = part[6 * c + k];
     }
     free(part);
     out[0] = acc[0] / (4.0 * nc * (double)V);

@@ -54,13 +54,13 @@ __global__ void __launch_bounds__(kReduceBlock) k_monitor_finalize(const double2
     const double2 dt = ordered_sum(partials + n, n);
     double2 m = make_double2(0.0, 0.0);
     for (int i = threadIdx.x; i < n; i += kReduceBlock) {
-        m.x = fmax(m.x, partials[2 * n + i].x);
-        m.y = fmax(m.y, partials[2 * n + i].y);
+        m.x = nan_max(m.x, partials[2 * n + i].x);
+        m.y = nan_max(m.y, partials[2 * n + i].y);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        m.x = fmax(m.x, __shfl_down(m.x, off, 64));
-        m.y = fmax(m.y, __shfl_down(m.y, off, 64));
+        m.x = nan_max(m.x, __shfl_down(m.x, off, 64));
+        m.y = nan_max(m.y, __shfl_down(m.y, off, 64));
     }
     __shared__ double2 wm[kReduceBlock / 64];
     if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
@@ -68,8 +68,8 @@ __global__ void __launch_bounds__(kReduceBlock) k_monitor_finalize(const double2
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int i = 1; i < kReduceBlock / 64; i++) {
-            m.x = fmax(m.x, wm[i].x);
-            m.y = fmax(m.y, wm[i].y);
+            m.x = nan_max(m.x, wm[i].x);
+            m.y = nan_max(m.y, wm[i].y);
         }
         out[0] = make_double2(tr.x * s_tr, tr.y * s_tr);
         out[1] = make_double2(dt.x * s_det, dt.y * s_det);

@@ -135,6 +135,14 @@ inline int polyakov_blocks(const Geom& g) {
     int ts = g.X[0] * g.X[1] * g.X[2];
     return (ts + kPolySites - 1) / kPolySites;
 }
+// maximum that keeps a NaN (fmax drops it): a monitor must not report a field that
+// went non-finite as an exact one
+SUN_HD double nan_max(double a, double b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    return b > a ? b : a;
+}
+
 // link monitor workgroups (one site per thread) and the partials every reduction of
 // a lattice needs: the monitor writes three segments of monitor_blocks entries
 inline int monitor_blocks(const Geom& g) { return (g.V + kReduceBlock - 1) / kReduceBlock; }

@@ -78,3 +78,11 @@ def test_haar_field(oracle, nc):
     n = 4 * 512
     sigma = np.sqrt(1.0 / n) / nc  # bounds the std of Re and Im of the mean of Tr U / N
     assert abs(got[0]) < 5 * sigma and abs(got[1]) < 5 * sigma
+
+
+def test_nan_propagates(oracle):
+    U = oracle.cold_start(3, (2, 2, 2, 4))
+    U[2, 5, 1, 2] = np.nan
+    got = oracle.link_monitor(U)
+    # an off-diagonal entry: the trace stays finite, everything else is NaN
+    assert np.isfinite(got[:2]).all() and np.isnan(got[2:]).all()
