@@ -154,6 +154,25 @@ int sun_link_trace(sun_lattice *lat, void *stream, double out[2]);
 /* GaugeDet<Real>::Run (linkdetsum.cu:84-101): out = mean det U */
 int sun_link_det(sun_lattice *lat, void *stream, double out[2]);
 
+/* ---- measurement chain (smear/ape.cu, smear/multihit.cu, meas/wilsonloop.cu) ----------- */
+/* The reference's static-potential chain (test.cpp:170-230) on the resident even/odd
+ * field; synchronous; scratch device memory is allocated for the call. */
+/* ApplyAPEinSpace / ApplyAPEinTime (ape.cu:198-264): `steps` rounds; each smears the
+ * spatial directions (time_dir 0) or the temporal one (time_dir 1) from the previous
+ * round's field with link <- Proj[(U + w S) / (1 + 6w)], S the staples in the spatial
+ * directions; Proj = SU3project with nhit hits for N = 3 (tol unused), project_link
+ * from the old link with nhit hits and tolerance tol otherwise.  In place. */
+int sun_ape_smear(sun_lattice *lat, double w, int steps, int nhit, double tol, int time_dir, void *stream);
+/* ApplyMultiHit (multihit.cu:291-333): out = in with every temporal link replaced by
+ * the mean of itself and nhit heatbath updates in its staple (beta of `in`), RNG
+ * stream (seed, site, hit); out: another lattice of the same N, extents and precision,
+ * 18-real storage (the mean is not in SU(N)). */
+int sun_multihit(sun_lattice *in, sun_lattice *out, uint64_t seed, int nhit, void *stream);
+/* WilsonLoop (wilsonloop.cu:494-516): out[2 (T + r (tmax+1)) + {0,1}] = Re, Im of
+ * W(r, T) = sum_{x, mu<3} Tr(L_r(x) t1 L_r(x+T e_3)^+ t0^+) / (3 N V) for r <= rmax,
+ * T <= tmax (rmax at most the smallest spatial extent, tmax at most the temporal). */
+int sun_wilson_loops(sun_lattice *lat, int rmax, int tmax, double *out, void *stream);
+
 /* ---- host exchange (gaugearray.h:521-596 Get/Set, canonical FP64 layout) ---------------- */
 int sun_links_to_host(sun_lattice *lat, double *host, int layout);
 int sun_links_from_host(sun_lattice *lat, const double *host, int layout);

@@ -39,6 +39,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "sun.h"
 
@@ -127,7 +128,10 @@ class _gauge {
         (void)size;
         (void)border;
         if (mode != Device) throw Error("_gauge: only Device arrays are supported");
-        if (!evenodd) throw Error("_gauge: only even/odd ordered arrays are supported");
+        // Device storage is always even/odd ordered; a lexicographic ("normal order")
+        // array, which the reference's smearing and Wilson loops require
+        // (ape.cu:202, wilsonloop.cu:504), differs only in its host exchange order.
+        evenodd_ = evenodd;
         if (!runtime().ctx) throw Error("_gauge: call initCULQCD first");
         sun_lattice_desc d;
         d.nc = runtime().nc;
@@ -148,18 +152,54 @@ class _gauge {
     void Init(RNG& r) { check(sun_hot_start(lat_, r.Seed(), nullptr), "_gauge::Init(RNG)"); }
     void Copy(_gauge<Real>& from) { check(sun_lattice_copy(lat_, from.lat_, nullptr), "_gauge::Copy"); }
     ArrayType Type() const { return type_; }
-    bool EvenOdd() const { return true; }
+    bool EvenOdd() const { return evenodd_; }
     int Size() const { return PARAMS::size; }
     sun_lattice* handle() const { return lat_; }
-    /* canonical FP64 host copy: double[4][V][N][N][2] */
-    void ToHost(double* host) const { check(sun_links_to_host(lat_, host, SUN_LAYOUT_CANONICAL), "_gauge::ToHost"); }
+    /* FP64 host copy double[4][V][N][N][2], sites in even/odd order (index.h:133-141)
+       or, for a normal-order array, lexicographic order */
+    void ToHost(double* host) const {
+        if (evenodd_) {
+            check(sun_links_to_host(lat_, host, SUN_LAYOUT_CANONICAL), "_gauge::ToHost");
+            return;
+        }
+        std::vector<double> eo(host_reals());
+        check(sun_links_to_host(lat_, eo.data(), SUN_LAYOUT_CANONICAL), "_gauge::ToHost");
+        permute(eo.data(), host, true);
+    }
     void FromHost(const double* host) {
-        check(sun_links_from_host(lat_, host, SUN_LAYOUT_CANONICAL), "_gauge::FromHost");
+        if (evenodd_) {
+            check(sun_links_from_host(lat_, host, SUN_LAYOUT_CANONICAL), "_gauge::FromHost");
+            return;
+        }
+        std::vector<double> eo(host_reals());
+        permute(host, eo.data(), false);
+        check(sun_links_from_host(lat_, eo.data(), SUN_LAYOUT_CANONICAL), "_gauge::FromHost");
     }
 
    private:
+    size_t host_reals() const { return (size_t)4 * PARAMS::Volume * 2 * runtime().nc * runtime().nc; }
+    // to_lex: even/odd src -> lexicographic dst; else the reverse
+    void permute(const double* src, double* dst, bool to_lex) const {
+        const size_t V = (size_t)PARAMS::Volume, m = (size_t)2 * runtime().nc * runtime().nc;
+        const int* X = PARAMS::Grid;
+        for (size_t lex = 0; lex < V; lex++) {
+            const size_t x0 = lex % X[0], x1 = (lex / X[0]) % X[1], x2 = (lex / ((size_t)X[0] * X[1])) % X[2],
+                         x3 = lex / ((size_t)X[0] * X[1] * X[2]);
+            const size_t eo = lex / 2 + ((x0 + x1 + x2 + x3) & 1) * (V / 2);
+            for (int mu = 0; mu < 4; mu++) {
+                const size_t a = (mu * V + eo) * m, b = (mu * V + lex) * m;
+                for (size_t k = 0; k < m; k++) {
+                    if (to_lex)
+                        dst[b + k] = src[a + k];
+                    else
+                        dst[a + k] = src[b + k];
+                }
+            }
+        }
+    }
     sun_lattice* lat_ = nullptr;
     ArrayType type_ = SOA;
+    bool evenodd_ = true;
 };
 typedef _gauge<float> gauges;
 typedef _gauge<double> gauged;
@@ -284,6 +324,37 @@ class GaugeDet : public GaugeDetCUB<Real> {
    public:
     explicit GaugeDet(_gauge<Real>& U) : GaugeDetCUB<Real>(U) {}
 };
+
+/* The measurement chain of the reference's test.cpp:170-230 (smearing, multihit,
+   Wilson loops) on a gauge array of either ordering. */
+/* ApplyAPEinSpace<Real>(gauge, w, steps, nhits, tol)   ape.cu:198-230 */
+template <class Real>
+void ApplyAPEinSpace(_gauge<Real>& U, Real w, int steps, int nhits, Real tol) {
+    check(sun_ape_smear(U.handle(), (double)w, steps, nhits, (double)tol, 0, nullptr), "ApplyAPEinSpace");
+}
+/* ApplyAPEinTime<Real>(gauge, w, steps, nhits, tol)   ape.cu:236-264 */
+template <class Real>
+void ApplyAPEinTime(_gauge<Real>& U, Real w, int steps, int nhits, Real tol) {
+    check(sun_ape_smear(U.handle(), (double)w, steps, nhits, (double)tol, 1, nullptr), "ApplyAPEinTime");
+}
+/* ApplyMultiHit(in, out, RNG, nhit)   multihit.cu:291-333; out must be SOA (18-real) */
+template <class Real>
+void ApplyMultiHit(_gauge<Real>& in, _gauge<Real>& out, RNG& rng, int nhit) {
+    check(sun_multihit(in.handle(), out.handle(), rng.Seed(), nhit, nullptr), "ApplyMultiHit");
+}
+/* WilsonLoop<Real>(gauge, res, Rmax, Tmax)   wilsonloop.cu:494-516:
+   res[it + r * (Tmax + 1)] = W(r, it), r <= Rmax, it <= Tmax */
+template <class Real>
+void WilsonLoop(_gauge<Real>& U, std::complex<double>* res, int radius, int Tmax) {
+    std::vector<double> out((size_t)2 * (radius + 1) * (Tmax + 1));
+    check(sun_wilson_loops(U.handle(), radius, Tmax, out.data(), nullptr), "WilsonLoop");
+    for (size_t i = 0; i < out.size() / 2; i++) res[i] = std::complex<double>(out[2 * i], out[2 * i + 1]);
+}
+/* CWilsonLoop: the reference's lower-memory variant (wilsonloop.cu:420-468), same values */
+template <class Real>
+void CWilsonLoop(_gauge<Real>& U, std::complex<double>* res, int radius, int Tmax) {
+    WilsonLoop<Real>(U, res, radius, Tmax);
+}
 
 /* SaveBin_Gauge<Real, RealSaveConf>(gauge, filename, withheader) / ReadBin_Gauge:
  * the reference's binary configuration layout (io_gauge.cpp:429-583, :605-700);

@@ -69,6 +69,11 @@ def lib() -> ctypes.CDLL:
         "or_plaquette": (None, [ci, ip, dp, dp]),
         "or_polyakov": (None, [ci, ip, dp, dp]),
         "or_link_monitor": (None, [ci, ctypes.c_long, dp, dp]),
+        "or_project_link": (None, [ci, dp, dp, ci, cd]),
+        "or_su3_project": (None, [dp, ci]),
+        "or_ape_smear": (ci, [ci, ip, dp, cd, ci, ci, cd, ci]),
+        "or_multihit": (ci, [ci, ip, cd, u64, ci, dp, dp]),
+        "or_wilson_loops": (ci, [ci, ip, dp, ci, ci, dp]),
         "or_matmul": (None, [ci, dp, dp, dp]),
         "or_det": (None, [ci, dp, dp]),
         "or_num_threads": (ci, []),
@@ -275,6 +280,38 @@ def link_monitor(U) -> np.ndarray:
     out = np.zeros(6)
     lib().or_link_monitor(nc, U.size // (4 * nc * nc), _dp(np.ascontiguousarray(U)), _dp(out))
     return out
+
+
+def project_link(w, q, nhit: int, tol: float = 0.0):
+    w = np.array(w, dtype=np.complex128)
+    q = np.ascontiguousarray(q, dtype=np.complex128)
+    lib().or_project_link(w.shape[-1], _dp(w), _dp(q), nhit, tol)
+    return w
+
+
+def su3_project(F, nhit: int):
+    F = np.array(F, dtype=np.complex128)
+    lib().or_su3_project(_dp(F), nhit)
+    return F
+
+
+def ape_smear(U, dims, w: float, steps: int, nhit: int, tol: float = 0.0, time_dir: bool = False):
+    """in place on an 18-real field"""
+    _check(lib().or_ape_smear(U.shape[-1], _dims(dims), _dp(U), w, steps, nhit, tol, int(time_dir)), "ape_smear")
+    return U
+
+
+def multihit(U, dims, beta: float, seed: int, nhit: int):
+    out = np.empty_like(U)
+    _check(lib().or_multihit(U.shape[-1], _dims(dims), beta, seed, nhit, _dp(U), _dp(out)), "multihit")
+    return out
+
+
+def wilson_loops(U, dims, rmax: int, tmax: int) -> np.ndarray:
+    """complex [rmax + 1, tmax + 1]"""
+    out = np.zeros((rmax + 1) * (tmax + 1) * 2)
+    _check(lib().or_wilson_loops(U.shape[-1], _dims(dims), _dp(U), rmax, tmax, _dp(out)), "wilson_loops")
+    return (out[0::2] + 1j * out[1::2]).reshape(rmax + 1, tmax + 1)
 
 
 def matmul(A, B):

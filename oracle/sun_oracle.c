@@ -24,6 +24,7 @@
 #define RNG_TAG_HB   0x10000000u
 #define RNG_TAG_HOT  0x20000000u
 #define RNG_TAG_TEST 0x30000000u
+#define RNG_TAG_MH   0x40000000u
 
 /* ------------------------------------------------------------------------- */
 /* complex arithmetic in the reference's operation order (complex.h:82-230)    */
@@ -519,14 +520,15 @@ void or_hot_start(int nc, const int X[4], uint64_t seed, int storage, double *U)
 /* ------------------------------------------------------------------------- */
 /* staple (monte.cu:63-80)                                                     */
 /* ------------------------------------------------------------------------- */
-static mat staple_at(int nc, const int X[4], const double *U, int parity, int mu, int id) {
+/* nu runs over 0..nu_end-1 without mu (nu_end 3: the spatial staples of APE smearing) */
+static mat staple_range(int nc, const int X[4], const double *U, int parity, int mu, int id, int nu_end) {
     long V = volume_of(X);
     long half = V / 2;
     long idxoddbit = id + parity * half;
     mat staple, link, t;
     mzero(nc, &staple);
     int newidmu1 = or_neighbor_eo(id, parity, X, mu, 1, -1, 0);
-    for (int nu = 0; nu < 4; nu++) {
+    for (int nu = 0; nu < nu_end; nu++) {
         if (nu == mu) continue;
         mat A, B, C;
         /* up: U_nu(x) U_mu(x+nu) U_nu(x+mu)^dagger */
@@ -548,6 +550,10 @@ static mat staple_at(int nc, const int X[4], const double *U, int parity, int mu
         madd_to(nc, &staple, &link);
     }
     return staple;
+}
+
+static mat staple_at(int nc, const int X[4], const double *U, int parity, int mu, int id) {
+    return staple_range(nc, X, U, parity, mu, id, 4);
 }
 
 void or_staple(int nc, const int X[4], const double *U, int parity, int mu, int id, double *S) {
@@ -914,6 +920,221 @@ void or_polyakov(int nc, const int X[4], const double *U, double out[2]) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* ------------------------------------------------------------------------- */
+/* measurement chain: SU(N) projection, APE smearing, multihit, Wilson loops   */
+/* ------------------------------------------------------------------------- */
+/* Re Tr(a b^+) (matrixsun.h:934-950, realtraceUVdagger) */
+static double retrace_uvdag(int nc, const mat *a, const mat *b) {
+    double s = 0.0;
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) s += a->e[i][j].re * b->e[i][j].re + a->e[i][j].im * b->e[i][j].im;
+    return s;
+}
+
+/* projectlink.h:16-64 (project_link): SU(2)-subgroup hits maximising Re Tr(w q^+);
+ * with tol > 0 stop once the relative change of realtraceUVdagger(w, q^+) / N is
+ * at most tol (the reference measures with the already daggered q). */
+static void su_project(int nc, mat *w, const mat *q_in, int nhit, double tol) {
+    mat q = mdag(nc, q_in);
+    double conver = 1.0, old_tr = 0.0;
+    if (tol > 0) old_tr = retrace_uvdag(nc, w, &q) / nc;
+    int nblocks = nc * (nc - 1) / 2;
+    for (int hit = 0; hit < nhit && conver > tol; hit++) {
+        for (int block = 0; block < nblocks; block++) {
+            int x, y;
+            index_block(nc, block, &x, &y);
+            mat action = mmul(nc, w, &q);
+            double r[4];
+            get_block_su2(&action, x, y, r);
+            double k = q_abs(r);
+            if (k == 0.0) {
+                k = 1.0;
+                r[0] = 1.0;
+            }
+            k = 1.0 / k;
+            r[0] *= k;
+            r[1] *= -k;
+            r[2] *= -k;
+            r[3] *= -k;
+            mul_block_sun(nc, r, w, x, y);
+        }
+        if (tol > 0) {
+            double new_tr = retrace_uvdag(nc, w, &q) / nc;
+            conver = (new_tr - old_tr) / old_tr;
+            old_tr = new_tr;
+        }
+    }
+}
+
+void or_project_link(int nc, double *w, const double *q, int nhit, double tol) {
+    mat a, b;
+    mload(nc, w, &a);
+    mload(nc, q, &b);
+    su_project(nc, &a, &b, nhit, tol);
+    mstore(nc, &a, w);
+}
+
+/* SU3project (projectlink.h:119-139): w from the identity, the blocks of the
+ * product V = w F^+ projected by SU2project (projectlink.h:68-94) and w <- V_b^+ w. */
+void or_su3_project(double *F, int nhit) {
+    mat f, w;
+    mload(3, F, &f);
+    munit(3, &w);
+    su_project(3, &w, &f, nhit, 0.0);
+    mstore(3, &w, F);
+}
+
+/* ape.cu:41-94 (kernel_APE_Space), one direction: link = (U + w S) / (1 + 6w) with S
+ * the spatial staples, projected (N = 3: SU3project, else project_link from U). */
+static void ape_dir(int nc, const int X[4], const double *Uin, double *Uout, int mu, double w, int nhit,
+                    double tol) {
+    long V = volume_of(X), half = V / 2;
+#pragma omp parallel for schedule(static)
+    for (long idd = 0; idd < V; idd++) {
+        int parity = idd >= half ? 1 : 0;
+        int id = (int)(idd - parity * half);
+        mat S = staple_range(nc, X, Uin, parity, mu, id, 3);
+        mat U, link, res;
+        mload(nc, link_at(nc, Uin, V, mu, idd), &U);
+        double d = 1.0 + 6.0 * w;
+        for (int i = 0; i < nc; i++)
+            for (int j = 0; j < nc; j++)
+                link.e[i][j] = cmk((U.e[i][j].re + S.e[i][j].re * w) / d, (U.e[i][j].im + S.e[i][j].im * w) / d);
+        if (nc == 3) {
+            munit(3, &res);
+            su_project(3, &res, &link, nhit, 0.0);
+        } else {
+            res = U;
+            su_project(nc, &res, &link, nhit, tol);
+        }
+        mstore(nc, &res, link_at_w(nc, Uout, V, mu, idd));
+    }
+}
+
+/* ape.cu:198-264: ApplyAPEinSpace (mu = 0, 1, 2 per step, every direction from the
+ * field of the previous step) or ApplyAPEinTime (mu = 3) */
+int or_ape_smear(int nc, const int X[4], double *U, double w, int steps, int nhit, double tol, int time_dir) {
+    int rc = check_args(nc, X, OR_STORE_18);
+    if (rc) return rc;
+    long n = 4 * volume_of(X) * nc * nc * 2;
+    double *out = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!out) return -4;
+    memcpy(out, U, sizeof(double) * (size_t)n);
+    for (int st = 0; st < steps; st++) {
+        if (time_dir) {
+            ape_dir(nc, X, U, out, 3, w, nhit, tol);
+        } else {
+            for (int mu = 0; mu < 3; mu++) ape_dir(nc, X, U, out, mu, w, nhit, tol);
+        }
+        memcpy(U, out, sizeof(double) * (size_t)n);
+    }
+    free(out);
+    return 0;
+}
+
+/* multihit.cu:112-147 (kernel_evenodd): every temporal link replaced by the mean of
+ * itself and nhit successive heatbath updates in its (fixed) staple, all from the
+ * input field; spatial links copied.  Hit k of the link (parity, id) draws from the
+ * stream (seed, id, k, TAG_MH | parity<<4 | 3). */
+int or_multihit(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout) {
+    int rc = check_args(nc, X, OR_STORE_18);
+    if (rc) return rc;
+    long V = volume_of(X), half = V / 2;
+    long n = 4 * V * nc * nc * 2;
+    memcpy(Uout, Uin, sizeof(double) * (size_t)n);
+    double bnc = beta / (double)nc;
+#pragma omp parallel for schedule(static)
+    for (long idd = 0; idd < V; idd++) {
+        int parity = idd >= half ? 1 : 0;
+        int id = (int)(idd - parity * half);
+        mat S = staple_at(nc, X, Uin, parity, 3, id);
+        mat F = mdag(nc, &S);
+        mat u, acc;
+        mload(nc, link_at(nc, Uin, V, 3, idd), &u);
+        acc = u;
+        for (int hit = 0; hit < nhit; hit++) {
+            usrc st;
+            usrc_stream(&st, seed, (uint32_t)id, (uint32_t)hit, RNG_TAG_MH | ((uint32_t)parity << 4) | 3u);
+            heatbath_mat(nc, bnc, &u, &F, &st);
+            madd_to(nc, &acc, &u);
+        }
+        for (int i = 0; i < nc; i++)
+            for (int j = 0; j < nc; j++)
+                acc.e[i][j] = cmk(acc.e[i][j].re / (double)(nhit + 1), acc.e[i][j].im / (double)(nhit + 1));
+        mstore(nc, &acc, link_at_w(nc, Uout, V, 3, idd));
+    }
+    return 0;
+}
+
+/* wilsonloop.cu:44-121,494-516 (WilsonLineSP + WilsonLoopSP): W(r, T) = sum over x and
+ * mu < 3 of Tr(L_r(x) t1 L_r(x + T t)^+ t0^+) / (3 N V), L_r(x) the product of r links
+ * along mu from x, t0 (t1) the product of T temporal links from x (x + r mu).
+ * out[2 (T + r (tmax + 1)) + {0, 1}]. */
+int or_wilson_loops(int nc, const int X[4], const double *U, int rmax, int tmax, double *out) {
+    int rc = check_args(nc, X, OR_STORE_18);
+    if (rc) return rc;
+    long V = volume_of(X);
+    int nrt = (rmax + 1) * (tmax + 1);
+    long nchunk = (V + CHUNK - 1) / CHUNK;
+    double *part = (double *)calloc((size_t)nchunk * nrt * 2, sizeof(double));
+    if (!part) return -4;
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nchunk; c++) {
+        double *p = part + (size_t)c * nrt * 2;
+        long end = (c + 1) * CHUNK < V ? (c + 1) * CHUNK : V;
+        for (long s = c * CHUNK; s < end; s++) {
+            int parity = s >= V / 2 ? 1 : 0;
+            int x[4];
+            or_eo_to_coords((int)(s - parity * (V / 2)), parity, X, x);
+            for (int mu = 0; mu < 3; mu++) {
+                for (int r = 0; r <= rmax; r++) {
+                    mat t0, t1;
+                    munit(nc, &t0);
+                    munit(nc, &t1);
+                    for (int T = 0; T <= tmax; T++) {
+                        mat lb, lt, a, m;
+                        munit(nc, &lb);
+                        munit(nc, &lt);
+                        for (int k = 0; k < r; k++) {
+                            int y[4] = {x[0], x[1], x[2], x[3]};
+                            y[mu] = (y[mu] + k) % X[mu];
+                            mload(nc, link_at(nc, U, V, mu, or_coords_to_eo(y, X)), &a);
+                            lb = mmul(nc, &lb, &a);
+                            y[3] = (y[3] + T) % X[3];
+                            mload(nc, link_at(nc, U, V, mu, or_coords_to_eo(y, X)), &a);
+                            lt = mmul(nc, &lt, &a);
+                        }
+                        m = mmul(nc, &lb, &t1);
+                        a = mdag(nc, &lt);
+                        m = mmul(nc, &m, &a);
+                        a = mdag(nc, &t0);
+                        m = mmul(nc, &m, &a);
+                        cx tr = mtrace(nc, &m);
+                        p[2 * (T + r * (tmax + 1))] += tr.re;
+                        p[2 * (T + r * (tmax + 1)) + 1] += tr.im;
+                        if (T < tmax) {
+                            int y[4] = {x[0], x[1], x[2], (x[3] + T) % X[3]};
+                            mload(nc, link_at(nc, U, V, 3, or_coords_to_eo(y, X)), &a);
+                            t0 = mmul(nc, &t0, &a);
+                            y[mu] = (y[mu] + r) % X[mu];
+                            mload(nc, link_at(nc, U, V, 3, or_coords_to_eo(y, X)), &a);
+                            t1 = mmul(nc, &t1, &a);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    double norm = 3.0 * nc * (double)V;
+    for (int k = 0; k < 2 * nrt; k++) {
+        double acc = 0.0;
+        for (long c = 0; c < nchunk; c++) acc += part[(size_t)c * nrt * 2 + k];
+        out[k] = acc / norm;
+    }
+    free(part);
+    return 0;
+}
+
 void or_matmul(int nc, const double *A, const double *B, double *C) {
     mat a, b, c;
     mload(nc, A, &a);

@@ -91,6 +91,22 @@ void or_polyakov(int nc, const int dims[4], const double *U, double out[2]);
  * U holds nlinks = 4V links in the canonical layout */
 void or_link_monitor(int nc, long V, const double *U, double out[6]);
 
+/* ---- measurement chain (smear/ape.cu, smear/multihit.cu, meas/wilsonloop.cu) -- */
+/* projectlink.h:16-64: w <- SU(N) element maximising Re Tr(w q^+) by nhit rounds of
+ * SU(2)-subgroup hits from w (tol > 0: early stop, the reference's measure) */
+void or_project_link(int nc, double *w, const double *q, int nhit, double tol);
+/* projectlink.h:119-139: F <- its SU(3) projection (hits from the identity) */
+void or_su3_project(double *F, int nhit);
+/* ape.cu:198-264: steps of APE smearing, spatial (time_dir 0: mu = 0,1,2 with
+ * spatial staples) or temporal (time_dir 1: mu = 3), weight w; 18-real U in place */
+int or_ape_smear(int nc, const int dims[4], double *U, double w, int steps, int nhit, double tol, int time_dir);
+/* multihit.cu:112-147: Uout = Uin with each temporal link replaced by the mean of
+ * itself and nhit heatbath updates in its staple (stream (seed, id, hit, TAG_MH|p<<4|3)) */
+int or_multihit(int nc, const int dims[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout);
+/* wilsonloop.cu:44-121,494-516: out[2(T + r(tmax+1)) + {0,1}] = W(r, T), r <= rmax,
+ * T <= tmax, averaged over sites and the three spatial directions, / (3 N V) */
+int or_wilson_loops(int nc, const int dims[4], const double *U, int rmax, int tmax, double *out);
+
 /* ---- helpers ---------------------------------------------------------------- */
 void or_matmul(int nc, const double *A, const double *B, double *C);     /* C = A*B */
 void or_det(int nc, const double *A, double out[2]);

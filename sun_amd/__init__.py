@@ -44,6 +44,7 @@ EXPORTS = [
     "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
     "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
     "sun_link_monitor", "sun_link_monitor_async", "sun_link_trace", "sun_link_det",
+    "sun_ape_smear", "sun_multihit", "sun_wilson_loops",
     "sun_links_to_host", "sun_links_from_host",
     "sun_generate",
     "sun_batch_create", "sun_batch_destroy", "sun_batch_heatbath", "sun_batch_overrelax",
@@ -152,6 +153,9 @@ def lib() -> ctypes.CDLL:
         "sun_link_monitor_async": (ci, [vp, vp, vp]),
         "sun_link_trace": (ci, [vp, vp, dp]),
         "sun_link_det": (ci, [vp, vp, dp]),
+        "sun_ape_smear": (ci, [vp, cd, ci, ci, cd, ci, vp]),
+        "sun_multihit": (ci, [vp, vp, u64, ci, vp]),
+        "sun_wilson_loops": (ci, [vp, ci, ci, dp, vp]),
         "sun_links_to_host": (ci, [vp, dp, ci]),
         "sun_links_from_host": (ci, [vp, dp, ci]),
         "sun_generate": (ci, [vp, ctypes.POINTER(GenerateDesc), vp, dp]),
@@ -339,6 +343,20 @@ class Lattice:
         out = np.zeros(2)
         _check(lib().sun_link_det(self.h, stream, _dp(out)), "sun_link_det")
         return complex(out[0], out[1])
+
+    # -- measurement chain (smearing, multihit, Wilson loops)
+    def ape_smear(self, w: float, steps: int, nhit: int, tol: float = 0.0, time_dir: bool = False, stream=None):
+        _check(lib().sun_ape_smear(self.h, w, steps, nhit, tol, int(time_dir), stream), "sun_ape_smear")
+
+    def multihit(self, out: "Lattice", seed: int, nhit: int, stream=None):
+        """out (18-real, same N / extents / precision) <- this field with multihit temporal links"""
+        _check(lib().sun_multihit(self.h, out.h, seed, nhit, stream), "sun_multihit")
+
+    def wilson_loops(self, rmax: int, tmax: int, stream=None) -> np.ndarray:
+        """complex [rmax + 1, tmax + 1]: W(r, T)"""
+        out = np.zeros((rmax + 1) * (tmax + 1) * 2)
+        _check(lib().sun_wilson_loops(self.h, rmax, tmax, _dp(out), stream), "sun_wilson_loops")
+        return (out[0::2] + 1j * out[1::2]).reshape(rmax + 1, tmax + 1)
 
     # -- host exchange (canonical layout, complex128 [4, V, N, N])
     def to_host(self) -> np.ndarray:

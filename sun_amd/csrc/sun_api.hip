@@ -117,16 +117,17 @@ size_t sun::link_bytes(const sun_lattice_desc* d) {
 
 static size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 
-static void set_planes(sun_lattice* L) {
-    const size_t nl = 4 * (size_t)L->g.V;
-    const size_t lb = link_bytes(&L->d);
+Field sun::make_field(void* base, size_t nrec, size_t lb) {
+    Field f;
+    memset(&f, 0, sizeof(f));
     const int nfull = (int)(lb / 16);
     const int tail = (int)(lb % 16);
-    memset(&L->f, 0, sizeof(L->f));
-    char* base = (char*)L->buf;
-    for (int c = 0; c < nfull; c++) L->f.p[c] = base + (size_t)c * nl * 16;
-    if (tail) L->f.p[nfull] = base + (size_t)nfull * nl * 16;
+    for (int c = 0; c < nfull; c++) f.p[c] = (char*)base + (size_t)c * nrec * 16;
+    if (tail) f.p[nfull] = (char*)base + (size_t)nfull * nrec * 16;
+    return f;
 }
+
+static void set_planes(sun_lattice* L) { L->f = make_field(L->buf, 4 * (size_t)L->g.V, link_bytes(&L->d)); }
 
 extern "C" size_t sun_lattice_bytes(const sun_lattice_desc* d) {
     if (check_desc(d) != SUN_OK) return 0;

@@ -77,6 +77,19 @@ __global__ void __launch_bounds__(kReduceBlock) k_monitor_finalize(const double2
     }
 }
 
+__global__ void __launch_bounds__(kReduceBlock) k_wl_finalize(const double2* partials, int nb, int tmax, double scale,
+                                                              double2* out) {
+    const int T = blockIdx.x;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int mu = 0; mu < 3; mu++) {
+        const double2 v = ordered_sum(partials + (size_t)(mu * (tmax + 1) + T) * nb, nb);
+        __syncthreads();
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    if (threadIdx.x == 0) out[T] = make_double2(acc.x * scale, acc.y * scale);
+}
+
 __global__ void __launch_bounds__(kReduceBlock) k_batch_finalize(const BatchEntry* table, double2* out) {
     const BatchEntry& E = table[blockIdx.x];
     double2 v = ordered_sum(E.partials, E.nblocks);
@@ -135,6 +148,11 @@ hipError_t launch_finalize(const double2* partials, int n, double s0, double s1,
 hipError_t launch_monitor_finalize(const double2* partials, int n, double s_tr, double s_det, double2* out,
                                    hipStream_t s) {
     hipLaunchKernelGGL(k_monitor_finalize, dim3(1), dim3(kReduceBlock), 0, s, partials, n, s_tr, s_det, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_wl_finalize(const double2* partials, int nb, int tmax, double scale, double2* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_wl_finalize, dim3(tmax + 1), dim3(kReduceBlock), 0, s, partials, nb, tmax, scale, out);
     return hipGetLastError();
 }
 

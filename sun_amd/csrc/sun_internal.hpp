@@ -127,6 +127,9 @@ struct sun_batch {
 namespace sun {
 int check_desc(const sun_lattice_desc* d);
 size_t link_bytes(const sun_lattice_desc* d);
+// plane view of nrec link records of lb bytes at base (the layout of a lattice's
+// field: 16-byte records per plane, an 8-byte tail plane when lb % 16 == 8)
+Field make_field(void* base, size_t nrec, size_t lb);
 int lattice_alloc(sun_ctx* ctx, const sun_lattice_desc* d, sun_lattice** out);
 // bytes of a lattice placed in caller memory: link field, reduction scratch,
 // result slots, sweep counter and an n_hist-row observable history

@@ -1,0 +1,111 @@
+// sun_meas.hip -- the reference's measurement chain on a resident lattice (host
+// code): APE smearing (smear/ape.cu:198-264), multihit of temporal links
+// (smear/multihit.cu:291-333) and Wilson loops (meas/wilsonloop.cu:494-516).
+// Scratch (the ping-pong field of APE, the Wilson lines and partial sums) is
+// allocated per call: these are measurements between generation steps, not part
+// of the generation loop.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "sun_internal.hpp"
+
+using namespace sun;
+
+namespace {
+
+struct DeviceScratch {
+    void* p = nullptr;
+    ~DeviceScratch() {
+        if (p) hipFree(p);
+    }
+};
+
+int check_lattice(const sun_lattice* lat, const char* what) {
+    if (!lat) return fail(SUN_ERR_INVALID, "%s: NULL lattice", what);
+    if (lat->external) return fail(SUN_ERR_STATE, "%s: not available on a server job's lattice", what);
+    return SUN_OK;
+}
+
+}  // namespace
+
+extern "C" int sun_ape_smear(sun_lattice* lat, double w, int steps, int nhit, double tol, int time_dir,
+                             void* stream) {
+    SUN_TRY(check_lattice(lat, "sun_ape_smear"));
+    if (steps < 0 || nhit < 0 || !(w >= 0.0) || (time_dir != 0 && time_dir != 1))
+        return fail(SUN_ERR_INVALID, "sun_ape_smear: steps %d, nhit %d, w %g, time_dir %d", steps, nhit, w,
+                    time_dir);
+    if (steps == 0) return SUN_OK;
+    hipStream_t s = pick_stream(lat, stream);
+    HIP_TRY(hipSetDevice(lat->ctx->device));
+    DeviceScratch tmp;
+    HIP_TRY(hipMalloc(&tmp.p, lat->bytes));
+    // ping-pong between the lattice and a copy: every direction of a step reads the
+    // field of the previous step (ape.cu:209-214); the directions a step does not
+    // smear are equal in both buffers
+    HIP_TRY(hipMemcpyAsync(tmp.p, lat->buf, lat->bytes, hipMemcpyDeviceToDevice, s));
+    const Field other = make_field(tmp.p, 4 * (size_t)lat->g.V, link_bytes(&lat->d));
+    for (int st = 0; st < steps; st++) {
+        const Field& src = (st & 1) ? other : lat->f;
+        const Field& dst = (st & 1) ? lat->f : other;
+        for (int mu = time_dir ? 3 : 0; mu < (time_dir ? 4 : 3); mu++)
+            HIP_TRY(lat->ops->ape(src, dst, lat->g, mu, w, nhit, tol, s));
+    }
+    if (steps & 1) HIP_TRY(hipMemcpyAsync(lat->buf, tmp.p, lat->bytes, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SUN_OK;
+}
+
+extern "C" int sun_multihit(sun_lattice* in, sun_lattice* out, uint64_t seed, int nhit, void* stream) {
+    SUN_TRY(check_lattice(in, "sun_multihit"));
+    SUN_TRY(check_lattice(out, "sun_multihit"));
+    if (in == out) return fail(SUN_ERR_INVALID, "sun_multihit: the output must be another lattice");
+    if (nhit < 0) return fail(SUN_ERR_INVALID, "sun_multihit: nhit %d", nhit);
+    if (out->d.storage != 18)
+        return fail(SUN_ERR_INVALID, "sun_multihit: the output lattice must use 18-real storage (the averaged "
+                                     "links are not in SU(N))");
+    if (in->ctx != out->ctx || in->d.nc != out->d.nc || in->d.precision != out->d.precision)
+        return fail(SUN_ERR_INVALID, "sun_multihit: input and output differ in context, N or precision");
+    for (int i = 0; i < 4; i++)
+        if (in->d.dims[i] != out->d.dims[i]) return fail(SUN_ERR_INVALID, "sun_multihit: the extents differ");
+    hipStream_t s = pick_stream(in, stream);
+    HIP_TRY(hipSetDevice(in->ctx->device));
+    HIP_TRY(in->ops->multihit(in->f, out->f, in->g, seed, nhit, in->d.beta / (double)in->d.nc, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SUN_OK;
+}
+
+extern "C" int sun_wilson_loops(sun_lattice* lat, int rmax, int tmax, double* out, void* stream) {
+    SUN_TRY(check_lattice(lat, "sun_wilson_loops"));
+    if (!out) return fail(SUN_ERR_INVALID, "sun_wilson_loops: NULL output");
+    const Geom& g = lat->g;
+    if (rmax < 0 || tmax < 0 || rmax > g.X[0] || rmax > g.X[1] || rmax > g.X[2] || tmax > g.X[3])
+        return fail(SUN_ERR_INVALID, "sun_wilson_loops: rmax %d, tmax %d outside the lattice", rmax, tmax);
+    hipStream_t s = pick_stream(lat, stream);
+    HIP_TRY(hipSetDevice(lat->ctx->device));
+    const size_t full = (size_t)2 * lat->d.nc * lat->d.nc * (size_t)lat->d.precision;  // 18-real record
+    const size_t nline = 3 * (size_t)g.V;
+    const int nb = monitor_blocks(g);
+    const size_t npart = (size_t)3 * (tmax + 1) * nb;
+    const size_t nres = (size_t)(rmax + 1) * (tmax + 1);
+    DeviceScratch lines, part, res;
+    HIP_TRY(hipMalloc(&lines.p, nline * full));
+    HIP_TRY(hipMalloc(&part.p, npart * sizeof(double2)));
+    HIP_TRY(hipMalloc(&res.p, nres * sizeof(double2)));
+    const Field lf = make_field(lines.p, nline, full);
+    const double scale = 1.0 / (3.0 * lat->d.nc * (double)g.V);  // wilsonloop.cu:512
+    for (int r = 0; r <= rmax; r++) {
+        HIP_TRY(lat->ops->wl_lines(lat->f, lf, g, r, s));
+        HIP_TRY(lat->ops->wl_loops(lat->f, lf, g, r, tmax, (double2*)part.p, s));
+        HIP_TRY(launch_wl_finalize((const double2*)part.p, nb, tmax, scale, (double2*)res.p + (size_t)r * (tmax + 1),
+                                   s));
+    }
+    std::vector<double2> h(nres);
+    HIP_TRY(hipMemcpyAsync(h.data(), res.p, nres * sizeof(double2), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (size_t i = 0; i < nres; i++) {
+        out[2 * i] = h[i].x;
+        out[2 * i + 1] = h[i].y;
+    }
+    return SUN_OK;
+}

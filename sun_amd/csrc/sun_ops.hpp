@@ -109,6 +109,17 @@ struct OpsTable {
     hipError_t (*batch_polyakov_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     // link trace / determinant sums and unitarity deviations, 3 * monitor_blocks(geom) partials
     hipError_t (*monitor_partial)(const Field&, const Geom&, double2* partials, hipStream_t);
+    // measurement chain: one APE direction in -> out; multihit in -> out (18-real
+    // field of the same N and precision); Wilson lines of radius r (3V-entry 18-real
+    // field, r = 0 sets the identity) and the loops of radius r, tmax + 1 extents in
+    // time, 3 * (tmax + 1) * monitor_blocks(geom) partials
+    hipError_t (*ape)(const Field& in, const Field& out, const Geom&, int mu, double w, int nhit, double tol,
+                      hipStream_t);
+    hipError_t (*multihit)(const Field& in, const Field& out, const Geom&, uint64_t seed, int nhit, double bnc,
+                           hipStream_t);
+    hipError_t (*wl_lines)(const Field& f, const Field& lines, const Geom&, int r, hipStream_t);
+    hipError_t (*wl_loops)(const Field& f, const Field& lines, const Geom&, int r, int tmax, double2* partials,
+                           hipStream_t);
     int max_block;   // largest workgroup the update kernels are compiled for
     int has_piped;   // the variant has the software-pipelined overrelaxation kernel
     int piped_max_block;
@@ -156,6 +167,9 @@ inline int link_blocks(const Geom& g) { return (4 * g.V + kReduceBlock - 1) / kR
 // sum of partials[0..n) in a fixed order, times `scale`, into out (device)
 hipError_t launch_finalize(const double2* partials, int n, double scale0, double scale1, double2* out,
                            hipStream_t s);
+// Wilson loops of one radius: out[T] = scale * sum over mu < 3 and the nb partials
+// of (mu, T), T = 0..tmax, in a fixed order
+hipError_t launch_wl_finalize(const double2* partials, int nb, int tmax, double scale, double2* out, hipStream_t s);
 // link monitor: out[0] = trace sum * scale_tr, out[1] = determinant sum * scale_det,
 // out[2] = (max deviation, max |det - 1|) from the three segments of n partials
 hipError_t launch_monitor_finalize(const double2* partials, int n, double scale_tr, double scale_det, double2* out,

@@ -16,6 +16,7 @@ enum : uint32_t {
     kTagHeatbath = 0x10000000u,
     kTagHotStart = 0x20000000u,
     kTagTest = 0x30000000u,
+    kTagMultihit = 0x40000000u,
 };
 
 struct Philox {

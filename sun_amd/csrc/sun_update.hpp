@@ -167,6 +167,57 @@ SUN_HD void overrelax_link(Mat<R, N>& U, const Mat<R, N>& F) {
     }
 }
 
+// Projection onto SU(N) by SU(2)-subgroup maximisation of Re Tr(w q^+)
+// (projectlink.h:16-64, project_link): per hit, for every subgroup block, the block
+// of w q^+ is normalised, conjugated and multiplied into w from the left.  With
+// tol > 0 the hits stop once the relative change of Re Tr(w q) / N (the reference's
+// measure, realtraceUVdagger of w and the daggered q) falls to tol.  SU3project
+// (projectlink.h:119-139), the reference's N = 3 choice, is this with w = 1,
+// tol = 0 and the blocks (0,1), (1,2), (0,2); it has no zero-block guard (a block
+// of exactly zero norm would give NaN there and the identity block here).
+template <typename R, int N>
+SUN_HD R retrace_conj(const Mat<R, N>& a, const Mat<R, N>& b) {  // Re Tr(a b) with b stored daggered
+    R s = R(0);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) s += a.e[i][j].re * b.e[i][j].re + a.e[i][j].im * b.e[i][j].im;
+    return s;
+}
+
+template <typename R, int N>
+SUN_HD void project_link(Mat<R, N>& w, const Mat<R, N>& q_in, int nhit, R tol) {
+    const Mat<R, N> q = dagger(q_in);
+    R conver = R(1), old_tr = R(0);
+    if (tol > R(0)) old_tr = retrace_conj<R, N>(w, q) / R(N);
+#pragma unroll 1
+    for (int hit = 0; hit < nhit && conver > tol; hit++) {
+        // unrolled: compile-time rows keep w in registers
+#pragma unroll
+        for (int block = 0; block < N * (N - 1) / 2; block++) {
+            const int x = block_p(N, block), y = block_q(N, block);
+            const Mat<R, N> action = w * q;
+            Quat<R> r = get_block_su2<R, N>(action, x, y);
+            R k = qabs(r);
+            if (k == R(0)) {
+                k = R(1);
+                r.a[0] = R(1);
+            }
+            k = R(1) / k;
+            r.a[0] *= k;
+            r.a[1] *= -k;
+            r.a[2] *= -k;
+            r.a[3] *= -k;
+            mul_block<R, N>(r, w, x, y);
+        }
+        if (tol > R(0)) {
+            const R new_tr = retrace_conj<R, N>(w, q) / R(N);
+            conver = (new_tr - old_tr) / old_tr;
+            old_tr = new_tr;
+        }
+    }
+}
+
 // Reunitarisation (reunitlink.h:16-85).  N=3: row Gram-Schmidt + cross product;
 // N>3: column Gram-Schmidt + determinant phase on the last column.  N=2 is not
 // compilable in the reference (reunitlink.h:86); here the textbook projection

@@ -1,0 +1,116 @@
+"""The measurement chain on the GPU against the FP64 oracle: APE smearing
+(ape.cu), multihit (multihit.cu) and Wilson loops (wilsonloop.cu), every variant.
+
+Bars: FP64 1e-12 per link (smearing, multihit) and 1e-13 on W(r, T); FP32 5e-5 per
+link plus four times the storage round trip of the oracle's result (8-real FP32
+re-derives phases), 2e-6 on W(r, T).  FP32 multihit runs heatbath chains in FP32, so
+an accept/reject branch may flip against the oracle; as for heatbath sweeps, the
+count of temporal links beyond the bar is bounded (1%) instead.  The whole chain of
+test.cpp:170-230 (APE 25 steps, multihit, Wilson loops) runs in FP64 against the
+oracle's chain to 1e-10, and in FP32-12 against it to 2e-3.
+"""
+import numpy as np
+import pytest
+
+import sun_amd
+from test_gpu_parity import BETAS, IDS, VARIANTS, load_config, per_link_diff, store_tol, thermalised, tol
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (4, 6, 4, 8)
+
+
+def stored(ctx, nc, storage, prec, U):
+    """U as the lattice's storage keeps it (the oracle's result projected)"""
+    with sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], storage, prec) as tmp:
+        tmp.from_host(U)
+        return tmp.to_host()
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+@pytest.mark.parametrize("time_dir", [False, True], ids=["space", "time"])
+def test_ape_parity(ctx, oracle, nc, storage, prec, time_dir):
+    with sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], storage, prec) as lat:
+        U = load_config(lat, thermalised(oracle, nc, DIMS, storage))
+        lat.ape_smear(0.2, 2, 10, 1e-10, time_dir)
+        got = lat.to_host()
+    ref = oracle.ape_smear(U.copy(), DIMS, 0.2, 2, 10, 1e-10, time_dir)
+    d = per_link_diff(got, ref)
+    bar = tol(prec) + 4 * per_link_diff(stored(ctx, nc, storage, prec, ref), ref)
+    assert np.all(d <= bar), (float(d.max()), float(bar.max()))
+    untouched = slice(0, 3) if time_dir else slice(3, 4)
+    assert np.array_equal(got[untouched], U[untouched])
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+def test_multihit_parity(ctx, oracle, nc, storage, prec):
+    beta, seed, nhit = BETAS[nc], 71, 10
+    with sun_amd.Lattice(ctx, nc, DIMS, beta, storage, prec) as lat, \
+            sun_amd.Lattice(ctx, nc, DIMS, beta, 18, prec) as out:
+        U = load_config(lat, thermalised(oracle, nc, DIMS, storage))
+        lat.multihit(out, seed, nhit)
+        got = out.to_host()
+    ref = oracle.multihit(U, DIMS, beta, seed, nhit)
+    # spatial links copied (decoded in the link precision on both sides)
+    assert np.abs(got[:3] - U[:3]).max() <= (0.0 if prec == 8 else store_tol(storage, prec))
+    d = per_link_diff(got[3:], ref[3:])
+    if prec == 8:
+        assert d.max() <= 1e-12
+    else:
+        assert int((d > 5e-5).sum()) <= max(2, d.size // 100), float(d.max())
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+def test_wilson_loops_parity(ctx, oracle, nc, storage, prec):
+    rmax, tmax = 3, 5
+    with sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], storage, prec) as lat:
+        U = load_config(lat, thermalised(oracle, nc, DIMS, storage))
+        got = lat.wilson_loops(rmax, tmax)
+    ref = oracle.wilson_loops(U, DIMS, rmax, tmax)
+    assert np.abs(got - ref).max() <= (1e-13 if prec == 8 else 2e-6), np.abs(got - ref).max()
+    assert np.abs(got[:, 0] - 1).max() < (1e-13 if prec == 8 else 1e-6)
+
+
+def test_wilson_loops_odd_volume_and_extent_limits(ctx, oracle):
+    dims = (6, 6, 4, 10)  # V = 1440: not a multiple of the 256-site workgroup
+    with sun_amd.Lattice(ctx, 3, dims, 6.0, 18, 8) as lat:
+        lat.hot_start(4)
+        U = lat.to_host()
+        got = lat.wilson_loops(4, 10)
+        assert np.abs(got - oracle.wilson_loops(U, dims, 4, 10)).max() <= 1e-13
+        for bad in ((5, 2), (2, 11), (-1, 0)):
+            with pytest.raises(sun_amd.SunError) as e:
+                lat.wilson_loops(*bad)
+            assert e.value.code == sun_amd.ERR_INVALID
+
+
+def test_multihit_refuses_bad_outputs(ctx):
+    with sun_amd.Lattice(ctx, 3, DIMS, 6.0, 12, 4) as lat, sun_amd.Lattice(ctx, 3, DIMS, 6.0, 12, 4) as o12, \
+            sun_amd.Lattice(ctx, 3, DIMS, 6.0, 18, 8) as o64, sun_amd.Lattice(ctx, 3, (4, 6, 4, 6), 6.0, 18, 4) as od:
+        lat.cold_start()
+        for out in (lat, o12, o64, od):
+            with pytest.raises(sun_amd.SunError) as e:
+                lat.multihit(out, 1, 3)
+            assert e.value.code == sun_amd.ERR_INVALID
+        with pytest.raises(sun_amd.SunError):
+            lat.ape_smear(0.2, -1, 10)
+
+
+@pytest.mark.parametrize("storage,prec,bar", [(18, 8, 1e-10), (12, 4, 2e-3)], ids=["f64_18", "f32_12"])
+def test_static_potential_chain(ctx, oracle, storage, prec, bar):
+    """test.cpp:170-230: APE in space (w 0.2, 25 steps, 10 hits), multihit, Wilson
+    loops -- GPU chain against the oracle's FP64 chain"""
+    dims, beta, seed = (8, 8, 8, 8), 6.0, 5
+    rmax, tmax = 4, 4
+    U0 = thermalised(oracle, 3, dims, storage)
+    with sun_amd.Lattice(ctx, 3, dims, beta, storage, prec) as lat, \
+            sun_amd.Lattice(ctx, 3, dims, beta, 18, prec) as mh:
+        U = load_config(lat, U0)
+        lat.ape_smear(0.2, 25, 10, 1e-10)
+        lat.multihit(mh, seed, 20)
+        got = mh.wilson_loops(rmax, tmax)
+    ref_s = oracle.ape_smear(U.copy(), dims, 0.2, 25, 10, 1e-10)
+    ref = oracle.wilson_loops(oracle.multihit(ref_s, dims, beta, seed, 20), dims, rmax, tmax)
+    assert np.abs(got - ref).max() <= bar, np.abs(got - ref).max()
+    # smearing and multihit make the large loops measurable: W(4, 4) well above noise
+    assert got[4, 4].real > 1e-3

@@ -1,0 +1,281 @@
+"""Pinning of the oracle's measurement chain: SU(N) projection (projectlink.h),
+APE smearing (ape.cu), multihit (multihit.cu) and Wilson loops (wilsonloop.cu).
+
+* projection: F = c W (W in SU(3), c > 0) projects to W; for random F the result is
+  in SU(3) and a local maximum of Re Tr(U F^+) (no small rotation increases it);
+  for SU(2) one hit gives the closed-form maximiser, the normalised quaternion part;
+* APE: an independent numpy implementation for SU(2) (lexicographic field, np.roll
+  staples, closed-form SU(2) projection) agrees link by link; SU(2) smearing is
+  exactly gauge covariant; the identity field is a fixed point; smearing raises the
+  spatial plaquette;
+* multihit: nhit = 0 is the identity; nhit = 1 is (U + HB(U)) / 2 with the pinned
+  heatbath link update on the same stream; for SU(2) the mean over hits approaches
+  the exact conditional mean I2(a)/I1(a) V^+ (a = beta/2 |F|), within its error;
+* Wilson loops: the identity field gives 1; W(r, 0) = W(0, T) = 1 on SU(N) fields;
+  W(1, 1) is the temporal plaquette (pinned separately); W is gauge invariant; an
+  abelian field with temporal phases exp(i phi_k x0) gives the Stokes closed form
+  (mean_k exp(i phi_k r T) + 2) / 3.
+"""
+import numpy as np
+import pytest
+from scipy import special
+
+NCS = [2, 3, 4]
+
+
+def haar(rng, n, shape=()):
+    Z = (rng.standard_normal(shape + (n, n)) + 1j * rng.standard_normal(shape + (n, n))) / np.sqrt(2)
+    Q, Rm = np.linalg.qr(Z)
+    d = np.diagonal(Rm, axis1=-2, axis2=-1)
+    Q = Q * (d / np.abs(d))[..., None, :]
+    det = np.linalg.det(Q)
+    return Q / (det ** (1.0 / n))[..., None, None]
+
+
+def retr(U, F):
+    return np.real(np.trace(U @ F.conj().T))
+
+
+def test_su3_project_fixed_point(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(10):
+        W = haar(rng, 3)
+        got = oracle.su3_project(2.7 * W, 200)
+        assert np.abs(got - W).max() < 1e-10
+
+
+def test_su3_project_local_maximum(oracle):
+    rng = np.random.default_rng(2)
+    for _ in range(10):
+        F = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+        U = oracle.su3_project(F, 300)
+        assert np.abs(U @ U.conj().T - np.eye(3)).max() < 1e-13
+        assert abs(np.linalg.det(U) - 1) < 1e-13
+        f0 = retr(U, F)
+        for _ in range(20):
+            H = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+            H = (H + H.conj().T) / 2
+            H -= np.trace(H) / 3 * np.eye(3)
+            w, v = np.linalg.eigh(H)
+            E = (v * np.exp(1e-3j * w)) @ v.conj().T
+            assert retr(E @ U, F) <= f0 + 1e-12
+        for _ in range(50):
+            assert retr(haar(rng, 3), F) <= f0 + 1e-12
+
+
+def quat_project_su2(F):
+    """maximiser of Re Tr(U F^+) over SU(2): the normalised quaternion part of F"""
+    a0 = (F[0, 0].real + F[1, 1].real) / 2
+    a3 = (F[0, 0].imag - F[1, 1].imag) / 2
+    a2 = (F[0, 1].real - F[1, 0].real) / 2
+    a1 = (F[0, 1].imag + F[1, 0].imag) / 2
+    n = np.sqrt(a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3)
+    a0, a1, a2, a3 = a0 / n, a1 / n, a2 / n, a3 / n
+    return np.array([[a0 + 1j * a3, a2 + 1j * a1], [-a2 + 1j * a1, a0 - 1j * a3]])
+
+
+def test_project_link_su2_closed_form(oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        F = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+        w = haar(rng, 2)
+        assert np.abs(oracle.project_link(w, F, 1) - quat_project_su2(F)).max() < 1e-14
+
+
+@pytest.mark.parametrize("nc", [2, 4])
+def test_project_link_tolerance_stops_early(oracle, nc):
+    """with tol > 0 the hits stop after the first one whose relative change of
+    Re Tr(w F) / N (projectlink.h:22,57: realtraceUVdagger of w and the daggered F)
+    is at most tol"""
+    rng = np.random.default_rng(4)
+    for tol in (1e-2, 1e-4, 1e-8):
+        F = rng.standard_normal((nc, nc)) + 1j * rng.standard_normal((nc, nc))
+        w = haar(rng, nc)
+        prev, m_prev = w, np.real(np.trace(w @ F)) / nc
+        for k in range(1, 60):
+            wk = oracle.project_link(w, F, k, 0.0)
+            m = np.real(np.trace(wk @ F)) / nc
+            conv = (m - m_prev) / m_prev
+            prev, m_prev = wk, m
+            if conv <= tol:
+                break
+        assert np.array_equal(oracle.project_link(w, F, 60, tol), prev), (tol, k)
+
+
+# ---- APE: an independent numpy restatement for SU(2) on a lexicographic field ----
+def to_lex(U, dims):
+    V = int(np.prod(dims))
+    X0, X1, X2, X3 = dims
+    L = np.empty((4, X3, X2, X1, X0, U.shape[-1], U.shape[-1]), dtype=complex)
+    for lex in range(V):
+        x0, x1, x2, x3 = lex % X0, (lex // X0) % X1, (lex // (X0 * X1)) % X2, lex // (X0 * X1 * X2)
+        eo = lex // 2 + ((x0 + x1 + x2 + x3) & 1) * (V // 2)
+        L[:, x3, x2, x1, x0] = U[:, eo]
+    return L
+
+
+def from_lex(L, dims):
+    V = int(np.prod(dims))
+    X0, X1, X2, X3 = dims
+    U = np.empty((4, V) + L.shape[-2:], dtype=complex)
+    for lex in range(V):
+        x0, x1, x2, x3 = lex % X0, (lex // X0) % X1, (lex // (X0 * X1)) % X2, lex // (X0 * X1 * X2)
+        eo = lex // 2 + ((x0 + x1 + x2 + x3) & 1) * (V // 2)
+        U[:, eo] = L[:, x3, x2, x1, x0]
+    return U
+
+
+AX = {0: 3, 1: 2, 2: 1, 3: 0}  # direction -> array axis of [x3, x2, x1, x0]
+
+
+def shift(A, mu, d):
+    """A(x + d e_mu)"""
+    return np.roll(A, -d, axis=AX[mu])
+
+
+def dag(A):
+    return np.conj(np.swapaxes(A, -1, -2))
+
+
+def ape_numpy_su2(L, w, steps):
+    for _ in range(steps):
+        out = L.copy()
+        for mu in range(3):
+            S = np.zeros_like(L[mu])
+            for nu in range(3):
+                if nu == mu:
+                    continue
+                S += L[nu] @ shift(L[mu], nu, 1) @ dag(shift(L[nu], mu, 1))
+                Ld = shift(L[nu], nu, -1)
+                S += dag(Ld) @ shift(L[mu], nu, -1) @ shift(shift(L[nu], nu, -1), mu, 1)
+            M = (L[mu] + w * S) / (1 + 6 * w)
+            P = np.empty_like(M)
+            for idx in np.ndindex(M.shape[:-2]):
+                P[idx] = quat_project_su2(M[idx])
+            out[mu] = P
+        L = out
+    return L
+
+
+def test_ape_su2_matches_numpy(oracle):
+    dims = (4, 4, 4, 6)
+    U = oracle.hot_start(2, dims, 5)
+    ref = from_lex(ape_numpy_su2(to_lex(U, dims), 0.3, 2), dims)
+    got = oracle.ape_smear(U.copy(), dims, 0.3, 2, 1)
+    assert np.abs(got - ref).max() < 1e-13
+
+
+def gauge_transform(U, dims, G):
+    """U_mu(x) -> G(x) U_mu(x) G(x + mu)^+ on the even/odd field"""
+    L = to_lex(U, dims)
+    Gl = to_lex(np.broadcast_to(G, (4,) + G.shape).copy(), dims)[0]
+    out = np.empty_like(L)
+    for mu in range(4):
+        out[mu] = Gl @ L[mu] @ dag(shift(Gl, mu, 1))
+    return from_lex(out, dims)
+
+
+def test_ape_su2_gauge_covariant(oracle):
+    dims = (4, 4, 4, 4)
+    rng = np.random.default_rng(6)
+    U = oracle.hot_start(2, dims, 7)
+    G = haar(rng, 2, (256,))
+    a = oracle.ape_smear(gauge_transform(U, dims, G), dims, 0.2, 3, 1)
+    b = gauge_transform(oracle.ape_smear(U.copy(), dims, 0.2, 3, 1), dims, G)
+    assert np.abs(a - b).max() < 1e-12
+
+
+@pytest.mark.parametrize("nc", NCS)
+def test_ape_identity_fixed_and_smooths(oracle, nc):
+    dims = (4, 4, 4, 4)
+    U = oracle.cold_start(nc, dims)
+    assert np.abs(oracle.ape_smear(U.copy(), dims, 0.2, 3, 10) - U).max() < 1e-14
+    H = oracle.hot_start(nc, dims, 8)
+    for it in range(4):  # a rough but not random field
+        oracle.heatbath_sweep(H, dims, {2: 2.3, 3: 6.0, 4: 11.0}[nc], 9, it)
+    p0 = oracle.plaquette(H, dims)
+    S = oracle.ape_smear(H.copy(), dims, 0.2, 5, 10, 1e-10)
+    p1 = oracle.plaquette(S, dims)
+    assert p1[0] > p0[0] + 0.05
+    assert np.abs(S[3] - H[3]).max() == 0.0  # time links untouched by spatial smearing
+    T = oracle.ape_smear(H.copy(), dims, 0.2, 1, 10, 1e-10, time_dir=True)
+    assert np.abs(T[:3] - H[:3]).max() == 0.0 and np.abs(T[3] - H[3]).max() > 0.01
+
+
+# ---- multihit ----
+def test_multihit_zero_and_one_hit(oracle):
+    dims, beta, seed = (4, 4, 4, 4), 6.0, 21
+    U = oracle.hot_start(3, dims, 3)
+    assert np.array_equal(oracle.multihit(U, dims, beta, seed, 0), U)
+    M = oracle.multihit(U, dims, beta, seed, 1)
+    assert np.array_equal(M[:3], U[:3])
+    V, half = 256, 128
+    for idd in (0, 77, 130, 255):
+        parity, sid = idd // half, idd % half
+        S = oracle.staple(U, dims, parity, 3, sid)
+        hb = oracle.heatbath_link(U[3, idd].copy(), S.conj().T, beta, seed, c0=sid, c2=0,
+                                  c3=0x40000000 | (parity << 4) | 3)
+        assert np.abs(M[3, idd] - (U[3, idd] + hb) / 2).max() < 1e-15
+
+
+def test_multihit_su2_conditional_mean(oracle):
+    dims, beta, nhit = (4, 4, 4, 4), 2.3, 800
+    U = oracle.hot_start(2, dims, 4)
+    M = oracle.multihit(U, dims, beta, 33, nhit)
+    dev = []
+    for idd in range(256):
+        parity, sid = idd // 128, idd % 128
+        F = oracle.staple(U, dims, parity, 3, sid).conj().T  # F = staple^+
+        k = np.sqrt(abs(np.linalg.det(F)))  # SU(2) staples are k x SU(2)
+        Vm = F / k
+        a = beta * k  # weight exp(beta/2 Re Tr(U F)) = exp(beta k x0), U = X Vm^+
+        mean = special.iv(2, a) / special.iv(1, a) * Vm.conj().T
+        # the output also carries the start link with weight 1 / (nhit + 1)
+        expect = (U[3, idd] + nhit * mean) / (nhit + 1)
+        dev.append((M[3, idd] - expect).ravel())
+    dev = np.array(dev)
+    # per component the mean over hits has std <= 1 / sqrt(nhit); over 256 links
+    assert np.abs(dev.mean(axis=0)).max() < 5 / np.sqrt(nhit * 256)
+    assert np.sqrt((np.abs(dev) ** 2).mean()) < 2 / np.sqrt(nhit)
+
+
+# ---- Wilson loops ----
+@pytest.mark.parametrize("nc", NCS)
+def test_wilson_identity_and_trivial_extents(oracle, nc):
+    dims = (4, 4, 6, 6)
+    W = oracle.wilson_loops(oracle.cold_start(nc, dims), dims, 3, 4)
+    assert np.abs(W - 1).max() < 1e-15
+    H = oracle.hot_start(nc, dims, 12)
+    W = oracle.wilson_loops(H, dims, 3, 4)
+    assert np.abs(W[:, 0] - 1).max() < 1e-13 and np.abs(W[0, :] - 1).max() < 1e-13
+    assert W[1, 1].real == pytest.approx(oracle.plaquette(H, dims)[1], abs=1e-14)
+
+
+def test_wilson_gauge_invariant(oracle):
+    dims = (4, 4, 4, 6)
+    rng = np.random.default_rng(13)
+    U = oracle.hot_start(3, dims, 14)
+    for it in range(3):
+        oracle.heatbath_sweep(U, dims, 6.0, 15, it)
+    G = haar(rng, 3, (384,))
+    a = oracle.wilson_loops(U, dims, 3, 4)
+    b = oracle.wilson_loops(gauge_transform(U, dims, G), dims, 3, 4)
+    assert np.abs(a - b).max() < 1e-13
+    assert np.abs(a[2:, 2:]).max() < 0.5  # not trivially 1
+
+
+def test_wilson_abelian_stokes(oracle):
+    dims = (8, 4, 4, 6)
+    n = np.array([1, 2, -3])
+    phi = 2 * np.pi * n / dims[0]
+    U = oracle.cold_start(3, dims)
+    V = int(np.prod(dims))
+    for s in range(V):
+        parity = s // (V // 2)
+        x = oracle.eo_to_coords(s % (V // 2), parity, dims)
+        U[3, s] = np.diag(np.exp(1j * phi * x[0]))
+    W = oracle.wilson_loops(U, dims, 4, 5)
+    r = np.arange(5)[:, None]
+    T = np.arange(6)[None, :]
+    expect = (np.exp(1j * phi[None, None, :] * (r * T)[..., None]).mean(axis=-1) + 2) / 3
+    assert np.abs(W - expect).max() < 1e-14
