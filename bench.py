@@ -267,6 +267,43 @@ def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
                       "12-real), FP64 reductions, hot/cold 50/50, 64 GiB budget"}
 
 
+def measurement_leg(sun_amd, ctx, lat, stream):
+    """The reference's static-potential chain (test.cpp:170-230) on the generated 32^4
+    FP32-12 field: link monitor (before and after a reunitarisation), APE smearing in
+    space (w 0.2, 25 steps, 10 hits),
+    multihit (100 hits) into an 18-real FP32 lattice and Wilson loops r, T <= 16."""
+    V = int(np.prod(DIMS))
+    reps = 20
+    # the launch timings before this leg ran sweeps without reunitarisation
+    before = lat.link_monitor(stream)
+    lat.reunitarize(stream)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mon = lat.link_monitor(stream)
+    mon_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    lat.ape_smear(0.2, 25, 10, 1e-10, stream=stream)
+    ape_s = time.perf_counter() - t0
+    with sun_amd.Lattice(ctx, 3, DIMS, BETA, 18, 4) as mh:
+        t0 = time.perf_counter()
+        lat.multihit(mh, 7, 100, stream=stream)
+        mh_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        W = mh.wilson_loops(16, 16, stream=stream)
+        wl_s = time.perf_counter() - t0
+    return {
+        "link_monitor": {"ms": mon_s * 1e3, "read_gbs": 4 * V * link_bytes(3, 12, 4) / mon_s / 1e9,
+                         "max_unitarity_dev": float(mon[4]), "mean_det": [float(mon[2]), float(mon[3])],
+                         "max_unitarity_dev_before_reunit": float(before[4]),
+                         "note": "synchronous call (kernel + finalize + 48-byte copy)"},
+        "ape_space_25_steps_s": ape_s,
+        "multihit_100_hits_s": mh_s,
+        "wilson_loops_r16_t16_s": wl_s,
+        "W_r_T4": [float(W[r, 4].real) for r in range(1, 9)],
+        "W_r4_T": [float(W[4, t].real) for t in range(1, 9)],
+    }
+
+
 def kernels_only(sun_amd, large=False):
     """For rocprofv3 counter passes.  Default: on the 32^4 FP32 12-real field after
     100 iterations, 2 copies of the field (calibration: a plain copy of known size),
@@ -316,6 +353,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-lattice leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 64^4 FP32 8-real leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (Poisson arrivals) leg")
+    ap.add_argument("--no-meas", action="store_true", help="skip the smearing / Wilson loop leg")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served leg")
     ap.add_argument("--large", action="store_true", help="with --kernels-only: the 64^4 FP32 8-real field")
     ap.add_argument("--kernels-only", action="store_true",
@@ -390,6 +428,8 @@ def main():
                                                                                         float(obs[-1, 3])]},
         "device": ctx.device_name(),
     }
+    if not args.no_meas:
+        out["measurement_chain"] = measurement_leg(sun_amd, ctx, lat, stream)
     lat.close()
 
     if not args.no_plain:

@@ -183,26 +183,43 @@ def test_errors_are_reported_not_fatal(ctx):
 
 def test_cpp_facade_program_matches_c_abi(ctx, tmp_path):
     """A CULQCD-style C++ program (tests/host/facade_demo.cpp against include/sun.hpp)
-    produces the same bits as the same run through the C ABI from Python."""
+    produces the same bits as the same run through the C ABI from Python: the
+    generation, the link monitors and the measurement chain on normal-order arrays
+    (whose host copy is lexicographic)."""
     import os
     import subprocess
     from sun_amd.build import FACADE_DEMO
     assert os.path.exists(FACADE_DEMO), "facade demo not built: run __graft_entry__.build()"
-    out = tmp_path / "links.bin"
+    out, lex = tmp_path / "links.bin", tmp_path / "lex.bin"
     dims, seed, iters = (8, 8, 8, 8), 1, 3
-    r = subprocess.run([FACADE_DEMO, *map(str, dims), str(seed), str(iters), str(out)], capture_output=True,
-                       text=True, timeout=100)
+    r = subprocess.run([FACADE_DEMO, *map(str, dims), str(seed), str(iters), str(out), str(lex)],
+                       capture_output=True, text=True, timeout=100)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.strip().splitlines()
-    assert lines[-1].startswith("monitor ")
-    monitor = [float(v) for v in lines[-1].split()[1:]]
-    rows = np.array([[float(v) for v in line.split()[1:]] for line in lines[:-1]])
+    mon = [ln for ln in lines if ln.startswith("monitor ")]
+    wl = [ln for ln in lines if ln.startswith("wloop ")]
+    rows = np.array([[float(v) for v in ln.split()[1:]] for ln in lines if ln[0].isdigit()])
     links = np.fromfile(out, dtype=np.float64).view(np.complex128).reshape(4, -1, 3, 3)
-    with sun_amd.Lattice(ctx, 3, dims, 6.0, 18, 8) as lat:
+    lexf = np.fromfile(lex, dtype=np.float64).view(np.complex128).reshape(4, -1, 3, 3)
+    with sun_amd.Lattice(ctx, 3, dims, 6.0, 18, 8) as lat, sun_amd.Lattice(ctx, 3, dims, 6.0, 18, 8) as mh:
         lat.hot_start(seed)
         obs = lat.generate(seed=seed, n_iter=iters, n_or=4, reunit_every=0, use_graph=False)
         G = lat.to_host()
         m = lat.link_monitor()
+        lat.ape_smear(0.2, 3, 10, 1e-10)
+        lat.multihit(mh, seed + 1, 5)
+        W = mh.wilson_loops(2, 3)
+        H = mh.to_host()
     assert np.array_equal(rows, obs)
-    assert monitor == list(m[:4])
     assert np.array_equal(links, G)
+    assert len(mon) == 1 and [float(v) for v in mon[0].split()[1:]] == list(m[:4])
+    assert len(wl) == 12
+    for ln in wl:
+        _, rr, tt, re, im = ln.split()
+        assert complex(float(re), float(im)) == W[int(rr), int(tt)]
+    # lexicographic host order of the normal-order array
+    V = 8 ** 4
+    for lexi in (0, 1, 9, 77, 4095):
+        x = [lexi % 8, (lexi // 8) % 8, (lexi // 64) % 8, lexi // 512]
+        eo = lexi // 2 + (sum(x) & 1) * (V // 2)
+        assert np.array_equal(lexf[:, lexi], H[:, eo])
