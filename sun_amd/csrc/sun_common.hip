@@ -77,6 +77,23 @@ __global__ void __launch_bounds__(kReduceBlock) k_monitor_finalize(const double2
     }
 }
 
+__global__ void __launch_bounds__(kReduceBlock) k_finalize_iteration(const double2* partials, int pb, int qb,
+                                                                     double s_plaq, double s_poly, double2* out,
+                                                                     double2* history, uint32_t* ctr, uint32_t first) {
+    const double2 p = ordered_sum(partials, pb);
+    __syncthreads();
+    const double2 q = ordered_sum(partials + pb, qb);
+    if (threadIdx.x == 0) {
+        const double2 a = make_double2(p.x * s_plaq, p.y * s_plaq), b = make_double2(q.x * s_poly, q.y * s_poly);
+        out[0] = a;
+        out[1] = b;
+        const size_t row = (size_t)(ctr[0] - first);
+        history[2 * row] = a;
+        history[2 * row + 1] = b;
+        ctr[0] += 1u;
+    }
+}
+
 __global__ void __launch_bounds__(kReduceBlock) k_wl_finalize(const double2* partials, int nb, int tmax, double scale,
                                                               double2* out) {
     const int T = blockIdx.x;
@@ -148,6 +165,13 @@ hipError_t launch_finalize(const double2* partials, int n, double s0, double s1,
 hipError_t launch_monitor_finalize(const double2* partials, int n, double s_tr, double s_det, double2* out,
                                    hipStream_t s) {
     hipLaunchKernelGGL(k_monitor_finalize, dim3(1), dim3(kReduceBlock), 0, s, partials, n, s_tr, s_det, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_iteration(const double2* partials, int pb, int qb, double s_plaq, double s_poly,
+                                     double2* out, double2* history, uint32_t* ctr, uint32_t first, hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize_iteration, dim3(1), dim3(kReduceBlock), 0, s, partials, pb, qb, s_plaq, s_poly, out,
+                       history, ctr, first);
     return hipGetLastError();
 }
 

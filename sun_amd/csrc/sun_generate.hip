@@ -42,11 +42,18 @@ int lattice_iteration(sun_lattice* lat, uint64_t seed, int n_or, bool reunit, bo
     for (int k = 0; k < n_or; k++) SUN_TRY(overrelax_sweep(lat, s));
     if (reunit) HIP_TRY(lat->ops->reunitarize(lat->f, 4 * lat->g.V, s));
     if (measure) {
-        SUN_TRY(plaquette_to(lat, s, lat->dres));
-        SUN_TRY(polyakov_to(lat, s, lat->dres + 1));
-        HIP_TRY(launch_record_obs(lat->dres, 2, lat->obs_dev, lat->sweep_dev, first, s));
+        // plaquette and Polyakov loop in one grid, then one workgroup that finalizes
+        // both (the sums of sun_plaquette / sun_polyakov, bit for bit), records them
+        // and advances the sweep counter
+        const double ts = (double)lat->g.X[0] * lat->g.X[1] * lat->g.X[2];
+        HIP_TRY(lat->ops->measure_partial(lat->f, lat->g, lat->partials, s));
+        HIP_TRY(launch_finalize_iteration(lat->partials, plaquette_blocks(lat->g), polyakov_blocks(lat->g),
+                                          1.0 / ((double)(3 * lat->d.nc) * (double)lat->g.V),
+                                          1.0 / ((double)lat->d.nc * ts), lat->dres, lat->obs_dev, lat->sweep_dev,
+                                          first, s));
+    } else {
+        HIP_TRY(launch_advance_counter(lat->sweep_dev, s));
     }
-    HIP_TRY(launch_advance_counter(lat->sweep_dev, s));
     return SUN_OK;
 }
 

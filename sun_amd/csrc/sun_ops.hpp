@@ -109,6 +109,9 @@ struct OpsTable {
     hipError_t (*batch_polyakov_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     // link trace / determinant sums and unitarity deviations, 3 * monitor_blocks(geom) partials
     hipError_t (*monitor_partial)(const Field&, const Geom&, double2* partials, hipStream_t);
+    // plaquette partials at [0, plaquette_blocks) and Polyakov partials after them,
+    // in one launch
+    hipError_t (*measure_partial)(const Field&, const Geom&, double2* partials, hipStream_t);
     // measurement chain: one APE direction in -> out; multihit in -> out (18-real
     // field of the same N and precision); Wilson lines of radius r (3V-entry 18-real
     // field, r = 0 sets the identity) and the loops of radius r, tmax + 1 extents in
@@ -158,8 +161,8 @@ SUN_HD double nan_max(double a, double b) {
 // a lattice needs: the monitor writes three segments of monitor_blocks entries
 inline int monitor_blocks(const Geom& g) { return (g.V + kReduceBlock - 1) / kReduceBlock; }
 inline int partial_slots(const Geom& g) {
-    const int a = 3 * monitor_blocks(g), b = plaquette_blocks(g), c = polyakov_blocks(g);
-    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+    const int a = 3 * monitor_blocks(g), b = plaquette_blocks(g) + polyakov_blocks(g);
+    return a > b ? a : b;
 }
 inline int update_blocks(const Geom& g, int block) { return (g.half + block - 1) / block; }
 inline int link_blocks(const Geom& g) { return (4 * g.V + kReduceBlock - 1) / kReduceBlock; }
@@ -167,6 +170,11 @@ inline int link_blocks(const Geom& g) { return (4 * g.V + kReduceBlock - 1) / kR
 // sum of partials[0..n) in a fixed order, times `scale`, into out (device)
 hipError_t launch_finalize(const double2* partials, int n, double scale0, double scale1, double2* out,
                            hipStream_t s);
+// end of a measured generation iteration, one workgroup: plaquette (partials
+// [0, pb)) and Polyakov loop (partials [pb, pb + qb)) summed as launch_finalize sums,
+// scaled into out[0], out[1], recorded in history row ctr - first, then ctr += 1
+hipError_t launch_finalize_iteration(const double2* partials, int pb, int qb, double s_plaq, double s_poly,
+                                     double2* out, double2* history, uint32_t* ctr, uint32_t first, hipStream_t s);
 // Wilson loops of one radius: out[T] = scale * sum over mu < 3 and the nb partials
 // of (mu, T), T = 0..tmax, in a fixed order
 hipError_t launch_wl_finalize(const double2* partials, int nb, int tmax, double scale, double2* out, hipStream_t s);
