@@ -522,9 +522,10 @@ SUN_HD void heatbath_su3(F3& U, const F3& S, double bnc, Stream& st) {
     for (int block = 0; block < 3; block++) {
         const int p = block_p(3, block), q = block_q(3, block);
         Quat<float> r = block_proj(U, S, p, q);
-        float k = qabs(r);
-        float ap = (float)bnc * k;
-        k = 1.f / k;
+        // |r| and 1/|r| from one reciprocal square root
+        const float n2 = r.a[0] * r.a[0] + r.a[1] * r.a[1] + r.a[2] * r.a[2] + r.a[3] * r.a[3];
+        const float k = f32_rsq(n2);
+        const float ap = (float)bnc * (n2 * k);
 #pragma unroll
         for (int i = 0; i < 4; i++) r.a[i] *= k;
         Quat<float> a;

@@ -111,9 +111,10 @@ SUN_HD void sincos2pi(float u, float& s, float& c) {
 
 // Transcendentals of the FP32 sampler.  On the device the native gfx950
 // instructions: v_log_f32 (log2), v_cos_f32 / v_sin_f32 (of 2 pi x: the argument is
-// in revolutions), v_sqrt_f32, v_rcp_f32.  Measured against libm over the uniforms
+// in revolutions), v_sqrt_f32, v_rsq_f32, v_rcp_f32.  Measured against libm over the uniforms
 // (0, 1]: |cos, sin error| <= 1.3e-7, log2 1.7e-6 absolute at 2^-32 (5e-8 relative),
-// sqrt and rcp <= 1e-7 relative -- the FP32 rounding level of the link they update.
+// sqrt, rsq and rcp <= 1e-7 relative -- the FP32 rounding level of the link they
+// update.
 SUN_HD float f32_log(float x) {
 #ifdef __HIP_DEVICE_COMPILE__
     return __builtin_amdgcn_logf(x) * 0.693147180559945f;
@@ -142,6 +143,13 @@ SUN_HD float f32_sqrt(float x) {
     return __builtin_amdgcn_sqrtf(x);
 #else
     return sqrtf(x);
+#endif
+}
+SUN_HD float f32_rsq(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.f / sqrtf(x);
 #endif
 }
 SUN_HD float f32_rcp(float x) {
