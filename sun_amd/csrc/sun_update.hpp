@@ -3,8 +3,12 @@
 // heatbath_link restates heatBathSUN (device_PHB_OVR.h:361-485) and overrelax_link
 // restates overrelaxationSUN (device_PHB_OVR.h:493-608): N=3 uses the reference's
 // block-projection fast path, other N the M = U*F path.  The link, the staple and
-// the SU(2) algebra run in the storage precision R; the sampler runs in double.
+// the SU(2) algebra run in the storage precision R; the sampler runs in double for
+// FP64 links and in FP32 (generate_su2_milc_f32) for FP32 links.
 #pragma once
+
+#include <type_traits>
+#include <utility>
 
 #include "sun_rng.hpp"
 
@@ -39,6 +43,18 @@ SUN_HD void conj_normalize(Quat<R>& r) {
     r.a[1] *= -norm;
     r.a[2] *= -norm;
     r.a[3] *= -norm;
+}
+
+// r r for a unit quaternion in the SU(2) product of msu2.h:90-97:
+// (a0^2 - |a|^2, 2 a0 a1, 2 a0 a2, 2 a0 a3)
+template <typename R>
+SUN_HD Quat<R> quat_square(const Quat<R>& v) {
+    Quat<R> r;
+    r.a[0] = v.a[0] * v.a[0] - v.a[1] * v.a[1] - v.a[2] * v.a[2] - v.a[3] * v.a[3];
+    r.a[1] = R(2) * v.a[0] * v.a[1];
+    r.a[2] = R(2) * v.a[0] * v.a[2];
+    r.a[3] = R(2) * v.a[0] * v.a[3];
+    return r;
 }
 
 // rows p,q of U times columns p,q of F, projected on SU(2) (device_PHB_OVR.h:385-400)
@@ -84,6 +100,38 @@ SUN_HD void mul_block(const Quat<R>& u, Mat<R, N>& link, int x, int y) {
     }
 }
 
+// one SU(2)-subgroup heatbath step of the N != 3 path on rows (x, y) of block B:
+// U <- r U, M <- r M with r = a (M-block)^+ normalised (device_PHB_OVR.h:430-482)
+template <typename R, int N, int B>
+SUN_HD void heatbath_block(Mat<R, N>& U, Mat<R, N>& M, double beta_over_nc, Stream& st) {
+    constexpr int x = block_p(N, B), y = block_q(N, B);
+    Quat<R> r = get_block_su2<R, N>(M, x, y);
+    R k = qabs(r);
+    R ap = R(beta_over_nc) * k;
+    k = R(1) / k;
+#pragma unroll
+    for (int i = 0; i < 4; i++) r.a[i] *= k;
+    Quat<R> a;
+    if constexpr (std::is_same<R, float>::value) {
+        // FP32 links: the FP32 sampler (as the SU(3) FP32 path)
+        generate_su2_milc_f32(ap, st, a.a);
+    } else {
+        double ad[4];
+        generate_su2_milc((double)ap, st, ad);
+#pragma unroll
+        for (int i = 0; i < 4; i++) a.a[i] = R(ad[i]);
+    }
+    const Quat<R> rr = mul_uv_dagger(a, r);
+    mul_block<R, N>(rr, U, x, y);
+    mul_block<R, N>(rr, M, x, y);
+}
+
+template <typename R, int N, int... B>
+SUN_HD void heatbath_blocks(Mat<R, N>& U, Mat<R, N>& M, double beta_over_nc, Stream& st,
+                            std::integer_sequence<int, B...>) {
+    (heatbath_block<R, N, B>(U, M, beta_over_nc, st), ...);
+}
+
 template <typename R, int N>
 SUN_HD void heatbath_link(Mat<R, N>& U, const Mat<R, N>& F, double beta_over_nc, Stream& st) {
     if constexpr (N == 3) {
@@ -113,24 +161,9 @@ SUN_HD void heatbath_link(Mat<R, N>& U, const Mat<R, N>& F, double beta_over_nc,
         }
     } else {
         Mat<R, N> M = U * F;
-#pragma unroll
-        for (int block = 0; block < N * (N - 1) / 2; block++) {
-            const int x = block_p(N, block), y = block_q(N, block);
-            Quat<R> r = get_block_su2<R, N>(M, x, y);
-            R k = qabs(r);
-            R ap = R(beta_over_nc) * k;
-            k = R(1) / k;
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.a[i] *= k;
-            double ad[4];
-            generate_su2_milc((double)ap, st, ad);
-            Quat<R> a;
-#pragma unroll
-            for (int i = 0; i < 4; i++) a.a[i] = R(ad[i]);
-            Quat<R> rr = mul_uv_dagger(a, r);
-            mul_block<R, N>(rr, U, x, y);
-            mul_block<R, N>(rr, M, x, y);
-        }
+        // one instance per block: the block's rows are compile-time constants (a
+        // rolled loop would index U and M at run time, i.e. from scratch memory)
+        heatbath_blocks<R, N>(U, M, beta_over_nc, st, std::make_integer_sequence<int, N * (N - 1) / 2>{});
     }
 }
 
@@ -159,10 +192,18 @@ SUN_HD void overrelax_link(Mat<R, N>& U, const Mat<R, N>& F) {
             const int x = block_p(N, block), y = block_q(N, block);
             Quat<R> r = get_block_su2<R, N>(M, x, y);
             conj_normalize(r);
-            mul_block<R, N>(r, U, x, y);
-            mul_block<R, N>(r, U, x, y);
-            mul_block<R, N>(r, M, x, y);
-            mul_block<R, N>(r, M, x, y);
+            if constexpr (std::is_same<R, float>::value) {
+                // FP32 links: the rotation applied once as r^2 (half the row updates;
+                // equal to applying r twice up to FP32 rounding)
+                const Quat<R> r2 = quat_square(r);
+                mul_block<R, N>(r2, U, x, y);
+                mul_block<R, N>(r2, M, x, y);
+            } else {
+                mul_block<R, N>(r, U, x, y);
+                mul_block<R, N>(r, U, x, y);
+                mul_block<R, N>(r, M, x, y);
+                mul_block<R, N>(r, M, x, y);
+            }
         }
     }
 }
