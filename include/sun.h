@@ -163,6 +163,12 @@ int sun_link_det(sun_lattice *lat, void *stream, double out[2]);
  * directions; Proj = SU3project with nhit hits for N = 3 (tol unused), project_link
  * from the old link with nhit hits and tolerance tol otherwise.  In place. */
 int sun_ape_smear(sun_lattice *lat, double w, int steps, int nhit, double tol, int time_dir, void *stream);
+/* ApplyStoutinSpace / ApplyStoutinTime (stout.cu:687-766), SU(3) only: `steps` rounds;
+ * each replaces the spatial links (time_dir 0) or the temporal ones (time_dir 1),
+ * from the previous round's field, by exp(iQ) U, Q the traceless Hermitian part of
+ * -i V U^+ with V = rho x the staples in the spatial directions (Morningstar-Peardon
+ * exponential, FP64 arithmetic).  In place; other N: SUN_ERR_UNSUPPORTED. */
+int sun_stout_smear(sun_lattice *lat, double rho, int steps, int time_dir, void *stream);
 /* ApplyMultiHit (multihit.cu:291-333): out = in with every temporal link replaced by
  * the mean of itself and nhit heatbath updates in its staple (beta of `in`), RNG
  * stream (seed, site, hit); out: another lattice of the same N, extents and precision,

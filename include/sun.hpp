@@ -337,6 +337,15 @@ template <class Real>
 void ApplyAPEinTime(_gauge<Real>& U, Real w, int steps, int nhits, Real tol) {
     check(sun_ape_smear(U.handle(), (double)w, steps, nhits, (double)tol, 1, nullptr), "ApplyAPEinTime");
 }
+/* ApplyStoutinSpace<Real>(gauge, w, steps) / ApplyStoutinTime   stout.cu:687-766 (SU(3)) */
+template <class Real>
+void ApplyStoutinSpace(_gauge<Real>& U, Real w, int steps) {
+    check(sun_stout_smear(U.handle(), (double)w, steps, 0, nullptr), "ApplyStoutinSpace");
+}
+template <class Real>
+void ApplyStoutinTime(_gauge<Real>& U, Real w, int steps) {
+    check(sun_stout_smear(U.handle(), (double)w, steps, 1, nullptr), "ApplyStoutinTime");
+}
 /* ApplyMultiHit(in, out, RNG, nhit)   multihit.cu:291-333; out must be SOA (18-real) */
 template <class Real>
 void ApplyMultiHit(_gauge<Real>& in, _gauge<Real>& out, RNG& rng, int nhit) {

@@ -73,6 +73,8 @@ def lib() -> ctypes.CDLL:
         "or_su3_project": (None, [dp, ci]),
         "or_ape_smear": (ci, [ci, ip, dp, cd, ci, ci, cd, ci]),
         "or_multihit": (ci, [ci, ip, cd, u64, ci, dp, dp]),
+        "or_exp_iq": (None, [dp, dp]),
+        "or_stout_smear": (ci, [ip, dp, cd, ci, ci]),
         "or_wilson_loops": (ci, [ci, ip, dp, ci, ci, dp]),
         "or_matmul": (None, [ci, dp, dp, dp]),
         "or_det": (None, [ci, dp, dp]),
@@ -298,6 +300,19 @@ def su3_project(F, nhit: int):
 def ape_smear(U, dims, w: float, steps: int, nhit: int, tol: float = 0.0, time_dir: bool = False):
     """in place on an 18-real field"""
     _check(lib().or_ape_smear(U.shape[-1], _dims(dims), _dp(U), w, steps, nhit, tol, int(time_dir)), "ape_smear")
+    return U
+
+
+def exp_iq(Q):
+    Q = np.ascontiguousarray(Q, dtype=np.complex128)
+    E = np.zeros_like(Q)
+    lib().or_exp_iq(_dp(Q), _dp(E))
+    return E
+
+
+def stout_smear(U, dims, rho: float, steps: int, time_dir: bool = False):
+    """in place on an SU(3) 18-real field"""
+    _check(lib().or_stout_smear(_dims(dims), _dp(U), rho, steps, int(time_dir)), "stout_smear")
     return U
 
 

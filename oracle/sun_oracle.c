@@ -1032,6 +1032,115 @@ int or_ape_smear(int nc, const int X[4], double *U, double w, int steps, int nhi
     return 0;
 }
 
+/* exp(iQ) for traceless Hermitian 3x3 Q: Morningstar & Peardon, Phys. Rev. D 69 (2004)
+ * 054501, eqs. (19)-(34): f0 + f1 Q + f2 Q^2, the f_j from c0 = Tr Q^3 / 3 and
+ * c1 = Tr Q^2 / 2; for c1 < 4e-3 the exponential series reduced with Q^3 = c1 Q + c0
+ * (order 16).  (stout.cu:94-466 computes the same function.) */
+static mat exp_iq(const mat *Q) {
+    mat QQ = mmul(3, Q, Q), QQQ = mmul(3, Q, &QQ);
+    double c1 = 0.5 * mrealtrace(3, &QQ), c0 = mrealtrace(3, &QQQ) / 3.0;
+    cx f0, f1, f2;
+    if (c1 < 4.0e-3) {
+        double a = 1.0, b = 0.0, c = 0.0;
+        cx t = cmk(1.0, 0.0);
+        f0 = f1 = f2 = cmk(0.0, 0.0);
+        for (int k = 0; k <= 16; k++) {
+            f0 = cadd(f0, cscale(t, a));
+            f1 = cadd(f1, cscale(t, b));
+            f2 = cadd(f2, cscale(t, c));
+            double na = c * c0, nb = a + c * c1, nc = b;
+            a = na;
+            b = nb;
+            c = nc;
+            t = cmk(-t.im / (k + 1), t.re / (k + 1));
+        }
+    } else {
+        double s3 = sqrt(c1 / 3.0), c0max = 2.0 * (c1 / 3.0) * s3;
+        double ratio = fabs(c0) / c0max;
+        double theta = acos(ratio < 1.0 ? ratio : 1.0);
+        double u = s3 * cos(theta / 3.0), w = sqrt(c1) * sin(theta / 3.0);
+        double u2 = u * u, w2 = w * w;
+        double xi0 = fabs(w) < 0.05 ? 1.0 - w2 / 6.0 * (1.0 - w2 / 20.0 * (1.0 - w2 / 42.0)) : sin(w) / w;
+        double cw = cos(w);
+        cx e2iu = cmk(cos(2.0 * u), sin(2.0 * u)), emiu = cmk(cos(u), -sin(u));
+        cx h0 = cadd(cscale(e2iu, u2 - w2), cmul(emiu, cmk(8.0 * u2 * cw, 2.0 * u * (3.0 * u2 + w2) * xi0)));
+        cx h1 = csub(cscale(e2iu, 2.0 * u), cmul(emiu, cmk(2.0 * u * cw, -(3.0 * u2 - w2) * xi0)));
+        cx h2 = csub(e2iu, cmul(emiu, cmk(cw, 3.0 * u * xi0)));
+        double den = 1.0 / (9.0 * u2 - w2);
+        f0 = cscale(h0, den);
+        f1 = cscale(h1, den);
+        f2 = cscale(h2, den);
+        if (c0 < 0) {
+            f0 = cconj(f0);
+            f1 = cmk(-f1.re, f1.im);
+            f2 = cconj(f2);
+        }
+    }
+    mat E;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            cx e = cadd(cmul(f1, Q->e[i][j]), cmul(f2, QQ.e[i][j]));
+            E.e[i][j] = i == j ? cadd(e, f0) : e;
+        }
+    return E;
+}
+
+void or_exp_iq(const double *Q, double *E) {
+    mat q, e;
+    mload(3, Q, &q);
+    e = exp_iq(&q);
+    mstore(3, &e, E);
+}
+
+/* stout.cu:86-92,519-535,556-601: W = exp(iQ) U, Q the traceless part of
+ * (i/2)(Omega^+ - Omega), Omega = rho S U^+, S the spatial staples */
+static void stout_dir(const int X[4], const double *Uin, double *Uout, int mu, double rho) {
+    long V = volume_of(X), half = V / 2;
+#pragma omp parallel for schedule(static)
+    for (long idd = 0; idd < V; idd++) {
+        int parity = idd >= half ? 1 : 0;
+        int id = (int)(idd - parity * half);
+        mat S = staple_range(3, X, Uin, parity, mu, id, 3);
+        mat U, Ud, Om, Q, E, W;
+        mload(3, link_at(3, Uin, V, mu, idd), &U);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) S.e[i][j] = cscale(S.e[i][j], rho);
+        Ud = mdag(3, &U);
+        Om = mmul(3, &S, &Ud);
+        cx tr = cmk(0.0, 0.0);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                cx d = csub(cconj(Om.e[j][i]), Om.e[i][j]);
+                Q.e[i][j] = cmk(-0.5 * d.im, 0.5 * d.re);
+            }
+        for (int i = 0; i < 3; i++) tr = cadd(tr, Q.e[i][i]);
+        for (int i = 0; i < 3; i++) Q.e[i][i] = csub(Q.e[i][i], cscale(tr, 1.0 / 3.0));
+        E = exp_iq(&Q);
+        W = mmul(3, &E, &U);
+        mstore(3, &W, link_at_w(3, Uout, V, mu, idd));
+    }
+}
+
+/* stout.cu:687-766: ApplyStoutinSpace (mu = 0, 1, 2) / ApplyStoutinTime (mu = 3) */
+int or_stout_smear(const int X[4], double *U, double rho, int steps, int time_dir) {
+    int rc = check_args(3, X, OR_STORE_18);
+    if (rc) return rc;
+    long n = 4 * volume_of(X) * 18;
+    double *out = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!out) return -4;
+    memcpy(out, U, sizeof(double) * (size_t)n);
+    for (int st = 0; st < steps; st++) {
+        if (time_dir) {
+            stout_dir(X, U, out, 3, rho);
+        } else {
+            for (int mu = 0; mu < 3; mu++) stout_dir(X, U, out, mu, rho);
+        }
+        memcpy(U, out, sizeof(double) * (size_t)n);
+    }
+    free(out);
+    return 0;
+}
+
 /* multihit.cu:112-147 (kernel_evenodd): every temporal link replaced by the mean of
  * itself and nhit successive heatbath updates in its (fixed) staple, all from the
  * input field; spatial links copied.  Hit k of the link (parity, id) draws from the

@@ -100,6 +100,11 @@ void or_su3_project(double *F, int nhit);
 /* ape.cu:198-264: steps of APE smearing, spatial (time_dir 0: mu = 0,1,2 with
  * spatial staples) or temporal (time_dir 1: mu = 3), weight w; 18-real U in place */
 int or_ape_smear(int nc, const int dims[4], double *U, double w, int steps, int nhit, double tol, int time_dir);
+/* exp(iQ), Q traceless Hermitian 3x3 (Morningstar-Peardon; stout.cu:94-466) */
+void or_exp_iq(const double *Q, double *E);
+/* stout.cu:687-766: steps of stout smearing with weight rho, spatial (time_dir 0) or
+ * temporal (time_dir 1) links, SU(3) 18-real U in place */
+int or_stout_smear(const int dims[4], double *U, double rho, int steps, int time_dir);
 /* multihit.cu:112-147: Uout = Uin with each temporal link replaced by the mean of
  * itself and nhit heatbath updates in its staple (stream (seed, id, hit, TAG_MH|p<<4|3)) */
 int or_multihit(int nc, const int dims[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout);

@@ -1,5 +1,6 @@
 // sun_meas.hip -- the reference's measurement chain on a resident lattice (host
-// code): APE smearing (smear/ape.cu:198-264), multihit of temporal links
+// code): APE smearing (smear/ape.cu:198-264), stout smearing (smear/stout.cu:687-766),
+// multihit of temporal links
 // (smear/multihit.cu:291-333) and Wilson loops (meas/wilsonloop.cu:494-516).
 // Scratch (the ping-pong field of APE, the Wilson lines and partial sums) is
 // allocated per call: these are measurements between generation steps, not part
@@ -50,6 +51,31 @@ extern "C" int sun_ape_smear(sun_lattice* lat, double w, int steps, int nhit, do
         const Field& dst = (st & 1) ? lat->f : other;
         for (int mu = time_dir ? 3 : 0; mu < (time_dir ? 4 : 3); mu++)
             HIP_TRY(lat->ops->ape(src, dst, lat->g, mu, w, nhit, tol, s));
+    }
+    if (steps & 1) HIP_TRY(hipMemcpyAsync(lat->buf, tmp.p, lat->bytes, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SUN_OK;
+}
+
+extern "C" int sun_stout_smear(sun_lattice* lat, double rho, int steps, int time_dir, void* stream) {
+    SUN_TRY(check_lattice(lat, "sun_stout_smear"));
+    if (!lat->ops->stout) return fail(SUN_ERR_UNSUPPORTED, "sun_stout_smear: SU(3) only (N = %d)", lat->d.nc);
+    if (steps < 0 || !(rho >= 0.0) || (time_dir != 0 && time_dir != 1))
+        return fail(SUN_ERR_INVALID, "sun_stout_smear: steps %d, rho %g, time_dir %d", steps, rho, time_dir);
+    if (steps == 0) return SUN_OK;
+    hipStream_t s = pick_stream(lat, stream);
+    HIP_TRY(hipSetDevice(lat->ctx->device));
+    DeviceScratch tmp;
+    HIP_TRY(hipMalloc(&tmp.p, lat->bytes));
+    // the ping-pong of sun_ape_smear: every direction of a step reads the previous
+    // step's field (stout.cu:697-705)
+    HIP_TRY(hipMemcpyAsync(tmp.p, lat->buf, lat->bytes, hipMemcpyDeviceToDevice, s));
+    const Field other = make_field(tmp.p, 4 * (size_t)lat->g.V, link_bytes(&lat->d));
+    for (int st = 0; st < steps; st++) {
+        const Field& src = (st & 1) ? other : lat->f;
+        const Field& dst = (st & 1) ? lat->f : other;
+        for (int mu = time_dir ? 3 : 0; mu < (time_dir ? 4 : 3); mu++)
+            HIP_TRY(lat->ops->stout(src, dst, lat->g, mu, rho, s));
     }
     if (steps & 1) HIP_TRY(hipMemcpyAsync(lat->buf, tmp.p, lat->bytes, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));

@@ -118,6 +118,8 @@ struct OpsTable {
     // time, 3 * (tmax + 1) * monitor_blocks(geom) partials
     hipError_t (*ape)(const Field& in, const Field& out, const Geom&, int mu, double w, int nhit, double tol,
                       hipStream_t);
+    // one stout direction in -> out (SU(3) variants; null otherwise)
+    hipError_t (*stout)(const Field& in, const Field& out, const Geom&, int mu, double rho, hipStream_t);
     hipError_t (*multihit)(const Field& in, const Field& out, const Geom&, uint64_t seed, int nhit, double bnc,
                            hipStream_t);
     hipError_t (*wl_lines)(const Field& f, const Field& lines, const Geom&, int r, hipStream_t);

@@ -327,4 +327,90 @@ SUN_HD void reunit_link(Mat<R, N>& U) {
     }
 }
 
+
+// exp(iQ) for a traceless Hermitian 3x3 Q (Morningstar and Peardon, Phys. Rev. D 69
+// (2004) 054501, eqs. (19)-(34)): exp(iQ) = f0 + f1 Q + f2 Q^2 with the f_j from
+// c0 = det Q = Tr Q^3 / 3 and c1 = Tr Q^2 / 2.  For c1 below kStoutSeriesC1 (|Q| <
+// 0.08) the f_j are the exponential series reduced by Cayley-Hamilton,
+// Q^3 = c1 Q + c0, summed to order 16 (the closed form divides by ~c1 there).
+constexpr double kStoutSeriesC1 = 4.0e-3;
+
+SUN_HD void exp_iq_su3(const Mat<double, 3>& Q, Mat<double, 3>& E) {
+    const Mat<double, 3> QQ = Q * Q;
+    const double c1 = 0.5 * realtrace(QQ);
+    const double c0 = realtrace(Q * QQ) / 3.0;
+    cplx<double> f0, f1, f2;
+    if (c1 < kStoutSeriesC1) {
+        // Q^k = a + b Q + c Q^2 -> Q^(k+1) = c c0 + (a + c c1) Q + b Q^2
+        double a = 1.0, b = 0.0, c = 0.0;
+        cplx<double> t = mk<double>(1.0, 0.0);  // i^k / k!
+        f0 = f1 = f2 = mk<double>(0.0, 0.0);
+#pragma unroll 1
+        for (int k = 0; k <= 16; k++) {
+            f0 = f0 + scale(t, a);
+            f1 = f1 + scale(t, b);
+            f2 = f2 + scale(t, c);
+            const double na = c * c0, nb = a + c * c1, nc = b;
+            a = na;
+            b = nb;
+            c = nc;
+            t = mk<double>(-t.im / (k + 1), t.re / (k + 1));
+        }
+    } else {
+        const double c0abs = fabs(c0);
+        const double s3 = sqrt(c1 / 3.0);
+        const double c0max = 2.0 * (c1 / 3.0) * s3;
+        const double theta = acos(fmin(c0abs / c0max, 1.0));
+        const double u = s3 * cos(theta / 3.0), w = sqrt(c1) * sin(theta / 3.0);
+        const double u2 = u * u, w2 = w * w;
+        const double xi0 = fabs(w) < 0.05 ? 1.0 - w2 / 6.0 * (1.0 - w2 / 20.0 * (1.0 - w2 / 42.0)) : sin(w) / w;
+        const double cw = cos(w);
+        const cplx<double> e2iu = mk<double>(cos(2.0 * u), sin(2.0 * u));
+        const cplx<double> emiu = mk<double>(cos(u), -sin(u));
+        const cplx<double> h0 = scale(e2iu, u2 - w2) + emiu * mk<double>(8.0 * u2 * cw, 2.0 * u * (3.0 * u2 + w2) * xi0);
+        const cplx<double> h1 = scale(e2iu, 2.0 * u) - emiu * mk<double>(2.0 * u * cw, -(3.0 * u2 - w2) * xi0);
+        const cplx<double> h2 = e2iu - emiu * mk<double>(cw, 3.0 * u * xi0);
+        const double den = 1.0 / (9.0 * u2 - w2);
+        f0 = scale(h0, den);
+        f1 = scale(h1, den);
+        f2 = scale(h2, den);
+        if (c0 < 0) {  // f_j(-c0) = (-1)^j conj f_j(c0)
+            f0 = conj(f0);
+            f1 = mk<double>(-f1.re, f1.im);
+            f2 = conj(f2);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplx<double> e = f1 * Q.e[i][j] + f2 * QQ.e[i][j];
+            if (i == j) e = e + f0;
+            E.e[i][j] = e;
+        }
+}
+
+// One stout step of a link (stout.cu:86-92, 519-535): Omega = V U^+ with V = rho x
+// the staple sum, Q = the traceless part of (i/2)(Omega^+ - Omega), W = exp(iQ) U.
+SUN_HD Mat<double, 3> stout_link(const Mat<double, 3>& V, const Mat<double, 3>& U) {
+    const Mat<double, 3> Om = V * dagger(U);
+    Mat<double, 3> Q;
+    cplx<double> tr = mk<double>(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            // (i/2)(conj(Om_ji) - Om_ij)
+            const cplx<double> d = conj(Om.e[j][i]) - Om.e[i][j];
+            Q.e[i][j] = mk<double>(-0.5 * d.im, 0.5 * d.re);
+        }
+#pragma unroll
+    for (int i = 0; i < 3; i++) tr = tr + Q.e[i][i];
+#pragma unroll
+    for (int i = 0; i < 3; i++) Q.e[i][i] = Q.e[i][i] - scale(tr, 1.0 / 3.0);
+    Mat<double, 3> E;
+    exp_iq_su3(Q, E);
+    return E * U;
+}
+
 }  // namespace sun

@@ -1,5 +1,6 @@
 """The measurement chain on the GPU against the FP64 oracle: APE smearing
-(ape.cu), multihit (multihit.cu) and Wilson loops (wilsonloop.cu), every variant.
+(ape.cu), stout smearing (stout.cu, SU(3) variants), multihit (multihit.cu) and
+Wilson loops (wilsonloop.cu), every variant.
 
 Bars: FP64 1e-12 per link (smearing, multihit) and 1e-13 on W(r, T); FP32 5e-5 per
 link plus four times the storage round trip of the oracle's result (8-real FP32
@@ -40,6 +41,33 @@ def test_ape_parity(ctx, oracle, nc, storage, prec, time_dir):
     assert np.all(d <= bar), (float(d.max()), float(bar.max()))
     untouched = slice(0, 3) if time_dir else slice(3, 4)
     assert np.array_equal(got[untouched], U[untouched])
+
+
+SU3 = [(v, i) for v, i in zip(VARIANTS, IDS) if v[0] == 3]
+
+
+@pytest.mark.parametrize("nc,storage,prec", [v for v, _ in SU3], ids=[i for _, i in SU3])
+@pytest.mark.parametrize("time_dir", [False, True], ids=["space", "time"])
+def test_stout_parity(ctx, oracle, nc, storage, prec, time_dir):
+    with sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], storage, prec) as lat:
+        U = load_config(lat, thermalised(oracle, nc, DIMS, storage))
+        lat.stout_smear(0.15, 3, time_dir)
+        got = lat.to_host()
+    ref = oracle.stout_smear(U.copy(), DIMS, 0.15, 3, time_dir)
+    d = per_link_diff(got, ref)
+    bar = tol(prec) + 4 * per_link_diff(stored(ctx, nc, storage, prec, ref), ref)
+    assert np.all(d <= bar), (float(d.max()), float(bar.max()))
+    untouched = slice(0, 3) if time_dir else slice(3, 4)
+    assert np.array_equal(got[untouched], U[untouched])
+
+
+@pytest.mark.parametrize("nc", [2, 4])
+def test_stout_su3_only(ctx, nc):
+    with sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], 18, 8) as lat:
+        lat.cold_start()
+        with pytest.raises(sun_amd.SunError) as e:
+            lat.stout_smear(0.1, 1)
+        assert e.value.code == sun_amd.ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)

@@ -279,3 +279,93 @@ def test_wilson_abelian_stokes(oracle):
     T = np.arange(6)[None, :]
     expect = (np.exp(1j * phi[None, None, :] * (r * T)[..., None]).mean(axis=-1) + 2) / 3
     assert np.abs(W - expect).max() < 1e-14
+
+
+# ---- stout smearing (Morningstar-Peardon exponential) ----
+def herm_traceless(rng, scale):
+    H = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+    H = (H + H.conj().T) / 2
+    H -= np.trace(H) / 3 * np.eye(3)
+    return H * scale / np.linalg.norm(H)
+
+
+def test_exp_iq_matches_expm(oracle):
+    from scipy.linalg import expm
+    rng = np.random.default_rng(30)
+    cases = []
+    for scale in (1e-9, 1e-4, 0.03, 0.0894, 0.0895, 0.2, 1.0, 3.0, 7.0):  # both sides of c1 = 4e-3
+        for _ in range(20):
+            cases.append(herm_traceless(rng, scale))
+    for a in (1e-6, 0.05, 0.7, 2.0):  # two equal eigenvalues: c0 = +-c0max
+        cases.append(np.diag([a, a, -2 * a]).astype(complex))
+        cases.append(np.diag([-a, -a, 2 * a]).astype(complex))
+    cases.append(np.zeros((3, 3), dtype=complex))
+    for Q in cases:
+        E = oracle.exp_iq(Q)
+        R = expm(1j * Q)
+        assert np.abs(E - R).max() < 1e-13 * max(1.0, np.abs(Q).max()), (np.abs(E - R).max(), np.abs(Q).max())
+        # random unitary conjugation keeps the eigenvalue structure (c0, c1)
+        W = haar(rng, 3)
+        assert np.abs(oracle.exp_iq(W @ Q @ W.conj().T) - W @ E @ W.conj().T).max() < 1e-12
+
+
+def test_stout_gauge_covariant_unitary_and_fixed_point(oracle):
+    dims = (4, 4, 4, 6)
+    rng = np.random.default_rng(31)
+    U = oracle.hot_start(3, dims, 32)
+    for it in range(3):
+        oracle.heatbath_sweep(U, dims, 6.0, 33, it)
+    G = haar(rng, 3, (384,))
+    a = oracle.stout_smear(gauge_transform(U, dims, G), dims, 0.15, 3)
+    b = gauge_transform(oracle.stout_smear(U.copy(), dims, 0.15, 3), dims, G)
+    assert np.abs(a - b).max() < 1e-12
+    S = oracle.stout_smear(U.copy(), dims, 0.15, 3)
+    L = S.reshape(-1, 3, 3)
+    assert np.abs(L @ np.conj(np.swapaxes(L, 1, 2)) - np.eye(3)).max() < 1e-13
+    assert np.abs(np.linalg.det(L) - 1).max() < 1e-13
+    assert np.array_equal(S[3], U[3])
+    assert oracle.plaquette(S, dims)[0] > oracle.plaquette(U, dims)[0] + 0.05
+    assert np.abs(oracle.stout_smear(U.copy(), dims, 0.0, 2) - U).max() < 1e-15
+    C = oracle.cold_start(3, dims)
+    assert np.abs(oracle.stout_smear(C.copy(), dims, 0.2, 2) - C).max() < 1e-15
+    T = oracle.stout_smear(U.copy(), dims, 0.15, 1, time_dir=True)
+    assert np.array_equal(T[:3], U[:3]) and np.abs(T[3] - U[3]).max() > 1e-3
+
+
+def test_stout_small_rho_is_the_lie_algebra_step(oracle):
+    """for small rho, W = (1 + i rho Q1 + O(rho^2)) U with Q1 the traceless Hermitian
+    part of -i S U^+ (MP eq. (2)-(3)): checked on one link against a direct numpy
+    evaluation of the first-order term"""
+    dims = (4, 4, 4, 4)
+    U = oracle.hot_start(3, dims, 34)
+    rho = 1e-6
+    S_ = oracle.stout_smear(U.copy(), dims, rho, 1)
+    for idd in (0, 100, 200):
+        for mu in range(3):
+            Om = rho * spatial_staple(U, dims, mu, idd) @ U[mu, idd].conj().T
+            Q = 0.5j * (Om.conj().T - Om)
+            Q -= np.trace(Q) / 3 * np.eye(3)
+            W1 = (np.eye(3) + 1j * Q) @ U[mu, idd]
+            assert np.abs(S_[mu, idd] - W1).max() < 1e-10
+
+
+def spatial_staple(U, dims, mu, idd):
+    L = to_lex(U, dims)
+    V = int(np.prod(dims))
+    X0, X1, X2, X3 = dims
+    # coordinates of idd
+    parity, sid = idd // (V // 2), idd % (V // 2)
+    found = None
+    for lex in range(V):
+        x0, x1, x2, x3 = lex % X0, (lex // X0) % X1, (lex // (X0 * X1)) % X2, lex // (X0 * X1 * X2)
+        if lex // 2 + ((x0 + x1 + x2 + x3) & 1) * (V // 2) == idd:
+            found = (x3, x2, x1, x0)
+            break
+    S = np.zeros((3, 3), dtype=complex)
+    for nu in range(3):
+        if nu == mu:
+            continue
+        Lnu_x = L[nu][found]
+        S += Lnu_x @ shift(L[mu], nu, 1)[found] @ dag(shift(L[nu], mu, 1)[found])
+        S += dag(shift(L[nu], nu, -1)[found]) @ shift(L[mu], nu, -1)[found] @ shift(shift(L[nu], nu, -1), mu, 1)[found]
+    return S
