@@ -169,6 +169,12 @@ int sun_ape_smear(sun_lattice *lat, double w, int steps, int nhit, double tol, i
  * -i V U^+ with V = rho x the staples in the spatial directions (Morningstar-Peardon
  * exponential, FP64 arithmetic).  In place; other N: SUN_ERR_UNSUPPORTED. */
 int sun_stout_smear(sun_lattice *lat, double rho, int steps, int time_dir, void *stream);
+/* ApplyHYPinSpace / ApplyHYPinTime (hyp.cu:277-368, ParamHYP defaults 0.75, 0.6, 0.3):
+ * `steps` rounds of hypercubic blocking of the spatial (time_dir 0) or temporal
+ * (time_dir 1) links, each from the previous round's field; every level projected by
+ * project_link from the unsmeared link with nhit hits and tolerance tol.  In place. */
+int sun_hyp_smear(sun_lattice *lat, double alpha1, double alpha2, double alpha3, int steps, int nhit, double tol,
+                  int time_dir, void *stream);
 /* ApplyMultiHit (multihit.cu:291-333): out = in with every temporal link replaced by
  * the mean of itself and nhit heatbath updates in its staple (beta of `in`), RNG
  * stream (seed, site, hit); out: another lattice of the same N, extents and precision,

@@ -346,6 +346,28 @@ template <class Real>
 void ApplyStoutinTime(_gauge<Real>& U, Real w, int steps) {
     check(sun_stout_smear(U.handle(), (double)w, steps, 1, nullptr), "ApplyStoutinTime");
 }
+/* ParamHYP (smear.h): alpha1..3, defaults 0.75, 0.6, 0.3 */
+struct ParamHYP {
+    float alpha1 = 0.75f, alpha2 = 0.6f, alpha3 = 0.3f;
+    ParamHYP() {}
+    ParamHYP(float a1, float a2, float a3) : alpha1(a1), alpha2(a2), alpha3(a3) {}
+    void set(float a1, float a2, float a3) {
+        alpha1 = a1;
+        alpha2 = a2;
+        alpha3 = a3;
+    }
+};
+/* ApplyHYPinSpace<Real>(gauge, steps, nhits, tol, ParamHYP) / ApplyHYPinTime   hyp.cu:277-368 */
+template <class Real>
+void ApplyHYPinSpace(_gauge<Real>& U, int steps, int nhits, Real tol, ParamHYP hyp) {
+    check(sun_hyp_smear(U.handle(), hyp.alpha1, hyp.alpha2, hyp.alpha3, steps, nhits, (double)tol, 0, nullptr),
+          "ApplyHYPinSpace");
+}
+template <class Real>
+void ApplyHYPinTime(_gauge<Real>& U, int steps, int nhits, Real tol, ParamHYP hyp) {
+    check(sun_hyp_smear(U.handle(), hyp.alpha1, hyp.alpha2, hyp.alpha3, steps, nhits, (double)tol, 1, nullptr),
+          "ApplyHYPinTime");
+}
 /* ApplyMultiHit(in, out, RNG, nhit)   multihit.cu:291-333; out must be SOA (18-real) */
 template <class Real>
 void ApplyMultiHit(_gauge<Real>& in, _gauge<Real>& out, RNG& rng, int nhit) {

@@ -75,6 +75,7 @@ def lib() -> ctypes.CDLL:
         "or_multihit": (ci, [ci, ip, cd, u64, ci, dp, dp]),
         "or_exp_iq": (None, [dp, dp]),
         "or_stout_smear": (ci, [ip, dp, cd, ci, ci]),
+        "or_hyp_smear": (ci, [ci, ip, dp, cd, cd, cd, ci, ci, cd, ci]),
         "or_wilson_loops": (ci, [ci, ip, dp, ci, ci, dp]),
         "or_matmul": (None, [ci, dp, dp, dp]),
         "or_det": (None, [ci, dp, dp]),
@@ -313,6 +314,15 @@ def exp_iq(Q):
 def stout_smear(U, dims, rho: float, steps: int, time_dir: bool = False):
     """in place on an SU(3) 18-real field"""
     _check(lib().or_stout_smear(_dims(dims), _dp(U), rho, steps, int(time_dir)), "stout_smear")
+    return U
+
+
+def hyp_smear(U, dims, alphas=(0.75, 0.6, 0.3), steps: int = 1, nhit: int = 10, tol: float = 0.0,
+              time_dir: bool = False):
+    """in place on an 18-real field"""
+    a1, a2, a3 = alphas
+    _check(lib().or_hyp_smear(U.shape[-1], _dims(dims), _dp(U), a1, a2, a3, steps, nhit, tol, int(time_dir)),
+           "hyp_smear")
     return U
 
 

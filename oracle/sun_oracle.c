@@ -1141,6 +1141,151 @@ int or_stout_smear(const int X[4], double *U, double rho, int steps, int time_di
     return 0;
 }
 
+/* HYP smearing, the reference's recursion restated (hyp.cu:51-187): decorated links
+ * recomputed for every use.  Geometry through coordinates, periodic. */
+typedef struct {
+    int nc;
+    const int *X;
+    const double *U;
+    long V;
+    double a1, a2, a3, tol;
+    int nhit;
+} hyp_ctx;
+
+static long hyp_eo(const hyp_ctx *h, const int x[4], int d1, int s1, int d2, int s2) {
+    int y[4] = {x[0], x[1], x[2], x[3]};
+    y[d1] = (y[d1] + s1 + h->X[d1]) % h->X[d1];
+    if (d2 >= 0) y[d2] = (y[d2] + s2 + h->X[d2]) % h->X[d2];
+    return or_coords_to_eo(y, h->X);
+}
+
+static mat hyp_link(const hyp_ctx *h, long eo, int mu) {
+    mat m;
+    mload(h->nc, link_at(h->nc, h->U, h->V, mu, eo), &m);
+    return m;
+}
+
+/* hyp.cu:52-79 (CalcStaple) + 82-103 (HYPSmearVbar) */
+static mat hyp_vbar(const hyp_ctx *h, const int x[4], int mu, int nu, int rho) {
+    int nc = h->nc, eta = 6 - mu - nu - rho;
+    mat a = hyp_link(h, hyp_eo(h, x, 0, 0, -1, 0), eta), b = hyp_link(h, hyp_eo(h, x, eta, 1, -1, 0), mu);
+    mat c = hyp_link(h, hyp_eo(h, x, mu, 1, -1, 0), eta), t, st, dn;
+    st = mmul(nc, &a, &b);
+    t = mdag(nc, &c);
+    st = mmul(nc, &st, &t);
+    a = hyp_link(h, hyp_eo(h, x, eta, -1, -1, 0), eta);
+    b = hyp_link(h, hyp_eo(h, x, eta, -1, -1, 0), mu);
+    c = hyp_link(h, hyp_eo(h, x, eta, -1, mu, 1), eta);
+    t = mdag(nc, &a);
+    dn = mmul(nc, &t, &b);
+    dn = mmul(nc, &dn, &c);
+    madd_to(nc, &st, &dn);
+    mat w = hyp_link(h, hyp_eo(h, x, 0, 0, -1, 0), mu), v;
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) v.e[i][j] = cadd(cscale(st.e[i][j], 0.5 * h->a3), cscale(w.e[i][j], 1.0 - h->a3));
+    su_project(nc, &w, &v, h->nhit, h->tol);
+    return w;
+}
+
+static void hyp_shift(const hyp_ctx *h, const int x[4], int d1, int s1, int d2, int s2, int y[4]) {
+    for (int i = 0; i < 4; i++) y[i] = x[i];
+    y[d1] = (y[d1] + s1 + h->X[d1]) % h->X[d1];
+    if (d2 >= 0) y[d2] = (y[d2] + s2 + h->X[d2]) % h->X[d2];
+}
+
+/* hyp.cu:105-134 (HYPSmearVtil) */
+static mat hyp_vtil(const hyp_ctx *h, const int x[4], int mu, int nu) {
+    int nc = h->nc, y[4];
+    mat vt, l, t;
+    mzero(nc, &vt);
+    for (int rho = 0; rho < 4; rho++) {
+        if (rho == mu || rho == nu) continue;
+        l = hyp_vbar(h, x, rho, nu, mu);
+        hyp_shift(h, x, rho, 1, -1, 0, y);
+        t = hyp_vbar(h, y, mu, rho, nu);
+        l = mmul(nc, &l, &t);
+        hyp_shift(h, x, mu, 1, -1, 0, y);
+        t = hyp_vbar(h, y, rho, nu, mu);
+        t = mdag(nc, &t);
+        l = mmul(nc, &l, &t);
+        madd_to(nc, &vt, &l);
+        hyp_shift(h, x, rho, -1, -1, 0, y);
+        t = hyp_vbar(h, y, rho, nu, mu);
+        l = mdag(nc, &t);
+        t = hyp_vbar(h, y, mu, rho, nu);
+        l = mmul(nc, &l, &t);
+        hyp_shift(h, x, rho, -1, mu, 1, y);
+        t = hyp_vbar(h, y, rho, nu, mu);
+        l = mmul(nc, &l, &t);
+        madd_to(nc, &vt, &l);
+    }
+    mat w = hyp_link(h, hyp_eo(h, x, 0, 0, -1, 0), mu), v;
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) v.e[i][j] = cadd(cscale(vt.e[i][j], 0.25 * h->a2), cscale(w.e[i][j], 1.0 - h->a2));
+    su_project(nc, &w, &v, h->nhit, h->tol);
+    return w;
+}
+
+/* hyp.cu:136-167 (HYPSmearV) */
+static mat hyp_v(const hyp_ctx *h, const int x[4], int mu) {
+    int nc = h->nc, y[4];
+    mat v, l, t;
+    mzero(nc, &v);
+    for (int nu = 0; nu < 4; nu++) {
+        if (nu == mu) continue;
+        l = hyp_vtil(h, x, nu, mu);
+        hyp_shift(h, x, nu, 1, -1, 0, y);
+        t = hyp_vtil(h, y, mu, nu);
+        l = mmul(nc, &l, &t);
+        hyp_shift(h, x, mu, 1, -1, 0, y);
+        t = hyp_vtil(h, y, nu, mu);
+        t = mdag(nc, &t);
+        l = mmul(nc, &l, &t);
+        madd_to(nc, &v, &l);
+        hyp_shift(h, x, nu, -1, -1, 0, y);
+        t = hyp_vtil(h, y, nu, mu);
+        l = mdag(nc, &t);
+        t = hyp_vtil(h, y, mu, nu);
+        l = mmul(nc, &l, &t);
+        hyp_shift(h, x, nu, -1, mu, 1, y);
+        t = hyp_vtil(h, y, nu, mu);
+        l = mmul(nc, &l, &t);
+        madd_to(nc, &v, &l);
+    }
+    mat w = hyp_link(h, hyp_eo(h, x, 0, 0, -1, 0), mu), vv;
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) vv.e[i][j] = cadd(cscale(v.e[i][j], h->a1 / 6.0), cscale(w.e[i][j], 1.0 - h->a1));
+    su_project(nc, &w, &vv, h->nhit, h->tol);
+    return w;
+}
+
+/* hyp.cu:277-368: ApplyHYPinSpace (mu = 0, 1, 2) / ApplyHYPinTime (mu = 3) */
+int or_hyp_smear(int nc, const int X[4], double *U, double a1, double a2, double a3, int steps, int nhit, double tol,
+                 int time_dir) {
+    int rc = check_args(nc, X, OR_STORE_18);
+    if (rc) return rc;
+    long V = volume_of(X), half = V / 2;
+    long n = 4 * V * nc * nc * 2;
+    double *out = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!out) return -4;
+    memcpy(out, U, sizeof(double) * (size_t)n);
+    hyp_ctx h = {nc, X, U, V, a1, a2, a3, tol, nhit};
+    for (int st = 0; st < steps; st++) {
+        for (int mu = time_dir ? 3 : 0; mu < (time_dir ? 4 : 3); mu++) {
+#pragma omp parallel for schedule(dynamic, 16)
+            for (long idd = 0; idd < V; idd++) {
+                int parity = idd >= half ? 1 : 0, x[4];
+                or_eo_to_coords((int)(idd - parity * half), parity, X, x);
+                mat w = hyp_v(&h, x, mu);
+                mstore(nc, &w, link_at_w(nc, out, V, mu, idd));
+            }
+        }
+        memcpy(U, out, sizeof(double) * (size_t)n);
+    }
+    free(out);
+    return 0;
+}
+
 /* multihit.cu:112-147 (kernel_evenodd): every temporal link replaced by the mean of
  * itself and nhit successive heatbath updates in its (fixed) staple, all from the
  * input field; spatial links copied.  Hit k of the link (parity, id) draws from the

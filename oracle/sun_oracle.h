@@ -105,6 +105,11 @@ void or_exp_iq(const double *Q, double *E);
 /* stout.cu:687-766: steps of stout smearing with weight rho, spatial (time_dir 0) or
  * temporal (time_dir 1) links, SU(3) 18-real U in place */
 int or_stout_smear(const int dims[4], double *U, double rho, int steps, int time_dir);
+/* hyp.cu:277-368: steps of HYP smearing (alphas a1, a2, a3; project_link from the
+ * unsmeared link with nhit hits, tolerance tol) of the spatial (time_dir 0) or
+ * temporal (time_dir 1) links; 18-real U in place.  The reference's recursion. */
+int or_hyp_smear(int nc, const int dims[4], double *U, double a1, double a2, double a3, int steps, int nhit,
+                 double tol, int time_dir);
 /* multihit.cu:112-147: Uout = Uin with each temporal link replaced by the mean of
  * itself and nhit heatbath updates in its staple (stream (seed, id, hit, TAG_MH|p<<4|3)) */
 int or_multihit(int nc, const int dims[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout);

@@ -44,7 +44,7 @@ EXPORTS = [
     "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
     "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
     "sun_link_monitor", "sun_link_monitor_async", "sun_link_trace", "sun_link_det",
-    "sun_ape_smear", "sun_stout_smear", "sun_multihit", "sun_wilson_loops",
+    "sun_ape_smear", "sun_stout_smear", "sun_hyp_smear", "sun_multihit", "sun_wilson_loops",
     "sun_links_to_host", "sun_links_from_host",
     "sun_generate",
     "sun_batch_create", "sun_batch_destroy", "sun_batch_heatbath", "sun_batch_overrelax",
@@ -155,6 +155,7 @@ def lib() -> ctypes.CDLL:
         "sun_link_det": (ci, [vp, vp, dp]),
         "sun_ape_smear": (ci, [vp, cd, ci, ci, cd, ci, vp]),
         "sun_stout_smear": (ci, [vp, cd, ci, ci, vp]),
+        "sun_hyp_smear": (ci, [vp, cd, cd, cd, ci, ci, cd, ci, vp]),
         "sun_multihit": (ci, [vp, vp, u64, ci, vp]),
         "sun_wilson_loops": (ci, [vp, ci, ci, dp, vp]),
         "sun_links_to_host": (ci, [vp, dp, ci]),
@@ -351,6 +352,11 @@ class Lattice:
 
     def stout_smear(self, rho: float, steps: int, time_dir: bool = False, stream=None):
         _check(lib().sun_stout_smear(self.h, rho, steps, int(time_dir), stream), "sun_stout_smear")
+
+    def hyp_smear(self, alphas=(0.75, 0.6, 0.3), steps: int = 1, nhit: int = 10, tol: float = 0.0,
+                  time_dir: bool = False, stream=None):
+        a1, a2, a3 = alphas
+        _check(lib().sun_hyp_smear(self.h, a1, a2, a3, steps, nhit, tol, int(time_dir), stream), "sun_hyp_smear")
 
     def multihit(self, out: "Lattice", seed: int, nhit: int, stream=None):
         """out (18-real, same N / extents / precision) <- this field with multihit temporal links"""

@@ -1,5 +1,6 @@
 // sun_meas.hip -- the reference's measurement chain on a resident lattice (host
 // code): APE smearing (smear/ape.cu:198-264), stout smearing (smear/stout.cu:687-766),
+// HYP smearing (smear/hyp.cu:277-368),
 // multihit of temporal links
 // (smear/multihit.cu:291-333) and Wilson loops (meas/wilsonloop.cu:494-516).
 // Scratch (the ping-pong field of APE, the Wilson lines and partial sums) is
@@ -76,6 +77,37 @@ extern "C" int sun_stout_smear(sun_lattice* lat, double rho, int steps, int time
         const Field& dst = (st & 1) ? lat->f : other;
         for (int mu = time_dir ? 3 : 0; mu < (time_dir ? 4 : 3); mu++)
             HIP_TRY(lat->ops->stout(src, dst, lat->g, mu, rho, s));
+    }
+    if (steps & 1) HIP_TRY(hipMemcpyAsync(lat->buf, tmp.p, lat->bytes, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SUN_OK;
+}
+
+extern "C" int sun_hyp_smear(sun_lattice* lat, double alpha1, double alpha2, double alpha3, int steps, int nhit,
+                             double tol, int time_dir, void* stream) {
+    SUN_TRY(check_lattice(lat, "sun_hyp_smear"));
+    if (steps < 0 || nhit < 0 || (time_dir != 0 && time_dir != 1) || !(alpha1 >= 0.0) || !(alpha2 >= 0.0) ||
+        !(alpha3 >= 0.0))
+        return fail(SUN_ERR_INVALID, "sun_hyp_smear: steps %d, nhit %d, alphas %g %g %g, time_dir %d", steps, nhit,
+                    alpha1, alpha2, alpha3, time_dir);
+    if (steps == 0) return SUN_OK;
+    hipStream_t s = pick_stream(lat, stream);
+    HIP_TRY(hipSetDevice(lat->ctx->device));
+    const size_t full = (size_t)2 * lat->d.nc * lat->d.nc * (size_t)lat->d.precision;  // 18-real record
+    const size_t nlev = 12 * (size_t)lat->g.V;
+    DeviceScratch tmp, bar, til;
+    HIP_TRY(hipMalloc(&tmp.p, lat->bytes));
+    HIP_TRY(hipMalloc(&bar.p, nlev * full));
+    HIP_TRY(hipMalloc(&til.p, nlev * full));
+    const Field fb = make_field(bar.p, nlev, full), ft = make_field(til.p, nlev, full);
+    // the ping-pong of sun_ape_smear (hyp.cu:291-313): each step reads the previous field
+    HIP_TRY(hipMemcpyAsync(tmp.p, lat->buf, lat->bytes, hipMemcpyDeviceToDevice, s));
+    const Field other = make_field(tmp.p, 4 * (size_t)lat->g.V, link_bytes(&lat->d));
+    for (int st = 0; st < steps; st++) {
+        const Field& src = (st & 1) ? other : lat->f;
+        const Field& dst = (st & 1) ? lat->f : other;
+        HIP_TRY(lat->ops->hyp(src, fb, ft, dst, lat->g, time_dir ? 3 : 0, time_dir ? 4 : 3, alpha1, alpha2, alpha3,
+                              nhit, tol, s));
     }
     if (steps & 1) HIP_TRY(hipMemcpyAsync(lat->buf, tmp.p, lat->bytes, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));

@@ -1,6 +1,7 @@
 """The measurement chain on the GPU against the FP64 oracle: APE smearing
-(ape.cu), stout smearing (stout.cu, SU(3) variants), multihit (multihit.cu) and
-Wilson loops (wilsonloop.cu), every variant.
+(ape.cu), stout smearing (stout.cu, SU(3) variants), HYP smearing (hyp.cu; the
+device computes each decoration level once, the oracle recurses as the reference
+does), multihit (multihit.cu) and Wilson loops (wilsonloop.cu), every variant.
 
 Bars: FP64 1e-12 per link (smearing, multihit) and 1e-13 on W(r, T); FP32 5e-5 per
 link plus four times the storage round trip of the oracle's result (8-real FP32
@@ -56,6 +57,25 @@ def test_stout_parity(ctx, oracle, nc, storage, prec, time_dir):
     ref = oracle.stout_smear(U.copy(), DIMS, 0.15, 3, time_dir)
     d = per_link_diff(got, ref)
     bar = tol(prec) + 4 * per_link_diff(stored(ctx, nc, storage, prec, ref), ref)
+    assert np.all(d <= bar), (float(d.max()), float(bar.max()))
+    untouched = slice(0, 3) if time_dir else slice(3, 4)
+    assert np.array_equal(got[untouched], U[untouched])
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+@pytest.mark.parametrize("time_dir", [False, True], ids=["space", "time"])
+def test_hyp_parity(ctx, oracle, nc, storage, prec, time_dir):
+    """the level-by-level device HYP against the oracle's recursion (hyp.cu)"""
+    dims = (4, 4, 4, 6)
+    with sun_amd.Lattice(ctx, nc, dims, BETAS[nc], storage, prec) as lat:
+        U = load_config(lat, thermalised(oracle, nc, dims, storage))
+        lat.hyp_smear((0.75, 0.6, 0.3), 1, 10, 1e-10, time_dir)
+        got = lat.to_host()
+        ref = oracle.hyp_smear(U.copy(), dims, (0.75, 0.6, 0.3), 1, 10, 1e-10, time_dir)
+        lat.from_host(ref)
+        stored_ref = lat.to_host()
+    d = per_link_diff(got, ref)
+    bar = tol(prec) + 4 * per_link_diff(stored_ref, ref)
     assert np.all(d <= bar), (float(d.max()), float(bar.max()))
     untouched = slice(0, 3) if time_dir else slice(3, 4)
     assert np.array_equal(got[untouched], U[untouched])

@@ -369,3 +369,82 @@ def spatial_staple(U, dims, mu, idd):
         S += Lnu_x @ shift(L[mu], nu, 1)[found] @ dag(shift(L[nu], mu, 1)[found])
         S += dag(shift(L[nu], nu, -1)[found]) @ shift(L[mu], nu, -1)[found] @ shift(shift(L[nu], nu, -1), mu, 1)[found]
     return S
+
+
+# ---- HYP smearing (hyp.cu): the oracle's recursion against a level-by-level numpy SU(2) ----
+def proj_field_su2(M):
+    P = np.empty_like(M)
+    for idx in np.ndindex(M.shape[:-2]):
+        P[idx] = quat_project_su2(M[idx])
+    return P
+
+
+def hyp_numpy_su2(L, a1, a2, a3, mus):
+    """one HYP step of the directions mus on a lexicographic SU(2) field; with one hit
+    project_link is the closed-form projection whatever its start"""
+    def staple(A_mu, A_nu, mu, nu):  # A_nu(x) A_mu(x+nu) A_nu(x+mu)^+ + down
+        up = A_nu @ shift(A_mu, nu, 1) @ dag(shift(A_nu, mu, 1))
+        dn = dag(shift(A_nu, nu, -1)) @ shift(A_mu, nu, -1) @ shift(shift(A_nu, nu, -1), mu, 1)
+        return up + dn
+    bar = {}
+    for mu in range(4):
+        for eta in range(4):
+            if eta != mu:
+                bar[mu, eta] = proj_field_su2(0.5 * a3 * staple(L[mu], L[eta], mu, eta) + (1 - a3) * L[mu])
+    til = {}
+    for mu in range(4):
+        for nu in range(4):
+            if nu == mu:
+                continue
+            s = np.zeros_like(L[mu])
+            for rho in range(4):
+                if rho in (mu, nu):
+                    continue
+                sig = 6 - mu - nu - rho
+                s += staple(bar[mu, sig], bar[rho, sig], mu, rho)
+            til[mu, nu] = proj_field_su2(0.25 * a2 * s + (1 - a2) * L[mu])
+    out = L.copy()
+    for mu in mus:
+        s = np.zeros_like(L[mu])
+        for nu in range(4):
+            if nu != mu:
+                s += staple(til[mu, nu], til[nu, mu], mu, nu)
+        out[mu] = proj_field_su2(a1 / 6 * s + (1 - a1) * L[mu])
+    return out
+
+
+@pytest.mark.parametrize("time_dir", [False, True])
+def test_hyp_su2_matches_level_by_level_numpy(oracle, time_dir):
+    dims = (4, 4, 4, 4)
+    U = oracle.hot_start(2, dims, 40)
+    for it in range(2):
+        oracle.heatbath_sweep(U, dims, 2.3, 41, it)
+    ref = from_lex(hyp_numpy_su2(to_lex(U, dims), 0.75, 0.6, 0.3, [3] if time_dir else [0, 1, 2]), dims)
+    got = oracle.hyp_smear(U.copy(), dims, (0.75, 0.6, 0.3), 1, 1, 0.0, time_dir)
+    assert np.abs(got - ref).max() < 1e-13
+
+
+def test_hyp_su2_gauge_covariant(oracle):
+    dims = (4, 4, 4, 4)
+    rng = np.random.default_rng(42)
+    U = oracle.hot_start(2, dims, 43)
+    G = haar(rng, 2, (256,))
+    a = oracle.hyp_smear(gauge_transform(U, dims, G), dims, (0.75, 0.6, 0.3), 2, 1)
+    b = gauge_transform(oracle.hyp_smear(U.copy(), dims, (0.75, 0.6, 0.3), 2, 1), dims, G)
+    assert np.abs(a - b).max() < 1e-12
+
+
+@pytest.mark.parametrize("nc", NCS)
+def test_hyp_fixed_points_and_smoothing(oracle, nc):
+    dims = (4, 4, 4, 4)
+    C = oracle.cold_start(nc, dims)
+    assert np.abs(oracle.hyp_smear(C.copy(), dims, (0.75, 0.6, 0.3), 1, 10, 1e-10) - C).max() < 1e-14
+    U = oracle.hot_start(nc, dims, 44)
+    for it in range(3):
+        oracle.heatbath_sweep(U, dims, {2: 2.3, 3: 6.0, 4: 11.0}[nc], 45, it)
+    assert np.abs(oracle.hyp_smear(U.copy(), dims, (0.0, 0.0, 0.0), 1, 10, 1e-10) - U).max() < 1e-13
+    S = oracle.hyp_smear(U.copy(), dims, (0.75, 0.6, 0.3), 1, 10, 1e-10)
+    assert np.array_equal(S[3], U[3])
+    assert oracle.plaquette(S, dims)[0] > oracle.plaquette(U, dims)[0] + 0.05
+    L = S.reshape(-1, nc, nc)
+    assert np.abs(L @ np.conj(np.swapaxes(L, 1, 2)) - np.eye(nc)).max() < 1e-12
