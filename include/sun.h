@@ -180,6 +180,11 @@ int sun_hyp_smear(sun_lattice *lat, double alpha1, double alpha2, double alpha3,
  * stream (seed, site, hit); out: another lattice of the same N, extents and precision,
  * 18-real storage (the mean is not in SU(N)). */
 int sun_multihit(sun_lattice *in, sun_lattice *out, uint64_t seed, int nhit, void *stream);
+/* ApplyMultiHitSpace (multihitsp.cu:246-266): out = in with every spatial link replaced
+ * by the mean of nhit >= 1 successive overrelaxation steps in its spatial staple (no
+ * random numbers: that kernel's heatbath call is commented out); temporal links
+ * copied from in (the reference leaves out's).  out as for sun_multihit. */
+int sun_multihit_space(sun_lattice *in, sun_lattice *out, int nhit, void *stream);
 /* WilsonLoop (wilsonloop.cu:494-516): out[2 (T + r (tmax+1)) + {0,1}] = Re, Im of
  * W(r, T) = sum_{x, mu<3} Tr(L_r(x) t1 L_r(x+T e_3)^+ t0^+) / (3 N V) for r <= rmax,
  * T <= tmax (rmax at most the smallest spatial extent, tmax at most the temporal). */

@@ -373,6 +373,12 @@ template <class Real>
 void ApplyMultiHit(_gauge<Real>& in, _gauge<Real>& out, RNG& rng, int nhit) {
     check(sun_multihit(in.handle(), out.handle(), rng.Seed(), nhit, nullptr), "ApplyMultiHit");
 }
+/* ApplyMultiHitSpace(in, out, RNG, nhit)   multihitsp.cu:246-266 (the RNG is unused) */
+template <class Real>
+void ApplyMultiHitSpace(_gauge<Real>& in, _gauge<Real>& out, RNG& rng, int nhit) {
+    (void)rng;
+    check(sun_multihit_space(in.handle(), out.handle(), nhit, nullptr), "ApplyMultiHitSpace");
+}
 /* WilsonLoop<Real>(gauge, res, Rmax, Tmax)   wilsonloop.cu:494-516:
    res[it + r * (Tmax + 1)] = W(r, it), r <= Rmax, it <= Tmax */
 template <class Real>

@@ -75,6 +75,7 @@ def lib() -> ctypes.CDLL:
         "or_multihit": (ci, [ci, ip, cd, u64, ci, dp, dp]),
         "or_exp_iq": (None, [dp, dp]),
         "or_stout_smear": (ci, [ip, dp, cd, ci, ci]),
+        "or_multihit_space": (ci, [ci, ip, ci, dp, dp]),
         "or_hyp_smear": (ci, [ci, ip, dp, cd, cd, cd, ci, ci, cd, ci]),
         "or_wilson_loops": (ci, [ci, ip, dp, ci, ci, dp]),
         "or_matmul": (None, [ci, dp, dp, dp]),
@@ -329,6 +330,12 @@ def hyp_smear(U, dims, alphas=(0.75, 0.6, 0.3), steps: int = 1, nhit: int = 10, 
 def multihit(U, dims, beta: float, seed: int, nhit: int):
     out = np.empty_like(U)
     _check(lib().or_multihit(U.shape[-1], _dims(dims), beta, seed, nhit, _dp(U), _dp(out)), "multihit")
+    return out
+
+
+def multihit_space(U, dims, nhit: int):
+    out = np.empty_like(U)
+    _check(lib().or_multihit_space(U.shape[-1], _dims(dims), nhit, _dp(U), _dp(out)), "multihit_space")
     return out
 
 

@@ -1320,6 +1320,36 @@ int or_multihit(int nc, const int X[4], double beta, uint64_t seed, int nhit, co
     return 0;
 }
 
+/* multihitsp.cu:115-161: spatial links -> mean of nhit successive overrelaxation
+ * steps in the spatial staple; temporal links copied */
+int or_multihit_space(int nc, const int X[4], int nhit, const double *Uin, double *Uout) {
+    int rc = check_args(nc, X, OR_STORE_18);
+    if (rc) return rc;
+    if (nhit < 1) return -5;
+    long V = volume_of(X), half = V / 2;
+    memcpy(Uout, Uin, sizeof(double) * (size_t)(4 * V * nc * nc * 2));
+#pragma omp parallel for schedule(static)
+    for (long idd = 0; idd < V; idd++) {
+        int parity = idd >= half ? 1 : 0;
+        int id = (int)(idd - parity * half);
+        for (int mu = 0; mu < 3; mu++) {
+            mat S = staple_range(nc, X, Uin, parity, mu, id, 3);
+            mat F = mdag(nc, &S);
+            mat u, acc;
+            mload(nc, link_at(nc, Uin, V, mu, idd), &u);
+            mzero(nc, &acc);
+            for (int hit = 0; hit < nhit; hit++) {
+                overrelax_mat(nc, &u, &F);
+                madd_to(nc, &acc, &u);
+            }
+            for (int i = 0; i < nc; i++)
+                for (int j = 0; j < nc; j++) acc.e[i][j] = cmk(acc.e[i][j].re / nhit, acc.e[i][j].im / nhit);
+            mstore(nc, &acc, link_at_w(nc, Uout, V, mu, idd));
+        }
+    }
+    return 0;
+}
+
 /* wilsonloop.cu:44-121,494-516 (WilsonLineSP + WilsonLoopSP): W(r, T) = sum over x and
  * mu < 3 of Tr(L_r(x) t1 L_r(x + T t)^+ t0^+) / (3 N V), L_r(x) the product of r links
  * along mu from x, t0 (t1) the product of T temporal links from x (x + r mu).

@@ -113,6 +113,9 @@ int or_hyp_smear(int nc, const int dims[4], double *U, double a1, double a2, dou
 /* multihit.cu:112-147: Uout = Uin with each temporal link replaced by the mean of
  * itself and nhit heatbath updates in its staple (stream (seed, id, hit, TAG_MH|p<<4|3)) */
 int or_multihit(int nc, const int dims[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout);
+/* multihitsp.cu:115-161: Uout = Uin with every spatial link replaced by the mean of
+ * nhit successive overrelaxation steps in its spatial staple */
+int or_multihit_space(int nc, const int dims[4], int nhit, const double *Uin, double *Uout);
 /* wilsonloop.cu:44-121,494-516: out[2(T + r(tmax+1)) + {0,1}] = W(r, T), r <= rmax,
  * T <= tmax, averaged over sites and the three spatial directions, / (3 N V) */
 int or_wilson_loops(int nc, const int dims[4], const double *U, int rmax, int tmax, double *out);

@@ -1,6 +1,6 @@
 // sun_meas.hip -- the reference's measurement chain on a resident lattice (host
 // code): APE smearing (smear/ape.cu:198-264), stout smearing (smear/stout.cu:687-766),
-// HYP smearing (smear/hyp.cu:277-368),
+// HYP smearing (smear/hyp.cu:277-368), spatial multihit (smear/multihitsp.cu:246-266),
 // multihit of temporal links
 // (smear/multihit.cu:291-333) and Wilson loops (meas/wilsonloop.cu:494-516).
 // Scratch (the ping-pong field of APE, the Wilson lines and partial sums) is
@@ -114,18 +114,35 @@ extern "C" int sun_hyp_smear(sun_lattice* lat, double alpha1, double alpha2, dou
     return SUN_OK;
 }
 
-extern "C" int sun_multihit(sun_lattice* in, sun_lattice* out, uint64_t seed, int nhit, void* stream) {
-    SUN_TRY(check_lattice(in, "sun_multihit"));
-    SUN_TRY(check_lattice(out, "sun_multihit"));
-    if (in == out) return fail(SUN_ERR_INVALID, "sun_multihit: the output must be another lattice");
-    if (nhit < 0) return fail(SUN_ERR_INVALID, "sun_multihit: nhit %d", nhit);
+// the output of a multihit: another lattice, same context, N, precision and extents,
+// 18-real storage (a mean of SU(N) matrices is not in SU(N))
+static int check_average_output(const sun_lattice* in, const sun_lattice* out, const char* what) {
+    SUN_TRY(check_lattice(in, what));
+    SUN_TRY(check_lattice(out, what));
+    if (in == out) return fail(SUN_ERR_INVALID, "%s: the output must be another lattice", what);
     if (out->d.storage != 18)
-        return fail(SUN_ERR_INVALID, "sun_multihit: the output lattice must use 18-real storage (the averaged "
-                                     "links are not in SU(N))");
+        return fail(SUN_ERR_INVALID, "%s: the output lattice must use 18-real storage (the averaged links are not "
+                                     "in SU(N))", what);
     if (in->ctx != out->ctx || in->d.nc != out->d.nc || in->d.precision != out->d.precision)
-        return fail(SUN_ERR_INVALID, "sun_multihit: input and output differ in context, N or precision");
+        return fail(SUN_ERR_INVALID, "%s: input and output differ in context, N or precision", what);
     for (int i = 0; i < 4; i++)
-        if (in->d.dims[i] != out->d.dims[i]) return fail(SUN_ERR_INVALID, "sun_multihit: the extents differ");
+        if (in->d.dims[i] != out->d.dims[i]) return fail(SUN_ERR_INVALID, "%s: the extents differ", what);
+    return SUN_OK;
+}
+
+extern "C" int sun_multihit_space(sun_lattice* in, sun_lattice* out, int nhit, void* stream) {
+    SUN_TRY(check_average_output(in, out, "sun_multihit_space"));
+    if (nhit < 1) return fail(SUN_ERR_INVALID, "sun_multihit_space: nhit %d (the mean needs at least one)", nhit);
+    hipStream_t s = pick_stream(in, stream);
+    HIP_TRY(hipSetDevice(in->ctx->device));
+    HIP_TRY(in->ops->multihit_space(in->f, out->f, in->g, nhit, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SUN_OK;
+}
+
+extern "C" int sun_multihit(sun_lattice* in, sun_lattice* out, uint64_t seed, int nhit, void* stream) {
+    SUN_TRY(check_average_output(in, out, "sun_multihit"));
+    if (nhit < 0) return fail(SUN_ERR_INVALID, "sun_multihit: nhit %d", nhit);
     hipStream_t s = pick_stream(in, stream);
     HIP_TRY(hipSetDevice(in->ctx->device));
     HIP_TRY(in->ops->multihit(in->f, out->f, in->g, seed, nhit, in->d.beta / (double)in->d.nc, s));

@@ -126,6 +126,8 @@ struct OpsTable {
                       int mu_hi, double a1, double a2, double a3, int nhit, double tol, hipStream_t);
     hipError_t (*multihit)(const Field& in, const Field& out, const Geom&, uint64_t seed, int nhit, double bnc,
                            hipStream_t);
+    // spatial links -> mean of nhit overrelaxation steps, temporal copied (out 18-real)
+    hipError_t (*multihit_space)(const Field& in, const Field& out, const Geom&, int nhit, hipStream_t);
     hipError_t (*wl_lines)(const Field& f, const Field& lines, const Geom&, int r, hipStream_t);
     hipError_t (*wl_loops)(const Field& f, const Field& lines, const Geom&, int r, int tmax, double2* partials,
                            hipStream_t);

@@ -109,6 +109,21 @@ def test_multihit_parity(ctx, oracle, nc, storage, prec):
 
 
 @pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+def test_multihit_space_parity(ctx, oracle, nc, storage, prec):
+    with sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], storage, prec) as lat, \
+            sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], 18, prec) as out:
+        U = load_config(lat, thermalised(oracle, nc, DIMS, storage))
+        lat.multihit_space(out, 4)
+        got = out.to_host()
+        with pytest.raises(sun_amd.SunError):
+            lat.multihit_space(out, 0)
+    ref = oracle.multihit_space(U, DIMS, 4)
+    assert np.abs(got[3] - U[3]).max() <= (0.0 if prec == 8 else store_tol(storage, prec))
+    d = per_link_diff(got[:3], ref[:3])
+    assert d.max() <= tol(prec), float(d.max())
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
 def test_wilson_loops_parity(ctx, oracle, nc, storage, prec):
     rmax, tmax = 3, 5
     with sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], storage, prec) as lat:

@@ -448,3 +448,26 @@ def test_hyp_fixed_points_and_smoothing(oracle, nc):
     assert oracle.plaquette(S, dims)[0] > oracle.plaquette(U, dims)[0] + 0.05
     L = S.reshape(-1, nc, nc)
     assert np.abs(L @ np.conj(np.swapaxes(L, 1, 2)) - np.eye(nc)).max() < 1e-12
+
+
+# ---- spatial multihit (multihitsp.cu): mean of successive overrelaxation steps ----
+def test_multihit_space_closed_forms(oracle):
+    dims = (4, 4, 4, 4)
+    U = oracle.hot_start(3, dims, 50)
+    M1 = oracle.multihit_space(U, dims, 1)
+    assert np.array_equal(M1[3], U[3])
+    for idd in (0, 77, 200):
+        for mu in range(3):
+            F = spatial_staple(U, dims, mu, idd).conj().T
+            # the numpy staple sums in another order: rounding, amplified by the
+            # reflection on a Haar field
+            assert np.abs(M1[mu, idd] - oracle.overrelax_link(U[mu, idd].copy(), F)).max() < 1e-13
+    # SU(2) overrelaxation is a reflection: two steps return the link, so every even
+    # nhit gives (OR(U) + U) / 2
+    W = oracle.hot_start(2, dims, 51)
+    O1 = oracle.multihit_space(W, dims, 1)
+    for n in (2, 4):
+        Mn = oracle.multihit_space(W, dims, n)
+        assert np.abs(Mn[:3] - (O1[:3] + W[:3]) / 2).max() < 1e-14
+    with pytest.raises(ValueError):
+        oracle.multihit_space(W, dims, 0)
