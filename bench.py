@@ -93,10 +93,34 @@ def cpu_baseline(min_seconds: float = 10.0, max_iters: int = 20):
         n += 1
     dt = time.perf_counter() - t0
     V = int(np.prod(DIMS))
+    # one thread, on a 16^4 lattice (the same per-site work; a 32^4 iteration would
+    # take ~20 s on one core)
+    o.set_threads(1)
+    small = (16, 16, 16, 16)
+    W = o.cold_start(3, small)
+    o.heatbath_sweep(W, small, BETA, 2, 0)
+    n1 = 0
+    t1 = time.perf_counter()
+    while n1 < 10 and (n1 == 0 or time.perf_counter() - t1 < 3.0):
+        o.heatbath_sweep(W, small, BETA, 2, 1 + n1)
+        for _ in range(4):
+            o.overrelax_sweep(W, small)
+        n1 += 1
+    dt1 = time.perf_counter() - t1
+    o.set_threads(cores)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": n * 5 * 4 * V / dt, "unit": "link-updates/s", "cores": cores, "kind": "port",
             "sample": f"FP64 oracle (C, OpenMP, {cores} threads): {n} iterations of 1 heatbath + 4 "
                       f"overrelaxation sweeps of SU(3) 32^4 beta=6.0 (after one heatbath sweep from cold), "
-                      f"{dt:.1f} s"}
+                      f"{dt:.1f} s",
+            "single_thread": {"value": n1 * 5 * 4 * int(np.prod(small)) / dt1, "unit": "link-updates/s",
+                              "sample": f"{n1} iterations of SU(3) 16^4 on one thread, {dt1:.1f} s"},
+            "host": {"cpu_model": model, "nproc": os.cpu_count()}}
 
 
 def batch_means(x, nblocks=10):
@@ -220,8 +244,8 @@ def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
     """Poisson arrivals of ensemble jobs (SU(2)/SU(3)/SU(4), volumes 8^4..32^4, beta in
     the usual range per N, 50..500 iterations of 1 HB + 4 OR + reunit + plaquette +
     Polyakov, FP32 links, FP64 reductions) into the job server's background loop.
-    The arrival rate targets `util` of the device from a bandwidth estimate of each
-    job's work.  Latency = completion - submission."""
+    The arrival rate targets `util` of the device from each job's service time, modelled
+    by a measured rate per (N, volume).  Latency = completion - submission."""
     rng = np.random.default_rng(5)
     jobs = []
     for j in range(n_jobs):
@@ -231,9 +255,21 @@ def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
         storage = 12 if nc == 3 else 18
         jobs.append(dict(nc=nc, dims=dims, beta=float(rng.uniform(lo, hi)), storage=storage, precision=4,
                          seed=5000 + j, hot_start=bool(rng.integers(0, 2)), n_iter=int(rng.integers(50, 501))))
-    # estimated service time at half of peak bandwidth: 9 link transfers per update
-    est = [j["n_iter"] * 5 * 4 * int(np.prod(j["dims"])) * 9 * link_bytes(j["nc"], j["storage"], 4)
-           / (0.5 * PEAK_HBM_GBS * 1e9) for j in jobs]
+    # service-time model measured on this device: one 20-iteration job of every (N,
+    # volume) class run alone through a server; the arrival rate then targets `util`
+    # of the device if the jobs ran one at a time (batching makes it less)
+    calib = {}
+    with sun_amd.Server(ctx, budget, max_jobs=16) as cs:
+        cs.start()
+        for nc in (2, 3, 4):
+            for dims in SERVE_VOLUMES:
+                jid = cs.submit(nc, dims, float(np.mean(SERVE_BETA[nc])), 1, 20, n_or=4, reunit_every=1,
+                                storage=12 if nc == 3 else 18, precision=4, hot_start=False)
+                cs.wait(jid, 120.0)
+                st = cs.status(jid)
+                calib[(nc, dims)] = 20 * 5 * 4 * int(np.prod(dims)) / max(st.t_end - st.t_start, 1e-6)
+        cs.stop()
+    est = [j["n_iter"] * 5 * 4 * int(np.prod(j["dims"])) / calib[(j["nc"], j["dims"])] for j in jobs]
     rate = util / float(np.mean(est))
     gaps = rng.exponential(1.0 / rate, n_jobs)
     updates = sum(j["n_iter"] * 5 * 4 * int(np.prod(j["dims"])) for j in jobs)
@@ -258,10 +294,13 @@ def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
     queue = np.array([s.t_start - s.t_submit for s in st])
     span = max(s.t_end for s in st) - min(s.t_submit for s in st)
     return {"value": updates / span, "unit": "link-updates/s", "jobs": n_jobs,
+            "busy_fraction": counters.busy_s / span,
+            "service_model": {f"su{nc}_{dims[0]}^4": round(r / 1e9, 3) for (nc, dims), r in calib.items()},
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "p50_queue_s": float(np.percentile(queue, 50)), "p99_queue_s": float(np.percentile(queue, 99)),
             "span_s": span, "arrival_rate_per_s": rate, "busy_s": counters.busy_s, "steps": int(counters.steps),
-            "config": "Poisson arrivals (numpy seed 5) at ~75% estimated load; N in {2,3,4}, volumes "
+            "config": "Poisson arrivals (numpy seed 5) at ~75% load of a measured service-time model "
+                      "(service_model: 1e9 link-updates/s per N and volume, one job alone); N in {2,3,4}, volumes "
                       "{8^4,12^4,16^4,24^4,32^4}, beta SU(2) [2.2,2.6], SU(3) [5.6,6.2], SU(4) [10.5,11.2], "
                       "50..500 iterations of HB + 4 OR + reunit + plaquette + Polyakov, FP32 links (SU(3) "
                       "12-real), FP64 reductions, hot/cold 50/50, 64 GiB budget"}
