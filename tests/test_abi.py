@@ -38,3 +38,15 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(sun_amd, "_lib", None)
     with pytest.raises(sun_amd.SunError, match="missing"):
         sun_amd.lib()
+
+
+def test_facade_templates_compile(tmp_path):
+    """every class and function template of include/sun.hpp instantiates (both
+    precisions) and links against libsun_hip.so"""
+    import sun_amd
+    src = os.path.join(ROOT, "tests", "host", "facade_all.cpp")
+    obj = str(tmp_path / "facade_all.so")
+    r = subprocess.run(["g++", "-std=c++17", "-shared", "-fPIC", "-Wall", "-Werror", f"-I{ROOT}/include", src,
+                        "-o", obj, f"-L{os.path.dirname(sun_amd.LIB_PATH)}", "-lsun_hip"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
