@@ -94,12 +94,21 @@ extern "C" int sun_hyp_smear(sun_lattice* lat, double alpha1, double alpha2, dou
     hipStream_t s = pick_stream(lat, stream);
     HIP_TRY(hipSetDevice(lat->ctx->device));
     const size_t full = (size_t)2 * lat->d.nc * lat->d.nc * (size_t)lat->d.precision;  // 18-real record
-    const size_t nlev = 12 * (size_t)lat->g.V;
-    DeviceScratch tmp, bar, til;
+    const size_t one = (size_t)lat->g.V * full;  // one pair's field
+    DeviceScratch tmp, bar, til, tab;
     HIP_TRY(hipMalloc(&tmp.p, lat->bytes));
-    HIP_TRY(hipMalloc(&bar.p, nlev * full));
-    HIP_TRY(hipMalloc(&til.p, nlev * full));
-    const Field fb = make_field(bar.p, nlev, full), ft = make_field(til.p, nlev, full);
+    HIP_TRY(hipMalloc(&bar.p, 12 * one));
+    HIP_TRY(hipMalloc(&til.p, 12 * one));
+    // 12 separate V-link fields per level (site-indexed plane views), as a device table
+    Field views[24];
+    for (int p = 0; p < 12; p++) {
+        views[p] = make_field((char*)bar.p + p * one, (size_t)lat->g.V, full);
+        views[12 + p] = make_field((char*)til.p + p * one, (size_t)lat->g.V, full);
+    }
+    HIP_TRY(hipMalloc(&tab.p, sizeof(views)));
+    HIP_TRY(hipMemcpy(tab.p, views, sizeof(views), hipMemcpyHostToDevice));
+    const Field* fb = (const Field*)tab.p;
+    const Field* ft = fb + 12;
     // the ping-pong of sun_ape_smear (hyp.cu:291-313): each step reads the previous field
     HIP_TRY(hipMemcpyAsync(tmp.p, lat->buf, lat->bytes, hipMemcpyDeviceToDevice, s));
     const Field other = make_field(tmp.p, 4 * (size_t)lat->g.V, link_bytes(&lat->d));

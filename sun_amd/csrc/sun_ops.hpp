@@ -120,9 +120,9 @@ struct OpsTable {
                       hipStream_t);
     // one stout direction in -> out (SU(3) variants; null otherwise)
     hipError_t (*stout)(const Field& in, const Field& out, const Geom&, int mu, double rho, hipStream_t);
-    // HYP: level-1 and level-2 fields (18-real, 12 V entries each) from in, then the
-    // directions [mu_lo, mu_hi) of out
-    hipError_t (*hyp)(const Field& in, const Field& bar, const Field& til, const Field& out, const Geom&, int mu_lo,
+    // HYP: level-1 and level-2 fields (device tables of 12 plane views of V 18-real
+    // links each) from in, then the directions [mu_lo, mu_hi) of out
+    hipError_t (*hyp)(const Field& in, const Field* bar, const Field* til, const Field& out, const Geom&, int mu_lo,
                       int mu_hi, double a1, double a2, double a3, int nhit, double tol, hipStream_t);
     hipError_t (*multihit)(const Field& in, const Field& out, const Geom&, uint64_t seed, int nhit, double bnc,
                            hipStream_t);

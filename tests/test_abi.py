@@ -50,3 +50,15 @@ def test_facade_templates_compile(tmp_path):
                         "-o", obj, f"-L{os.path.dirname(sun_amd.LIB_PATH)}", "-lsun_hip"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_descriptor_limits():
+    """the largest accepted volume (4V x 16-byte records addressable by 32-bit plane
+    offsets: V < 2^26, e.g. 90^4) and the rejected shapes; host-only calls"""
+    import sun_amd
+    assert sun_amd.lattice_bytes(3, (90, 90, 90, 90), 12, 4) == 4 * 90 ** 4 * 48
+    assert sun_amd.lattice_bytes(3, (92, 92, 92, 92), 12, 4) == 0       # 4V x 16 B > 2^32
+    assert sun_amd.lattice_bytes(3, (2, 2, 2, 2), 18, 8) == 4 * 16 * 144  # the smallest lattice
+    for bad in [(3, (4, 4, 4, 5), 18, 8), (3, (0, 4, 4, 4), 18, 8), (5, (4, 4, 4, 4), 18, 8),
+                (2, (4, 4, 4, 4), 12, 4), (3, (4, 4, 4, 4), 10, 4), (3, (4, 4, 4, 4), 18, 2)]:
+        assert sun_amd.lattice_bytes(*bad) == 0, bad
