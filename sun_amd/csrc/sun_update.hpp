@@ -35,10 +35,14 @@ SUN_HD R qabs(const Quat<R>& r) {
     return sqrt(r.a[0] * r.a[0] + r.a[1] * r.a[1] + r.a[2] * r.a[2] + r.a[3] * r.a[3]);
 }
 
-// msu2.h:148-155
+// msu2.h:148-155 (FP32: the reciprocal square root is one v_rsq_f32)
 template <typename R>
 SUN_HD void conj_normalize(Quat<R>& r) {
-    R norm = R(1) / sqrt(r.a[0] * r.a[0] + r.a[1] * r.a[1] + r.a[2] * r.a[2] + r.a[3] * r.a[3]);
+    R norm;
+    if constexpr (std::is_same<R, float>::value)
+        norm = f32_rsq(r.a[0] * r.a[0] + r.a[1] * r.a[1] + r.a[2] * r.a[2] + r.a[3] * r.a[3]);
+    else
+        norm = R(1) / sqrt(r.a[0] * r.a[0] + r.a[1] * r.a[1] + r.a[2] * r.a[2] + r.a[3] * r.a[3]);
     r.a[0] *= norm;
     r.a[1] *= -norm;
     r.a[2] *= -norm;
