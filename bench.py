@@ -198,6 +198,38 @@ def large_leg(sun_amd, ctx, stream, n_iter=6, warm=2):
                           "plaquette + spatially averaged Polyakov loop (FP64 sums), hipGraph replay"}
 
 
+XGMI_LINK_GBS = 153.0  # one xGMI link of an MI355X, per direction
+
+
+def shard_leg(sun_amd, ctx, stream, n_iter=4, warm=2):
+    """One GPU's share of the 64^4 FP32-8 lattice split 8 ways: 64 x 64 x 8 x 64 (split
+    in x2, spatial, so Polyakov lines stay on one GPU) and 64^3 x 8 (split in t), run
+    periodic as a stand-in for the share.  A split lattice would exchange, after every
+    (parity, mu) launch, one plane of the just-updated U_mu links with each of its two
+    neighbours; the leg reports the local launch time beside that face's bytes and its
+    time on one xGMI link, i.e. how much interior work there is to hide the exchange."""
+    out = {}
+    for name, dims, d in (("x2_split_64x64x8x64", (64, 64, 8, 64), 2), ("t_split_64x64x64x8", (64, 64, 64, 8), 3)):
+        V = int(np.prod(dims))
+        with sun_amd.Lattice(ctx, 3, dims, 6.2, 8, 4) as lat:
+            lat.hot_start(3, stream)
+            lat.generate(seed=3, n_iter=warm, n_or=4, reunit_every=0, stream=stream)
+            sun_amd.sync(stream)
+            t0 = time.perf_counter()
+            lat.generate(seed=3, n_iter=n_iter, n_or=4, reunit_every=0, first_sweep=warm, stream=stream)
+            sun_amd.sync(stream)
+            dt = time.perf_counter() - t0
+            or_ms = time_launches(sun_amd, lat, stream, hb=False, reps=16)
+        face = V // dims[d] // 2 * 32  # one plane of U_mu at one parity, 8-real FP32
+        out[name] = {"lattice": list(dims), "ms_per_iteration": dt / n_iter * 1e3,
+                     "overrelax_launch_ms": or_ms, "face_bytes_per_launch": face,
+                     "xgmi_ms_per_face": face / (XGMI_LINK_GBS * 1e9) * 1e3,
+                     "exchange_to_launch": face / (XGMI_LINK_GBS * 1e9) * 1e3 / or_ms}
+    out["note"] = ("periodic local lattice as the share; the exchange (2 faces, one per neighbour link, "
+                   "both directions) and its overlap with interior sites are not run on one GPU")
+    return out
+
+
 BATCH_VOLUMES = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 16), (16, 16, 16, 32), (24, 24, 24, 24)]
 
 
@@ -476,6 +508,7 @@ def main():
                                                                   updates_per_step)
     if not args.no_large:
         out["large_64x4_f32_8"] = large_leg(sun_amd, ctx, stream)
+        out["shard_share_64x4_f32_8"] = shard_leg(sun_amd, ctx, stream)
     if not args.no_batch:
         out["batched"] = batched_leg(sun_amd, ctx, stream)
     if not args.no_serve:
