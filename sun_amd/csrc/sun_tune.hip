@@ -23,10 +23,15 @@ namespace sun {
 struct Shape {
     int block, b1, b2, b3;
 };
+// The x3-deep bricks (b3 = 2..4 with thin x1, x2) can span the whole slab of x3 planes
+// that one XCD's contiguous range of bricks covers (X3 / 8 planes), so that the x3
+// neighbours of a brick are its own rows rather than a brick ~n1 n2 positions later.
 static const Shape kShapes[] = {{64, -1, 0, 0},  {128, -1, 0, 0}, {256, -1, 0, 0}, {512, -1, 0, 0},
                                 {0, 1, 1, 1},    {0, 2, 0, 2},    {0, 2, 1, 1},    {0, 1, 1, 2},
                                 {0, 2, 2, 1},    {0, 2, 1, 2},    {0, 1, 2, 2},    {0, 2, 2, 2},
-                                {0, 3, 1, 2},    {0, 2, 1, 3}};
+                                {0, 3, 1, 2},    {0, 2, 1, 3},    {0, 0, 0, 2},    {0, 0, 0, 3},
+                                {0, 1, 0, 2},    {0, 0, 1, 2},    {0, 1, 0, 3},    {0, 0, 1, 3},
+                                {0, 0, 0, 4}};
 // dynamic LDS that no kernel reads: caps the workgroups per CU (160 KiB of LDS)
 static const int kLds[] = {0, 16384, 32768, 40960, 54272, 65536, 81920};
 

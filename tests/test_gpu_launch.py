@@ -33,6 +33,10 @@ SHAPES = [  # (block, brick, lds, piped)
     (256, (-1, 0, 0), 0, 2),
     (0, (2, 0, 2), 16384, 2),
     (0, (1, 1, 2), 0, 2),
+    (0, (0, 0, 2), 0, 0),      # x3-deep bricks
+    (0, (0, 0, 3), 0, 2),
+    (0, (1, 0, 3), 16384, 1),
+    (0, (0, 0, 4), 0, 0),
 ]
 DIMS = [(8, 8, 8, 16), (32, 4, 4, 8)]
 
