@@ -432,7 +432,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(lattices/jobs shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+              "(independent lattices and jobs run one process per GPU without communication)",
+              file=sys.stderr)
         sys.exit(2)
 
     import sun_amd

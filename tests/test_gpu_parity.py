@@ -1,6 +1,6 @@
 """GPU parity: the HIP engine, called through the C ABI, against the FP64 oracle.
 
-Tolerances (north star): link-by-link after one overrelaxation or heatbath sweep
+Tolerances: link-by-link after one overrelaxation or heatbath sweep
 <= 1e-12 in FP64 storage and <= 5e-5 in FP32 storage, on thermalised fields (the
 operating regime: oracle cold start + 12 heatbath/overrelaxation iterations).
 
