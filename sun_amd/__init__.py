@@ -47,7 +47,8 @@ EXPORTS = [
     "sun_ape_smear", "sun_stout_smear", "sun_hyp_smear", "sun_multihit", "sun_multihit_space", "sun_wilson_loops",
     "sun_links_to_host", "sun_links_from_host",
     "sun_generate",
-    "sun_batch_create", "sun_batch_destroy", "sun_batch_heatbath", "sun_batch_overrelax",
+    "sun_batch_create", "sun_batch_destroy", "sun_batch_get_launch", "sun_batch_set_launch",
+    "sun_batch_heatbath", "sun_batch_overrelax",
     "sun_batch_reunitarize", "sun_batch_plaquette", "sun_batch_generate",
     "sun_server_create", "sun_server_destroy", "sun_server_submit", "sun_server_cancel",
     "sun_server_step", "sun_server_start", "sun_server_stop", "sun_server_wait", "sun_server_status",
@@ -164,6 +165,8 @@ def lib() -> ctypes.CDLL:
         "sun_generate": (ci, [vp, ctypes.POINTER(GenerateDesc), vp, dp]),
         "sun_batch_create": (ci, [vp, ci, ctypes.POINTER(vp), ctypes.POINTER(u64), ctypes.POINTER(vp)]),
         "sun_batch_destroy": (ci, [vp]),
+        "sun_batch_get_launch": (ci, [vp, ci, ctypes.POINTER(ci), ctypes.POINTER(ci)]),
+        "sun_batch_set_launch": (ci, [vp, ci, ci, ci]),
         "sun_batch_heatbath": (ci, [vp, ctypes.POINTER(u32), vp]),
         "sun_batch_overrelax": (ci, [vp, vp]),
         "sun_batch_reunitarize": (ci, [vp, vp]),
@@ -410,6 +413,17 @@ class Batch:
         if self.h:
             _check(lib().sun_batch_destroy(self.h), "sun_batch_destroy")
             self.h = ctypes.c_void_p()
+
+    def get_launch(self, kernel: int):
+        """(piped, brick) of the heatbath (kernel 0) or overrelaxation (1) launches:
+        piped 0 plain, 1 / 2 direction / half-direction software-pipelined staple;
+        brick 1 the lattices' update bricks, 0 the linear site order."""
+        p, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib().sun_batch_get_launch(self.h, kernel, ctypes.byref(p), ctypes.byref(b)), "sun_batch_get_launch")
+        return p.value, b.value
+
+    def set_launch(self, kernel: int, piped: int, brick: int = 0):
+        _check(lib().sun_batch_set_launch(self.h, kernel, piped, brick), "sun_batch_set_launch")
 
     def heatbath(self, sweeps, stream=None):
         n = len(self.lattices)

@@ -267,6 +267,58 @@ SUN_HD void reconstruct8(Mat<R, 3>& m, R th0, R th1) {
     m.e[2][2] = scale(m.e[2][2], -p2);
 }
 
+// FP32: the same steps with every multiply-add an explicit fmaf.  Left to the
+// compiler, the products and sums above are fused into FMAs differently depending on
+// the code the decode is inlined into, and the kernels that decode the same link
+// (plain and software-pipelined staples) would round it differently.
+namespace r8f {
+SUN_HD cplx<float> mul(cplx<float> a, cplx<float> b) {  // a*b
+    return {fmaf(-a.im, b.im, a.re * b.re), fmaf(a.im, b.re, a.re * b.im)};
+}
+SUN_HD cplx<float> conj_mul(cplx<float> a, cplx<float> b) {  // conj(a*b)
+    return {fmaf(-a.im, b.im, a.re * b.re), -fmaf(a.im, b.re, a.re * b.im)};
+}
+SUN_HD cplx<float> mul_ca(cplx<float> a, cplx<float> b) {  // conj(a)*b
+    return {fmaf(a.im, b.im, a.re * b.re), fmaf(-a.im, b.re, a.re * b.im)};
+}
+SUN_HD cplx<float> add_mul(cplx<float> x, cplx<float> a, cplx<float> b) {  // x + a*b
+    return {fmaf(-a.im, b.im, fmaf(a.re, b.re, x.re)), fmaf(a.im, b.re, fmaf(a.re, b.im, x.im))};
+}
+SUN_HD cplx<float> sub_mul(cplx<float> x, cplx<float> a, cplx<float> b) {  // x - a*b
+    return {fmaf(a.im, b.im, fmaf(-a.re, b.re, x.re)), fmaf(-a.im, b.re, fmaf(-a.re, b.im, x.im))};
+}
+SUN_HD cplx<float> scaled(cplx<float> a, float s) { return {a.re * s, a.im * s}; }
+}  // namespace r8f
+
+SUN_HD void reconstruct8(Mat<float, 3>& m, float th0, float th1) {
+    using namespace r8f;
+    float p1 = m.e[0][1].re * m.e[0][1].re;
+    p1 = fmaf(m.e[0][1].im, m.e[0][1].im, p1);
+    p1 = fmaf(m.e[0][2].re, m.e[0][2].re, p1);
+    p1 = fmaf(m.e[0][2].im, m.e[0][2].im, p1);
+    const float p2 = decode_rcp(p1);
+    float diff = 1.f - p1;
+    p1 = decode_sqrt(diff >= 0.f ? diff : 0.f);
+    float c0, s0, c1, s1;
+    phase(th0, s0, c0);
+    phase(th1, s1, c1);
+    m.e[0][0].re = p1 * c0;
+    m.e[0][0].im = p1 * s0;
+    diff = fmaf(-m.e[0][0].re, m.e[0][0].re, 1.f);
+    diff = fmaf(-m.e[0][0].im, m.e[0][0].im, diff);
+    diff = fmaf(-m.e[1][0].re, m.e[1][0].re, diff);
+    diff = fmaf(-m.e[1][0].im, m.e[1][0].im, diff);
+    p1 = decode_sqrt(diff >= 0.f ? diff : 0.f);
+    m.e[2][0].re = p1 * c1;
+    m.e[2][0].im = p1 * s1;
+    const cplx<float> t22 = mul_ca(m.e[0][0], m.e[1][0]);
+    m.e[1][1] = scaled(add_mul(conj_mul(m.e[2][0], m.e[0][2]), t22, m.e[0][1]), -p2);
+    m.e[1][2] = scaled(sub_mul(conj_mul(m.e[2][0], m.e[0][1]), t22, m.e[0][2]), p2);
+    const cplx<float> temp = mul_ca(m.e[0][0], m.e[2][0]);
+    m.e[2][1] = scaled(sub_mul(conj_mul(m.e[1][0], m.e[0][2]), temp, m.e[0][1]), p2);
+    m.e[2][2] = scaled(add_mul(conj_mul(m.e[1][0], m.e[0][1]), temp, m.e[0][2]), -p2);
+}
+
 template <typename R, int N, int STORE>
 SUN_HD Mat<R, N> unpack(const R* r) {
     Mat<R, N> m;

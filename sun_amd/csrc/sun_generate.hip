@@ -155,6 +155,10 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
     if (!b) return fail(SUN_ERR_NOMEM, "host allocation");
     b->ctx = ctx;
     b->ops = ops;
+    for (int k = 0; k < 2; k++) {
+        b->piped[k] = default_batch_piped(ops, k);
+        b->brick[k] = default_batch_brick(ops, k);
+    }
     b->lats.assign(lats, lats + n);
     std::vector<BatchEntry> tu(n), tr(n), tp(n);
     int cu = 0, cr = 0, cp = 0;
@@ -167,6 +171,7 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
         e.seed = seeds ? seeds[i] : 0;
         e.bnc = L->d.beta / L->d.nc;
         e.sweep = 0;
+        batch_brick(L->g, e.ub1, e.ub2, e.ub3);
         BatchEntry a = e, r = e, p = e;
         a.first_block = cu;
         a.nblocks = update_blocks(L->g, 256);
@@ -216,6 +221,25 @@ extern "C" int sun_batch_destroy(sun_batch* b) {
     return SUN_OK;
 }
 
+extern "C" int sun_batch_get_launch(const sun_batch* b, int kernel, int* piped, int* brick) {
+    if (!b || !piped || !brick || (kernel != 0 && kernel != 1))
+        return fail(SUN_ERR_INVALID, "sun_batch_get_launch: bad argument");
+    *piped = b->piped[kernel];
+    *brick = b->brick[kernel];
+    return SUN_OK;
+}
+
+extern "C" int sun_batch_set_launch(sun_batch* b, int kernel, int piped, int brick) {
+    if (!b || (kernel != 0 && kernel != 1)) return fail(SUN_ERR_INVALID, "sun_batch_set_launch: bad argument");
+    if (piped < 0 || piped > 2) return fail(SUN_ERR_INVALID, "sun_batch_set_launch: piped %d (0, 1 or 2)", piped);
+    if (brick != 0 && brick != 1) return fail(SUN_ERR_INVALID, "sun_batch_set_launch: brick %d (0 or 1)", brick);
+    if (piped && !b->ops->has_piped)
+        return fail(SUN_ERR_INVALID, "sun_batch_set_launch: no pipelined kernel for this N/storage/precision");
+    b->piped[kernel] = piped;
+    b->brick[kernel] = brick;
+    return SUN_OK;
+}
+
 static hipStream_t batch_stream(sun_batch* b, void* stream) {
     return stream ? (hipStream_t)stream : b->ctx->stream;
 }
@@ -231,7 +255,8 @@ static int batch_set_sweeps(sun_batch* b, const uint32_t* sweeps, uint32_t unifo
 static int batch_hb(sun_batch* b, hipStream_t s) {
     const int n = (int)b->lats.size();
     for (int parity = 0; parity < 2; parity++)
-        for (int mu = 0; mu < 4; mu++) HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, true, s));
+        for (int mu = 0; mu < 4; mu++)
+            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, true, b->piped[0], b->brick[0], s));
     return SUN_OK;
 }
 
@@ -239,7 +264,7 @@ static int batch_or(sun_batch* b, hipStream_t s) {
     const int n = (int)b->lats.size();
     for (int parity = 0; parity < 2; parity++)
         for (int mu = 0; mu < 4; mu++)
-            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, false, s));
+            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, false, b->piped[1], b->brick[1], s));
     return SUN_OK;
 }
 

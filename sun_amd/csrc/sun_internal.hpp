@@ -115,6 +115,8 @@ struct sun_batch {
     sun::BatchEntry* t_reunit = nullptr;
     sun::BatchEntry* t_plaq = nullptr;
     int blocks_update = 0, blocks_reunit = 0, blocks_plaq = 0;
+    int piped[2] = {0, 0};  // staple kernel of the heatbath / overrelaxation launches
+    int brick[2] = {0, 0};  // their lattices' update bricks instead of linear order
     std::vector<sun::BatchEntry> h_update;
     double2* partials = nullptr;
     double2* d_out = nullptr;

@@ -84,7 +84,34 @@ struct BatchEntry {
     double scale;        // normalisation of the reduced sum
     double2* hist;       // observable history: {plaquette, polyakov} per iteration
     uint32_t hist_iter;  // next history row
+    int ub1, ub2, ub3;   // brick of the 256-thread update workgroups (ub1 < 0: none)
 };
+
+// The update brick of a batched lattice: 256 threads = (X0/2) * 2^k rows.  The k
+// doublings go to x1, x3, x1, x3 (the tuned single-lattice shape at 32^4 is such a
+// (2, 0, 2) brick), then x2, x2, then round-robin, each only where the extent stays
+// divisible.  b1 < 0 when no brick tiles the lattice.
+inline void batch_brick(const Geom& g, int& b1, int& b2, int& b3) {
+    b1 = -1;
+    b2 = b3 = 0;
+    const int L = g.X[0] / 2;
+    if (L <= 0 || L > 256 || 256 % L) return;
+    int k = 0;
+    while ((L << k) < 256) k++;
+    static const int seq[] = {1, 3, 1, 3, 2, 2, 1, 3, 2, 1, 3, 2, 1, 3, 2, 1, 3, 2};
+    int c[4] = {0, 0, 0, 0};
+    for (int i = 0; i < (int)(sizeof(seq) / sizeof(seq[0])) && k > 0; i++) {
+        const int d = seq[i];
+        if (g.X[d] % (2 << c[d]) == 0) {
+            c[d]++;
+            k--;
+        }
+    }
+    if (k > 0) return;
+    b1 = c[1];
+    b2 = c[2];
+    b3 = c[3];
+}
 
 struct OpsTable {
     int nc, storage, real_bytes;
@@ -101,9 +128,11 @@ struct OpsTable {
     hipError_t (*polyakov_partial)(const Field&, const Geom&, double2* partials, hipStream_t);
     hipError_t (*to_canonical)(const Field&, int nlinks, double* out, hipStream_t);
     hipError_t (*from_canonical)(const Field&, int nlinks, const double* in, hipStream_t);
-    // batched: every lattice of the table, one (parity, mu) launch
+    // batched: every lattice of the table, one (parity, mu) launch; piped 0 plain,
+    // 1 / 2 software-pipelined staple (only where has_piped); brick: the entries'
+    // update bricks (ub1 >= 0) instead of the linear site order
     hipError_t (*batch_update)(const BatchEntry* dev_table, int nlat, int total_blocks, int parity,
-                               int mu, bool hb, hipStream_t);
+                               int mu, bool hb, int piped, int brick, hipStream_t);
     hipError_t (*batch_reunitarize)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     hipError_t (*batch_plaquette_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     hipError_t (*batch_polyakov_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
@@ -135,6 +164,22 @@ struct OpsTable {
     int has_piped;   // the variant has the software-pipelined overrelaxation kernel
     int piped_max_block;
 };
+
+// The launch shape of batched updates unless the caller picks one (kernel 0
+// heatbath, 1 overrelaxation).  Staple kernel (0 plain, 1 / 2 direction /
+// half-direction pipelined): measured over 64 FP32 SU(3) lattices of 8^4..24^4 and a
+// 32^4 lattice alone, half-direction pipelining is the fastest for both kernels with
+// 12-real links (+3% and +17% over plain); with 8-real links for the
+// overrelaxation only (+13% over 64 lattices), the pipelined heatbath being slower.
+inline int default_batch_piped(const OpsTable* ops, int kernel) {
+    if (!ops->has_piped) return 0;
+    if (ops->storage == 12) return 2;
+    return kernel == 0 ? 0 : 2;
+}
+// Bricks (where a 256-thread brick tiles a lattice): a 32^4 lattice runs 12% faster
+// with them (its workgroups' shared neighbour links stay in one L2); over the
+// 64-lattice mix they change the rate by -0.8..+1.8%.
+inline int default_batch_brick(const OpsTable*, int) { return 1; }
 
 // gfx950 dispatches workgroup b of a grid to XCD (b mod 8); each XCD has its own
 // 4 MB L2.  xcd_remap turns the dispatch order into a contiguous range of tiles per

@@ -71,6 +71,8 @@ struct Job {
 struct Group {
     const OpsTable* ops = nullptr;
     int n_or = 0, reunit = 0, vkey = 0;
+    int piped[2] = {0, 0};  // staple kernel of the heatbath / overrelaxation launches
+    int brick[2] = {0, 0};  // their members' update bricks instead of linear order
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     int cap = 0;
@@ -144,6 +146,10 @@ int make_group(sun_server* S, const sun_lattice* L, int n_or, int reunit, Group*
     Group* g = new (std::nothrow) Group();
     if (!g) return fail(SUN_ERR_NOMEM, "server: host allocation");
     g->ops = L->ops;
+    for (int k = 0; k < 2; k++) {
+        g->piped[k] = default_batch_piped(L->ops, k);
+        g->brick[k] = default_batch_brick(L->ops, k);
+    }
     g->n_or = n_or;
     g->reunit = reunit;
     g->vkey = group_vkey(L);
@@ -183,6 +189,7 @@ int rebuild_tables(Group* g) {
         e.sweep = (uint32_t)J->iters_done;
         e.hist = L->hist;
         e.hist_iter = (uint32_t)J->iters_done;
+        batch_brick(L->g, e.ub1, e.ub2, e.ub3);
         BatchEntry a = e, r = e, p = e, q = e;
         a.first_block = cu;
         a.nblocks = update_blocks(L->g, 256);
@@ -226,11 +233,11 @@ int group_iteration(Group* g) {
     hipStream_t s = g->stream;
     for (int parity = 0; parity < 2; parity++)
         for (int mu = 0; mu < 4; mu++)
-            HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, true, s));
+            HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, true, g->piped[0], g->brick[0], s));
     for (int k = 0; k < g->n_or; k++)
         for (int parity = 0; parity < 2; parity++)
             for (int mu = 0; mu < 4; mu++)
-                HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, false, s));
+                HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, false, g->piped[1], g->brick[1], s));
     if (g->reunit) HIP_TRY(g->ops->batch_reunitarize(g->t_reunit, n, g->blocks_reunit, s));
     HIP_TRY(g->ops->batch_plaquette_partial(g->t_plaq, n, g->blocks_plaq, s));
     HIP_TRY(launch_batch_finalize(g->t_plaq, n, g->d_plaq, s));

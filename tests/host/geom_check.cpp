@@ -89,6 +89,31 @@ int main(int argc, char** argv) {
             if (!seen[i]) { printf("REDUCE MISSING %d\n", i); return 1; }
         fprintf(stderr, "reduce brick %d %d %d ok\n", g.rb1, g.rb2, g.rb3);
     }
+    // the batched update brick, where one exists, is a shape brick_block accepts with
+    // exactly 256 threads, so ceil(V/2 / 256) workgroups tile the parity exactly
+    {
+        int b1, b2, b3;
+        sun::batch_brick(g, b1, b2, b3);
+        const int L = g.X[0] / 2;
+        const bool pow2 = L > 0 && L <= 256 && (256 % L) == 0;
+        if (b1 >= 0) {
+            if (sun::brick_block(g, b1, b2, b3, 256) != 256 || g.half % 256) {
+                printf("BATCH BRICK FAIL %d %d %d\n", b1, b2, b3);
+                return 1;
+            }
+            fprintf(stderr, "batch brick %d %d %d ok\n", b1, b2, b3);
+        } else if (pow2 && (long long)g.half % 256 == 0) {
+            // no brick although X0/2 divides 256: only when no split of the rows over
+            // x1, x2, x3 divides the extents
+            for (int c1 = 0; c1 <= 8; c1++)
+                for (int c2 = 0; c2 <= 8; c2++)
+                    for (int c3 = 0; c3 <= 8; c3++)
+                        if (sun::brick_block(g, c1, c2, c3, 256) == 256 && c1 + c2 + c3 <= 6) {
+                            printf("BATCH BRICK MISSED %d %d %d\n", c1, c2, c3);
+                            return 1;
+                        }
+        }
+    }
     // XCD remap of workgroups is a bijection of [0, n) for every grid size
     for (int n = 1; n <= 5000; n++) {
         std::vector<int> seen(n, 0);

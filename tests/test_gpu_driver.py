@@ -129,6 +129,55 @@ def test_batch_bitwise_equals_alone(ctx, storage, prec):
         lat.close()
 
 
+@pytest.mark.parametrize("storage", [12, 8])
+def test_batch_launch_shapes_bitwise_equal(ctx, storage):
+    """Every batched launch shape -- plain, direction- and half-direction-pipelined
+    staple, linear or per-lattice brick order -- gives the same bits, on lattices
+    whose X0/2 is 4 (bricks of 4x4x4 rows), 5 (no brick: linear) and 16 (4x1x4 rows)."""
+    specs = [((8, 8, 8, 8), 5.8, 31), ((10, 6, 4, 8), 6.1, 32), ((32, 4, 4, 8), 6.3, 33)]
+    shapes = [(0, 0, 0), (0, 0, 1), (1, 2, 0), (2, 1, 1), (2, 2, 1), (0, 1, 1), (1, 1, 0)]
+    res = {}
+    for hb_mode, or_mode, brick in shapes:
+        lats = []
+        for dims, beta, seed in specs:
+            lat = sun_amd.Lattice(ctx, 3, dims, beta, storage, 4)
+            lat.hot_start(seed)
+            lats.append(lat)
+        b = sun_amd.Batch(ctx, lats, [s[2] for s in specs])
+        b.set_launch(0, hb_mode, brick)
+        b.set_launch(1, or_mode, brick)
+        assert (b.get_launch(0), b.get_launch(1)) == ((hb_mode, brick), (or_mode, brick))
+        obs = b.generate(first_sweep=0, n_iter=2, n_or=3, reunit_every=1, use_graph=True)
+        res[(hb_mode, or_mode, brick)] = (obs, [l.to_host() for l in lats])
+        b.close()
+        for lat in lats:
+            lat.close()
+    ref_obs, ref_links = res.pop((0, 0, 0))
+    for key, (obs, links) in res.items():
+        assert np.array_equal(obs, ref_obs), key
+        for a, r in zip(links, ref_links):
+            assert np.array_equal(a, r), key
+
+
+def test_batch_piped_refused_without_kernel(ctx):
+    lats = [sun_amd.Lattice(ctx, 3, (4, 4, 4, 4), 6.0, 18, 4), sun_amd.Lattice(ctx, 2, (4, 4, 4, 4), 2.3, 18, 4)]
+    for lat in lats:
+        b = sun_amd.Batch(ctx, [lat], [1])
+        assert b.get_launch(0)[0] == 0 and b.get_launch(1)[0] == 0
+        b.set_launch(1, 0, 1)  # bricks need no pipelined kernel
+        assert b.get_launch(1) == (0, 1)
+        for kernel in (0, 1):
+            with pytest.raises(sun_amd.SunError) as e:
+                b.set_launch(kernel, 1)
+            assert e.value.code == sun_amd.ERR_INVALID
+        with pytest.raises(sun_amd.SunError):
+            b.set_launch(2, 0)
+        with pytest.raises(sun_amd.SunError):
+            b.set_launch(0, 0, 2)
+        b.close()
+        lat.close()
+
+
 def test_batch_step_api_matches_generate(ctx):
     specs = [((8, 8, 8, 8), 6.0, 21), ((4, 4, 4, 8), 5.7, 22)]
     lats = [sun_amd.Lattice(ctx, 3, d, b, 12, 4) for d, b, _ in specs]
