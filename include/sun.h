@@ -239,11 +239,12 @@ int sun_batch_destroy(sun_batch *b);
  * launches.  piped: the staple kernel, 0 plain, 1 / 2 direction / half-direction
  * software-pipelined (FP32 SU(3) 12- and 8-real only; SUN_ERR_INVALID otherwise).
  * brick: 1 maps each 256-thread workgroup to a brick of rows of its lattice where one
- * tiles it (X0/2 a power of two), 0 the linear site order.  Every shape gives
- * bit-identical links.  Replaces the block-size search of tune.cpp:158-414 over the
- * sweep launches of monte.cu:224-226 */
-int sun_batch_get_launch(const sun_batch *b, int kernel, int *piped, int *brick);
-int sun_batch_set_launch(sun_batch *b, int kernel, int piped, int brick);
+ * tiles it (X0/2 a power of two), 0 the linear site order.  lds: unused dynamic LDS
+ * per workgroup, 0..65536 bytes, capping the workgroups resident per CU.  Every
+ * shape gives bit-identical links.  Replaces the block-size search of
+ * tune.cpp:158-414 over the sweep launches of monte.cu:224-226 */
+int sun_batch_get_launch(const sun_batch *b, int kernel, int *piped, int *brick, int *lds);
+int sun_batch_set_launch(sun_batch *b, int kernel, int piped, int brick, int lds);
 int sun_batch_heatbath(sun_batch *b, const uint32_t *sweeps, void *stream);
 int sun_batch_overrelax(sun_batch *b, void *stream);
 int sun_batch_reunitarize(sun_batch *b, void *stream);

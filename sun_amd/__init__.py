@@ -165,8 +165,8 @@ def lib() -> ctypes.CDLL:
         "sun_generate": (ci, [vp, ctypes.POINTER(GenerateDesc), vp, dp]),
         "sun_batch_create": (ci, [vp, ci, ctypes.POINTER(vp), ctypes.POINTER(u64), ctypes.POINTER(vp)]),
         "sun_batch_destroy": (ci, [vp]),
-        "sun_batch_get_launch": (ci, [vp, ci, ctypes.POINTER(ci), ctypes.POINTER(ci)]),
-        "sun_batch_set_launch": (ci, [vp, ci, ci, ci]),
+        "sun_batch_get_launch": (ci, [vp, ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
+        "sun_batch_set_launch": (ci, [vp, ci, ci, ci, ci]),
         "sun_batch_heatbath": (ci, [vp, ctypes.POINTER(u32), vp]),
         "sun_batch_overrelax": (ci, [vp, vp]),
         "sun_batch_reunitarize": (ci, [vp, vp]),
@@ -415,15 +415,17 @@ class Batch:
             self.h = ctypes.c_void_p()
 
     def get_launch(self, kernel: int):
-        """(piped, brick) of the heatbath (kernel 0) or overrelaxation (1) launches:
-        piped 0 plain, 1 / 2 direction / half-direction software-pipelined staple;
-        brick 1 the lattices' update bricks, 0 the linear site order."""
-        p, b = ctypes.c_int(), ctypes.c_int()
-        _check(lib().sun_batch_get_launch(self.h, kernel, ctypes.byref(p), ctypes.byref(b)), "sun_batch_get_launch")
-        return p.value, b.value
+        """(piped, brick, lds) of the heatbath (kernel 0) or overrelaxation (1)
+        launches: piped 0 plain, 1 / 2 direction / half-direction software-pipelined
+        staple; brick 1 the lattices' update bricks, 0 the linear site order; lds the
+        unused dynamic LDS per workgroup (occupancy cap)."""
+        p, b, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().sun_batch_get_launch(self.h, kernel, ctypes.byref(p), ctypes.byref(b), ctypes.byref(l)),
+               "sun_batch_get_launch")
+        return p.value, b.value, l.value
 
-    def set_launch(self, kernel: int, piped: int, brick: int = 0):
-        _check(lib().sun_batch_set_launch(self.h, kernel, piped, brick), "sun_batch_set_launch")
+    def set_launch(self, kernel: int, piped: int, brick: int = 0, lds: int = 0):
+        _check(lib().sun_batch_set_launch(self.h, kernel, piped, brick, lds), "sun_batch_set_launch")
 
     def heatbath(self, sweeps, stream=None):
         n = len(self.lattices)

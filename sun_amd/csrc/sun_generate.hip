@@ -158,6 +158,7 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
     for (int k = 0; k < 2; k++) {
         b->piped[k] = default_batch_piped(ops, k);
         b->brick[k] = default_batch_brick(ops, k);
+        b->lds[k] = default_batch_lds(ops, k);
     }
     b->lats.assign(lats, lats + n);
     std::vector<BatchEntry> tu(n), tr(n), tp(n);
@@ -221,22 +222,25 @@ extern "C" int sun_batch_destroy(sun_batch* b) {
     return SUN_OK;
 }
 
-extern "C" int sun_batch_get_launch(const sun_batch* b, int kernel, int* piped, int* brick) {
-    if (!b || !piped || !brick || (kernel != 0 && kernel != 1))
+extern "C" int sun_batch_get_launch(const sun_batch* b, int kernel, int* piped, int* brick, int* lds) {
+    if (!b || !piped || !brick || !lds || (kernel != 0 && kernel != 1))
         return fail(SUN_ERR_INVALID, "sun_batch_get_launch: bad argument");
     *piped = b->piped[kernel];
     *brick = b->brick[kernel];
+    *lds = b->lds[kernel];
     return SUN_OK;
 }
 
-extern "C" int sun_batch_set_launch(sun_batch* b, int kernel, int piped, int brick) {
+extern "C" int sun_batch_set_launch(sun_batch* b, int kernel, int piped, int brick, int lds) {
     if (!b || (kernel != 0 && kernel != 1)) return fail(SUN_ERR_INVALID, "sun_batch_set_launch: bad argument");
     if (piped < 0 || piped > 2) return fail(SUN_ERR_INVALID, "sun_batch_set_launch: piped %d (0, 1 or 2)", piped);
     if (brick != 0 && brick != 1) return fail(SUN_ERR_INVALID, "sun_batch_set_launch: brick %d (0 or 1)", brick);
+    if (lds < 0 || lds > 65536) return fail(SUN_ERR_INVALID, "sun_batch_set_launch: lds %d (0..65536)", lds);
     if (piped && !b->ops->has_piped)
         return fail(SUN_ERR_INVALID, "sun_batch_set_launch: no pipelined kernel for this N/storage/precision");
     b->piped[kernel] = piped;
     b->brick[kernel] = brick;
+    b->lds[kernel] = lds;
     return SUN_OK;
 }
 
@@ -256,7 +260,8 @@ static int batch_hb(sun_batch* b, hipStream_t s) {
     const int n = (int)b->lats.size();
     for (int parity = 0; parity < 2; parity++)
         for (int mu = 0; mu < 4; mu++)
-            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, true, b->piped[0], b->brick[0], s));
+            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, true, b->piped[0], b->brick[0],
+                                         b->lds[0], s));
     return SUN_OK;
 }
 
@@ -264,7 +269,8 @@ static int batch_or(sun_batch* b, hipStream_t s) {
     const int n = (int)b->lats.size();
     for (int parity = 0; parity < 2; parity++)
         for (int mu = 0; mu < 4; mu++)
-            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, false, b->piped[1], b->brick[1], s));
+            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, false, b->piped[1], b->brick[1],
+                                         b->lds[1], s));
     return SUN_OK;
 }
 

@@ -117,6 +117,7 @@ struct sun_batch {
     int blocks_update = 0, blocks_reunit = 0, blocks_plaq = 0;
     int piped[2] = {0, 0};  // staple kernel of the heatbath / overrelaxation launches
     int brick[2] = {0, 0};  // their lattices' update bricks instead of linear order
+    int lds[2] = {0, 0};    // unused dynamic LDS per workgroup (occupancy cap)
     std::vector<sun::BatchEntry> h_update;
     double2* partials = nullptr;
     double2* d_out = nullptr;

@@ -130,9 +130,10 @@ struct OpsTable {
     hipError_t (*from_canonical)(const Field&, int nlinks, const double* in, hipStream_t);
     // batched: every lattice of the table, one (parity, mu) launch; piped 0 plain,
     // 1 / 2 software-pipelined staple (only where has_piped); brick: the entries'
-    // update bricks (ub1 >= 0) instead of the linear site order
+    // update bricks (ub1 >= 0) instead of the linear site order; lds: unused dynamic
+    // LDS per workgroup (occupancy cap)
     hipError_t (*batch_update)(const BatchEntry* dev_table, int nlat, int total_blocks, int parity,
-                               int mu, bool hb, int piped, int brick, hipStream_t);
+                               int mu, bool hb, int piped, int brick, int lds, hipStream_t);
     hipError_t (*batch_reunitarize)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     hipError_t (*batch_plaquette_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
     hipError_t (*batch_polyakov_partial)(const BatchEntry* dev_table, int nlat, int total_blocks, hipStream_t);
@@ -180,6 +181,7 @@ inline int default_batch_piped(const OpsTable* ops, int kernel) {
 // with them (its workgroups' shared neighbour links stay in one L2); over the
 // 64-lattice mix they change the rate by -0.8..+1.8%.
 inline int default_batch_brick(const OpsTable*, int) { return 1; }
+inline int default_batch_lds(const OpsTable*, int) { return 0; }
 
 // gfx950 dispatches workgroup b of a grid to XCD (b mod 8); each XCD has its own
 // 4 MB L2.  xcd_remap turns the dispatch order into a contiguous range of tiles per

@@ -73,6 +73,7 @@ struct Group {
     int n_or = 0, reunit = 0, vkey = 0;
     int piped[2] = {0, 0};  // staple kernel of the heatbath / overrelaxation launches
     int brick[2] = {0, 0};  // their members' update bricks instead of linear order
+    int lds[2] = {0, 0};    // unused dynamic LDS per workgroup (occupancy cap)
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     int cap = 0;
@@ -149,6 +150,7 @@ int make_group(sun_server* S, const sun_lattice* L, int n_or, int reunit, Group*
     for (int k = 0; k < 2; k++) {
         g->piped[k] = default_batch_piped(L->ops, k);
         g->brick[k] = default_batch_brick(L->ops, k);
+        g->lds[k] = default_batch_lds(L->ops, k);
     }
     g->n_or = n_or;
     g->reunit = reunit;
@@ -233,11 +235,13 @@ int group_iteration(Group* g) {
     hipStream_t s = g->stream;
     for (int parity = 0; parity < 2; parity++)
         for (int mu = 0; mu < 4; mu++)
-            HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, true, g->piped[0], g->brick[0], s));
+            HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, true, g->piped[0], g->brick[0],
+                                         g->lds[0], s));
     for (int k = 0; k < g->n_or; k++)
         for (int parity = 0; parity < 2; parity++)
             for (int mu = 0; mu < 4; mu++)
-                HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, false, g->piped[1], g->brick[1], s));
+                HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, false, g->piped[1],
+                                             g->brick[1], g->lds[1], s));
     if (g->reunit) HIP_TRY(g->ops->batch_reunitarize(g->t_reunit, n, g->blocks_reunit, s));
     HIP_TRY(g->ops->batch_plaquette_partial(g->t_plaq, n, g->blocks_plaq, s));
     HIP_TRY(launch_batch_finalize(g->t_plaq, n, g->d_plaq, s));

@@ -132,27 +132,29 @@ def test_batch_bitwise_equals_alone(ctx, storage, prec):
 @pytest.mark.parametrize("storage", [12, 8])
 def test_batch_launch_shapes_bitwise_equal(ctx, storage):
     """Every batched launch shape -- plain, direction- and half-direction-pipelined
-    staple, linear or per-lattice brick order -- gives the same bits, on lattices
-    whose X0/2 is 4 (bricks of 4x4x4 rows), 5 (no brick: linear) and 16 (4x1x4 rows)."""
+    staple, linear or per-lattice brick order, LDS occupancy caps -- gives the same
+    bits, on lattices whose X0/2 is 4 (bricks of 4x4x4 rows), 5 (no brick: linear)
+    and 16 (4x1x4 rows)."""
     specs = [((8, 8, 8, 8), 5.8, 31), ((10, 6, 4, 8), 6.1, 32), ((32, 4, 4, 8), 6.3, 33)]
-    shapes = [(0, 0, 0), (0, 0, 1), (1, 2, 0), (2, 1, 1), (2, 2, 1), (0, 1, 1), (1, 1, 0)]
+    shapes = [(0, 0, 0, 0), (0, 0, 1, 0), (1, 2, 0, 0), (2, 1, 1, 32768), (2, 2, 1, 54272), (0, 1, 1, 0),
+              (1, 1, 0, 16384)]
     res = {}
-    for hb_mode, or_mode, brick in shapes:
+    for hb_mode, or_mode, brick, lds in shapes:
         lats = []
         for dims, beta, seed in specs:
             lat = sun_amd.Lattice(ctx, 3, dims, beta, storage, 4)
             lat.hot_start(seed)
             lats.append(lat)
         b = sun_amd.Batch(ctx, lats, [s[2] for s in specs])
-        b.set_launch(0, hb_mode, brick)
-        b.set_launch(1, or_mode, brick)
-        assert (b.get_launch(0), b.get_launch(1)) == ((hb_mode, brick), (or_mode, brick))
+        b.set_launch(0, hb_mode, brick, lds)
+        b.set_launch(1, or_mode, brick, lds)
+        assert (b.get_launch(0), b.get_launch(1)) == ((hb_mode, brick, lds), (or_mode, brick, lds))
         obs = b.generate(first_sweep=0, n_iter=2, n_or=3, reunit_every=1, use_graph=True)
-        res[(hb_mode, or_mode, brick)] = (obs, [l.to_host() for l in lats])
+        res[(hb_mode, or_mode, brick, lds)] = (obs, [l.to_host() for l in lats])
         b.close()
         for lat in lats:
             lat.close()
-    ref_obs, ref_links = res.pop((0, 0, 0))
+    ref_obs, ref_links = res.pop((0, 0, 0, 0))
     for key, (obs, links) in res.items():
         assert np.array_equal(obs, ref_obs), key
         for a, r in zip(links, ref_links):
@@ -164,8 +166,8 @@ def test_batch_piped_refused_without_kernel(ctx):
     for lat in lats:
         b = sun_amd.Batch(ctx, [lat], [1])
         assert b.get_launch(0)[0] == 0 and b.get_launch(1)[0] == 0
-        b.set_launch(1, 0, 1)  # bricks need no pipelined kernel
-        assert b.get_launch(1) == (0, 1)
+        b.set_launch(1, 0, 1, 16384)  # bricks and LDS caps need no pipelined kernel
+        assert b.get_launch(1) == (0, 1, 16384)
         for kernel in (0, 1):
             with pytest.raises(sun_amd.SunError) as e:
                 b.set_launch(kernel, 1)
@@ -174,6 +176,8 @@ def test_batch_piped_refused_without_kernel(ctx):
             b.set_launch(2, 0)
         with pytest.raises(sun_amd.SunError):
             b.set_launch(0, 0, 2)
+        with pytest.raises(sun_amd.SunError):
+            b.set_launch(0, 0, 0, 65537)
         b.close()
         lat.close()
 
