@@ -256,11 +256,27 @@ def batched_leg(sun_amd, ctx, stream, n_lat=64, n_iter=30, warm=3):
     sun_amd.sync(stream)
     dt = time.perf_counter() - t0
     sites = sum(int(np.prod(v)) for v in vols)
+    # one overrelaxation sweep of all 64 lattices (8 batched launches), HIP events on
+    # its stream: algorithmic bytes 9 * 48 B per link update, 4 * sites updates
+    t = sun_amd.Timer()
+    batch.overrelax(stream)
+    t.start(stream)
+    reps = 10
+    for _ in range(reps):
+        batch.overrelax(stream)
+    t.stop(stream)
+    or_sweep_ms = t.ms() / reps
+    t.close()
+    or_gbs = 9 * link_bytes(3, 12, 4) * 4 * sites / (or_sweep_ms * 1e-3) / 1e9
+    shapes = {k: dict(zip(("piped", "brick", "lds"), batch.get_launch(i)))
+              for i, k in enumerate(("heatbath", "overrelax"))}
     batch.close()
     for l in lats:
         l.close()
     return {"value": 5 * 4 * sites * n_iter / dt, "unit": "link-updates/s", "lattices": n_lat,
             "total_sites": sites, "iterations": n_iter, "ms_per_iteration": dt / n_iter * 1e3,
+            "overrelax_sweep_ms": or_sweep_ms, "overrelax_gbs": or_gbs, "overrelax_frac": or_gbs / PEAK_HBM_GBS,
+            "launch_shapes": shapes,
             "plaquette_range_last": [float(obs[-1, :, :].mean(axis=1).min()),
                                      float(obs[-1, :, :].mean(axis=1).max())],
             "config": "64 SU(3) lattices, volumes from {8^4,12^4,16^4,16^3x32,24^4} (numpy seed 4), beta "
