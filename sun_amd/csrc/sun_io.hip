@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <new>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -147,7 +148,11 @@ int sun::read_native(FILE* f, const sun_lattice_desc& d, size_t bytes, std::vect
                     "%s: checkpoint of N=%d %dx%dx%dx%d storage %d precision %d does not match the lattice", path,
                     h.nc, h.dims[0], h.dims[1], h.dims[2], h.dims[3], h.storage, h.precision);
     if (h.payload_bytes != bytes) return fail(SUN_ERR_INVALID, "%s: payload size mismatch", path);
-    payload.resize(bytes);
+    try {
+        payload.resize(bytes);
+    } catch (const std::bad_alloc&) {
+        return fail(SUN_ERR_NOMEM, "%s: host allocation of %zu bytes", path, bytes);
+    }
     if (fread(payload.data(), 1, payload.size(), f) != payload.size())
         return fail(SUN_ERR_INVALID, "%s: truncated payload", path);
     if (crc32(payload.data(), payload.size()) != h.payload_crc32)

@@ -29,6 +29,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -613,19 +614,27 @@ extern "C" int sun_server_destroy(sun_server* S) {
     return SUN_OK;
 }
 
-extern "C" int sun_server_submit(sun_server* S, const sun_job_desc* jd, uint64_t* id) {
-    if (!S || !jd || !id) return fail(SUN_ERR_INVALID, "sun_server_submit: NULL argument");
+// checks of a job description (submitted, or read from a job file: with_state when
+// the file carries the job's lattice) and the server memory it needs
+int check_job(const sun_server* S, const sun_job_desc* jd, bool with_state, size_t* bytes) {
     sun_lattice_desc d = {jd->nc, {jd->dims[0], jd->dims[1], jd->dims[2], jd->dims[3]}, jd->beta, jd->storage,
                           jd->precision};
     SUN_TRY(check_desc(&d));
     if (jd->n_iter < 1 || jd->n_or < 0) return fail(SUN_ERR_INVALID, "job: n_iter >= 1 and n_or >= 0 required");
     if (jd->reunit_every != 0 && jd->reunit_every != 1)
         return fail(SUN_ERR_UNSUPPORTED, "job: reunit_every must be 0 or 1 in the server");
-    if (!jd->hot_start && jd->storage == SUN_STORE_8)
+    if (!with_state && !jd->hot_start && jd->storage == SUN_STORE_8)
         return fail(SUN_ERR_UNSUPPORTED, "job: 8-real storage cannot hold a cold start (gaugearray.cu:293-297)");
-    const size_t bytes = Arena::round_up(lattice_footprint(&d, jd->n_iter));
-    if (bytes > S->arena.capacity())
-        return fail(SUN_ERR_BUDGET, "job needs %zu bytes, the server budget is %zu", bytes, S->arena.capacity());
+    *bytes = Arena::round_up(lattice_footprint(&d, jd->n_iter));
+    if (*bytes > S->arena.capacity())
+        return fail(SUN_ERR_BUDGET, "job needs %zu bytes, the server budget is %zu", *bytes, S->arena.capacity());
+    return SUN_OK;
+}
+
+extern "C" int sun_server_submit(sun_server* S, const sun_job_desc* jd, uint64_t* id) {
+    if (!S || !jd || !id) return fail(SUN_ERR_INVALID, "sun_server_submit: NULL argument");
+    size_t bytes = 0;
+    SUN_TRY(check_job(S, jd, false, &bytes));
     Job* J = new (std::nothrow) Job();
     if (!J) return fail(SUN_ERR_NOMEM, "host allocation");
     J->d = *jd;
@@ -810,14 +819,22 @@ extern "C" int sun_server_resume(sun_server* S, const char* path, uint64_t* id) 
          get(f, d.n_iter) && get(f, d.n_or) && get(f, d.reunit_every) && get(f, iters_done);
     if (!ok) return fail(SUN_ERR_INVALID, "%s: not a job file", path);
     sun_lattice_desc ld = {d.nc, {d.dims[0], d.dims[1], d.dims[2], d.dims[3]}, d.beta, d.storage, d.precision};
-    SUN_TRY(check_desc(&ld));
-    if (iters_done < 0 || iters_done > d.n_iter || (iters_done > 0 && !has_lattice))
+    if (iters_done < 0 || iters_done > d.n_iter || (iters_done > 0 && !has_lattice) || has_lattice > 1)
         return fail(SUN_ERR_INVALID, "%s: inconsistent job state", path);
+    // every size below follows from a description that fits the budget
+    size_t bytes = 0;
+    SUN_TRY(check_job(S, &d, has_lattice != 0, &bytes));
     Job* J = new (std::nothrow) Job();
     if (!J) return fail(SUN_ERR_NOMEM, "host allocation");
     J->d = d;
     J->iters_done = iters_done;
-    J->resume_hist.resize((size_t)2 * iters_done);
+    J->bytes = bytes;
+    try {
+        J->resume_hist.resize((size_t)2 * iters_done);
+    } catch (const std::bad_alloc&) {
+        delete J;
+        return fail(SUN_ERR_NOMEM, "%s: host allocation for %d observable rows", path, iters_done);
+    }
     if (fread(J->resume_hist.data(), sizeof(double2), J->resume_hist.size(), f) != J->resume_hist.size()) {
         delete J;
         return fail(SUN_ERR_INVALID, "%s: truncated observables", path);
@@ -831,11 +848,6 @@ extern "C" int sun_server_resume(sun_server* S, const char* path, uint64_t* id) 
             delete J;
             return rc != SUN_OK ? rc : fail(SUN_ERR_INVALID, "%s: lattice checkpoint does not match the job", path);
         }
-    }
-    J->bytes = Arena::round_up(lattice_footprint(&ld, d.n_iter));
-    if (J->bytes > S->arena.capacity()) {
-        delete J;
-        return fail(SUN_ERR_BUDGET, "job needs %zu bytes, the server budget is %zu", J->bytes, S->arena.capacity());
     }
     std::lock_guard<std::mutex> lk(S->mu);
     J->id = S->next_id++;

@@ -277,3 +277,46 @@ def test_suspend_with_background_thread(ctx, tmp_path):
         srv.wait(j2, 60.0)
         srv.stop()
         assert np.array_equal(srv.observables(j2, job[6]), alone(ctx, job))
+
+
+def job_file(path, magic=b"SUNJOB01", version=1, has_lattice=0, nc=3, dims=(8, 8, 8, 8), beta=6.0, storage=12,
+             prec=4, seed=1, hot=1, n_iter=10, n_or=4, reunit=1, iters_done=0, rows=0):
+    """a job file in the server's layout (magic, version, has-lattice flag, the job
+    description field by field, iterations done, observable rows, checkpoint)"""
+    import struct
+    with open(path, "wb") as f:
+        f.write(struct.pack("<8sIIi4idiiQiiiii", magic, version, has_lattice, nc, *dims, beta, storage, prec, seed,
+                            hot, n_iter, n_or, reunit, iters_done))
+        f.write(b"\0" * (32 * rows))
+    return path
+
+
+@pytest.mark.gpu
+def test_resume_rejects_bad_job_files(ctx, tmp_path):
+    """Corrupt or inconsistent job files are refused with a status, before any
+    allocation sized from the file; a well-formed pending job file is accepted."""
+    p = str(tmp_path / "j.bin")
+    cases = [
+        (dict(magic=b"NOTAJOB!"), sun_amd.ERR_INVALID),
+        (dict(version=2), sun_amd.ERR_INVALID),
+        (dict(n_iter=0), sun_amd.ERR_INVALID),
+        (dict(n_or=-1), sun_amd.ERR_INVALID),
+        (dict(reunit=2), sun_amd.ERR_UNSUPPORTED),
+        (dict(nc=5), sun_amd.ERR_UNSUPPORTED),
+        (dict(dims=(8, 8, 7, 8)), sun_amd.ERR_INVALID),
+        (dict(iters_done=11), sun_amd.ERR_INVALID),
+        (dict(iters_done=3), sun_amd.ERR_INVALID),                        # progress without a lattice
+        (dict(has_lattice=1, iters_done=3, rows=1), sun_amd.ERR_INVALID),  # truncated rows
+        (dict(has_lattice=1, iters_done=3, rows=3), sun_amd.ERR_INVALID),  # no checkpoint after them
+        (dict(has_lattice=1, n_iter=2 ** 31 - 1, iters_done=2 ** 30), sun_amd.ERR_BUDGET),
+        (dict(storage=8, hot=0), sun_amd.ERR_UNSUPPORTED),
+    ]
+    with sun_amd.Server(ctx, 1 << 30) as srv:
+        for fields, code in cases:
+            with pytest.raises(sun_amd.SunError) as e:
+                srv.resume(job_file(p, **fields))
+            assert e.value.code == code, (fields, e.value.code, str(e.value))
+        assert srv.stats().pending == 0 and srv.stats().bytes_in_use == 0
+        jid = srv.resume(job_file(p, n_iter=2, seed=9))
+        run_until(srv, jid, sun_amd.JOB_DONE)
+        assert srv.status(jid).iters_done == 2
