@@ -74,7 +74,7 @@ int capture(hipStream_t s, GraphGuard& gg, const std::function<int()>& body) {
 
 }  // namespace
 
-extern "C" int sun_generate(sun_lattice* lat, const sun_generate_desc* d, void* stream, double* obs) {
+static int sun_generate_body(sun_lattice* lat, const sun_generate_desc* d, void* stream, double* obs) {
     if (!lat || !d) return fail(SUN_ERR_INVALID, "sun_generate: NULL argument");
     if (d->n_iter < 0 || d->n_or < 0 || d->reunit_every < 0)
         return fail(SUN_ERR_INVALID, "sun_generate: negative count");
@@ -120,6 +120,10 @@ extern "C" int sun_generate(sun_lattice* lat, const sun_generate_desc* d, void* 
         HIP_TRY(hipStreamSynchronize(s));
     }
     return SUN_OK;
+}
+
+extern "C" int sun_generate(sun_lattice* lat, const sun_generate_desc* d, void* stream, double* obs) {
+    return guard_alloc("sun_generate", [&] { return sun_generate_body(lat, d, stream, obs); });
 }
 
 // ---------------------------------------------------------------------------------
@@ -327,8 +331,8 @@ static int batch_iteration(sun_batch* b, int n_or, bool reunit, bool measure, hi
     return SUN_OK;
 }
 
-extern "C" int sun_batch_generate(sun_batch* b, uint32_t first_sweep, int n_iter, int n_or, int reunit_every,
-                                  int use_graph, void* stream, double* obs) {
+static int sun_batch_generate_body(sun_batch* b, uint32_t first_sweep, int n_iter, int n_or, int reunit_every,
+                                   int use_graph, void* stream, double* obs) {
     if (!b || n_iter < 0 || n_or < 0 || reunit_every < 0) return fail(SUN_ERR_INVALID, "sun_batch_generate: bad argument");
     if (n_iter == 0) return SUN_OK;
     hipStream_t s = batch_stream(b, stream);
@@ -368,4 +372,11 @@ extern "C" int sun_batch_generate(sun_batch* b, uint32_t first_sweep, int n_iter
         HIP_TRY(hipStreamSynchronize(s));
     }
     return SUN_OK;
+}
+
+extern "C" int sun_batch_generate(sun_batch* b, uint32_t first_sweep, int n_iter, int n_or, int reunit_every,
+                                  int use_graph, void* stream, double* obs) {
+    return guard_alloc("sun_batch_generate", [&] {
+        return sun_batch_generate_body(b, first_sweep, n_iter, n_or, reunit_every, use_graph, stream, obs);
+    });
 }

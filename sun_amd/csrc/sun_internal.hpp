@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <map>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,18 @@
 namespace sun {
 
 int fail(int code, const char* fmt, ...);
+
+// Runs the body of an entry point whose host buffers scale with the lattice or with
+// a caller's count: a failed host allocation becomes SUN_ERR_NOMEM instead of an
+// exception crossing the C ABI.
+template <class F>
+int guard_alloc(const char* what, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(SUN_ERR_NOMEM, "%s: host allocation failed", what);
+    }
+}
 
 #define HIP_TRY(x)                                                                                 \
     do {                                                                                           \

@@ -289,8 +289,8 @@ int load_reference(sun_lattice* L, const char* path, bool header, int save_prec)
 
 }  // namespace
 
-extern "C" int sun_lattice_save(sun_lattice* lat, const char* path, int format, int save_prec, uint64_t seed,
-                                uint32_t next_sweep) {
+static int sun_lattice_save_body(sun_lattice* lat, const char* path, int format, int save_prec, uint64_t seed,
+                                 uint32_t next_sweep) {
     if (!lat || !path) return fail(SUN_ERR_INVALID, "sun_lattice_save: NULL argument");
     HIP_TRY(hipSetDevice(lat->ctx->device));
     HIP_TRY(hipDeviceSynchronize());  // work queued on any stream completes first
@@ -307,8 +307,15 @@ extern "C" int sun_lattice_save(sun_lattice* lat, const char* path, int format, 
     }
 }
 
-extern "C" int sun_lattice_load(sun_lattice* lat, const char* path, int format, int save_prec, uint64_t* seed,
-                                uint32_t* next_sweep) {
+extern "C" int sun_lattice_save(sun_lattice* lat, const char* path, int format, int save_prec, uint64_t seed,
+                                uint32_t next_sweep) {
+    return guard_alloc("sun_lattice_save", [&] {
+        return sun_lattice_save_body(lat, path, format, save_prec, seed, next_sweep);
+    });
+}
+
+static int sun_lattice_load_body(sun_lattice* lat, const char* path, int format, int save_prec, uint64_t* seed,
+                                 uint32_t* next_sweep) {
     if (!lat || !path) return fail(SUN_ERR_INVALID, "sun_lattice_load: NULL argument");
     HIP_TRY(hipSetDevice(lat->ctx->device));
     HIP_TRY(hipDeviceSynchronize());
@@ -323,4 +330,11 @@ extern "C" int sun_lattice_load(sun_lattice* lat, const char* path, int format, 
         default:
             return fail(SUN_ERR_INVALID, "sun_lattice_load: format %d", format);
     }
+}
+
+extern "C" int sun_lattice_load(sun_lattice* lat, const char* path, int format, int save_prec, uint64_t* seed,
+                                uint32_t* next_sweep) {
+    return guard_alloc("sun_lattice_load", [&] {
+        return sun_lattice_load_body(lat, path, format, save_prec, seed, next_sweep);
+    });
 }

@@ -159,7 +159,7 @@ extern "C" int sun_multihit(sun_lattice* in, sun_lattice* out, uint64_t seed, in
     return SUN_OK;
 }
 
-extern "C" int sun_wilson_loops(sun_lattice* lat, int rmax, int tmax, double* out, void* stream) {
+static int sun_wilson_loops_body(sun_lattice* lat, int rmax, int tmax, double* out, void* stream) {
     SUN_TRY(check_lattice(lat, "sun_wilson_loops"));
     if (!out) return fail(SUN_ERR_INVALID, "sun_wilson_loops: NULL output");
     const Geom& g = lat->g;
@@ -192,4 +192,8 @@ extern "C" int sun_wilson_loops(sun_lattice* lat, int rmax, int tmax, double* ou
         out[2 * i + 1] = h[i].y;
     }
     return SUN_OK;
+}
+
+extern "C" int sun_wilson_loops(sun_lattice* lat, int rmax, int tmax, double* out, void* stream) {
+    return guard_alloc("sun_wilson_loops", [&] { return sun_wilson_loops_body(lat, rmax, tmax, out, stream); });
 }
