@@ -50,6 +50,36 @@ def profile_traffic(shape):
     return None, None
 
 
+PROFILE_32 = "su3_32x4_f32_12_iteration_kernels.md"
+PROFILE_64 = "su3_64x4_f32_8_iteration_kernels.md"
+
+
+def profiled_shapes(name):
+    """The overrelaxation and heatbath launch shapes the committed rocprof summary
+    profiles/<name> was taken with, or None.  The benchmark runs its lattices with
+    them, so the summary's traffic is that of the launches timed here (left to the
+    autotuner, near-equal shapes can differ from one run to the next)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("Tuned shapes: overrelaxation "):
+            body = line[len("Tuned shapes: overrelaxation "):].rstrip().rstrip(".")
+            or_part, hb_part = body.split(", heatbath ")
+            return {"overrelax": json.loads(or_part), "heatbath": json.loads(hb_part), "source": f"profiles/{name}"}
+    return None
+
+
+def pin_shapes(lat, shapes):
+    """set the lattice's launch shapes (marks it tuned); returns their source"""
+    if not shapes:
+        return "autotuner"
+    for kernel, key in ((1, "overrelax"), (0, "heatbath")):
+        c = shapes[key]
+        lat.set_launch(kernel, c["block"], tuple(c["brick"]), c["lds"], c["piped"])
+    return shapes["source"]
+
+
 def link_bytes(nc, storage, prec):
     return (2 * nc * nc if storage == 18 else storage) * prec
 
@@ -167,7 +197,7 @@ def plain_and_precision(sun_amd, ctx, stream, obs32, value, updates_per_step):
     return plain, prec
 
 
-def large_leg(sun_amd, ctx, stream, n_iter=6, warm=2):
+def large_leg(sun_amd, ctx, stream, n_iter=6, warm=2, shapes=None):
     """SU(3) 64^4 (16.7M sites, 2.15 GB of links), beta=6.2, FP32 8-real links from a
     Haar start (8 reals cannot hold a cold start); iterations of 1 heatbath + 4
     overrelaxation sweeps + plaquette + Polyakov loop (FP64 sums), graph replay.
@@ -176,6 +206,7 @@ def large_leg(sun_amd, ctx, stream, n_iter=6, warm=2):
     dims = (64, 64, 64, 64)
     V = int(np.prod(dims))
     with sun_amd.Lattice(ctx, 3, dims, 6.2, 8, 4) as lat:
+        shape_source = pin_shapes(lat, shapes)
         lat.hot_start(3, stream)
         lat.generate(seed=3, n_iter=warm, n_or=4, reunit_every=0, stream=stream)
         sun_amd.sync(stream)
@@ -192,6 +223,7 @@ def large_leg(sun_amd, ctx, stream, n_iter=6, warm=2):
                 "overrelax_frac": per_launch / (or_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
                 "heatbath_launch_ms": hb_ms, "heatbath_gbs": per_launch / (hb_ms * 1e-3) / 1e9,
                 "launch_shape": lat.get_launch(1).as_dict(), "heatbath_launch_shape": lat.get_launch(0).as_dict(),
+                "launch_shape_source": shape_source,
                 "plaquette_last": [float(obs[-1, 0]), float(obs[-1, 1])],
                 "polyakov_last": [float(obs[-1, 2]), float(obs[-1, 3])],
                 "config": "SU(3) 64^4 beta=6.2 Haar start, FP32 8-real links; iteration = HB + 4 OR + "
@@ -391,18 +423,19 @@ def measurement_leg(sun_amd, ctx, lat, stream):
     }
 
 
-def kernels_only(sun_amd, large=False):
+def kernels_only(sun_amd, large=False, shapes=None):
     """For rocprofv3 counter passes.  Default: on the 32^4 FP32 12-real field after
     100 iterations, 2 copies of the field (calibration: a plain copy of known size),
     then 8 overrelaxation launches, 8 heatbath launches, reunitarize, plaquette,
     Polyakov loop.  With large: the same on the 64^4 FP32 8-real field (Haar start,
-    beta=6.2) after 20 iterations.  Launch shapes come from the tune cache in
-    $SUN_RESOURCE_PATH (written by a preceding full bench.py run with the same
-    variable), so the profiled kernels are the ones the benchmark times; without a
-    cache the autotuner runs first and its trial launches appear in the profile too."""
+    beta=6.2) after 20 iterations.  Launch shapes: those of the committed summary
+    (as in the benchmark), or with --autotune the tune cache in $SUN_RESOURCE_PATH
+    written by a preceding `bench.py --autotune` run with the same variable; without
+    a cache the autotuner runs first and its trial launches appear in the profile too."""
     ctx = sun_amd.Context(0, 0, 1)
     dims, beta, storage = ((64, 64, 64, 64), 6.2, 8) if large else (DIMS, BETA, 12)
     lat = sun_amd.Lattice(ctx, 3, dims, beta, storage, 4)
+    pin_shapes(lat, shapes)
     tmp = sun_amd.Lattice(ctx, 3, dims, beta, storage, 4)
     s = sun_amd.new_stream()
     if large:
@@ -445,6 +478,8 @@ def main():
     ap.add_argument("--large", action="store_true", help="with --kernels-only: the 64^4 FP32 8-real field")
     ap.add_argument("--kernels-only", action="store_true",
                     help="fixed short launch sequence for profiler counter passes (no JSON line)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="launch shapes from the autotuner instead of those of the committed profiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -453,14 +488,17 @@ def main():
         sys.exit(2)
 
     import sun_amd
+    shapes32 = None if args.autotune else profiled_shapes(PROFILE_32)
+    shapes64 = None if args.autotune else profiled_shapes(PROFILE_64)
     if args.kernels_only:
-        kernels_only(sun_amd, large=args.large)
+        kernels_only(sun_amd, large=args.large, shapes=shapes64 if args.large else shapes32)
         return
     ctx = sun_amd.Context(0, 0, 1)
     V = int(np.prod(DIMS))
     updates_per_step = 5 * 4 * V
     storage, prec = 12, 4
     lat = sun_amd.Lattice(ctx, 3, DIMS, BETA, storage, prec)
+    shape_source = pin_shapes(lat, shapes32)
     stream = sun_amd.new_stream()
     lat.cold_start(stream)
     sun_amd.sync(stream)
@@ -511,6 +549,7 @@ def main():
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": or_ms,
                      "heatbath_launch_ms": hb_ms, "heatbath_gbs": hb_gbs,
                      "launch_shape": launch_or, "heatbath_launch_shape": launch_hb,
+                     "launch_shape_source": shape_source,
                      "iteration_frac": iter_bytes / (step_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "observables": {"plaquette_last": float(obs[-1, :2].mean()), "polyakov_last": [float(obs[-1, 2]),
                                                                                         float(obs[-1, 3])]},
@@ -524,7 +563,7 @@ def main():
         out["plain_fp64"], out["precision"] = plain_and_precision(sun_amd, ctx, stream, obs_all, value,
                                                                   updates_per_step)
     if not args.no_large:
-        out["large_64x4_f32_8"] = large_leg(sun_amd, ctx, stream)
+        out["large_64x4_f32_8"] = large_leg(sun_amd, ctx, stream, shapes=shapes64)
         out["shard_share_64x4_f32_8"] = shard_leg(sun_amd, ctx, stream)
     if not args.no_batch:
         out["batched"] = batched_leg(sun_amd, ctx, stream)

@@ -7,11 +7,14 @@
 // $CULQCD_RESOURCE_PATH/tunecache.csv).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <utility>
+#include <vector>
 
 #include "sun_internal.hpp"
 
@@ -127,55 +130,69 @@ int tune_lattice(sun_lattice* lat, hipStream_t s) {
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     const double bnc = lat->d.beta / lat->d.nc;
-    // (heatbath, overrelaxation) x (plain, software-pipelined where the variant has it)
+    // n launches over mu = 0..3 of candidate c, after one warm-up launch: average ms
+    auto time_cand = [&](bool hb, const LaunchCfg& c, int n, float* ms) {
+        e = lat->ops->update(lat->f, lat->g, 0, 0, hb, 0, nullptr, bnc, 1, c, s);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < n && e == hipSuccess; r++)
+            e = lat->ops->update(lat->f, lat->g, 0, r & 3, hb, 0, nullptr, bnc, 1, c, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+        *ms /= (float)n;
+        return e == hipSuccess;
+    };
     LaunchCfg best_k[2];
     float best_k_ms[2] = {1e30f, 1e30f};
-    // piped: 0 plain kernel, 1 pipelined by directions, 2 pipelined by half-directions
-    for (int which = 0; which < 6 && e == hipSuccess; which++) {
-        const bool hb = (which & 1) == 0;
-        const int piped = which >> 1;
-        if (piped && !lat->ops->has_piped) continue;
-        const int max_block = piped ? lat->ops->piped_max_block : lat->ops->max_block;
-        LaunchCfg best;
-        float best_ms = 1e30f;
-        const int nshape = (int)(sizeof(kShapes) / sizeof(Shape)), nlds = (int)(sizeof(kLds) / sizeof(int));
-        for (int cand = 0; cand < nshape * nlds; cand++) {
-            const Shape& sh = kShapes[cand / nlds];
-            LaunchCfg c;
-            c.b1 = sh.b1;
-            c.b2 = sh.b2;
-            c.b3 = sh.b3;
-            c.block = sh.b1 < 0 ? sh.block : brick_block(lat->g, sh.b1, sh.b2, sh.b3, max_block);
-            c.lds = kLds[cand % nlds];
-            c.piped = piped;
-            if (c.block == 0 || c.block > max_block) continue;
-            // warm-up, then 2 timed passes over mu = 0..3
-            e = lat->ops->update(lat->f, lat->g, 0, 0, hb, 0, nullptr, bnc, 1, c, s);
-            if (e == hipSuccess) e = hipEventRecord(e0, s);
-            for (int r = 0; r < 8 && e == hipSuccess; r++)
-                e = lat->ops->update(lat->f, lat->g, 0, r & 3, hb, 0, nullptr, bnc, 1, c, s);
-            if (e == hipSuccess) e = hipEventRecord(e1, s);
-            if (e == hipSuccess) e = hipEventSynchronize(e1);
-            float ms = 0.f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-            if (e != hipSuccess) break;
-            ms /= 8.f;
-            if (ctx->verbosity >= SUN_VERBOSE)
-                fprintf(stderr, "[sun tune] %s block=%d brick=%d,%d,%d lds=%d %.4f ms\n",
-                        hb ? (piped == 2 ? "heatbath-piped2" : piped ? "heatbath-piped" : "heatbath")
-                           : (piped == 2 ? "overrelax-piped2" : piped ? "overrelax-piped" : "overrelax"),
-                        c.block, c.b1, c.b2, c.b3, c.lds, ms);
-            // candidates come in increasing LDS cap and shape order; a later one must
-            // win by more than 1% (timing noise), so the choice is stable run to run
-            if (ms < best_ms * 0.99f) {
-                best_ms = ms;
-                best = c;
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        const bool hb = k == 0;
+        // coarse pass: every (kernel, shape, LDS cap) candidate, 8 launches each; piped
+        // 0 plain kernel, 1 pipelined by directions, 2 pipelined by half-directions
+        std::vector<std::pair<LaunchCfg, float>> cands;
+        for (int piped = 0; piped < 3 && e == hipSuccess; piped++) {
+            if (piped && !lat->ops->has_piped) continue;
+            const int max_block = piped ? lat->ops->piped_max_block : lat->ops->max_block;
+            const int nshape = (int)(sizeof(kShapes) / sizeof(Shape)), nlds = (int)(sizeof(kLds) / sizeof(int));
+            for (int cand = 0; cand < nshape * nlds; cand++) {
+                const Shape& sh = kShapes[cand / nlds];
+                LaunchCfg c;
+                c.b1 = sh.b1;
+                c.b2 = sh.b2;
+                c.b3 = sh.b3;
+                c.block = sh.b1 < 0 ? sh.block : brick_block(lat->g, sh.b1, sh.b2, sh.b3, max_block);
+                c.lds = kLds[cand % nlds];
+                c.piped = piped;
+                if (c.block == 0 || c.block > max_block) continue;
+                float ms = 0.f;
+                if (!time_cand(hb, c, 8, &ms)) break;
+                if (ctx->verbosity >= SUN_VERBOSE)
+                    fprintf(stderr, "[sun tune] %s piped=%d block=%d brick=%d,%d,%d lds=%d %.4f ms\n",
+                            hb ? "heatbath" : "overrelax", piped, c.block, c.b1, c.b2, c.b3, c.lds, ms);
+                cands.push_back({c, ms});
             }
         }
-        if (e == hipSuccess && best_ms < best_k_ms[hb ? 0 : 1] * 0.99f) {
-            best_k[hb ? 0 : 1] = best;
-            best_k_ms[hb ? 0 : 1] = best_ms;
-        }
+        if (e != hipSuccess || cands.empty()) break;
+        // refinement: the candidates within 4% of the fastest (at most 8, in candidate
+        // order) timed again in three interleaved rounds of 16 launches; the lowest
+        // total wins, a later candidate only by more than 0.5%.  Near-equal shapes
+        // otherwise swap places from run to run with the timing noise of 8 launches.
+        float coarse_best = 1e30f;
+        for (const auto& c : cands) coarse_best = std::min(coarse_best, c.second);
+        std::vector<LaunchCfg> fin;
+        for (const auto& c : cands)
+            if (c.second <= coarse_best * 1.04f && fin.size() < 8) fin.push_back(c.first);
+        std::vector<float> total(fin.size(), 0.f);
+        for (int round = 0; round < 3 && e == hipSuccess; round++)
+            for (size_t i = 0; i < fin.size() && e == hipSuccess; i++) {
+                float ms = 0.f;
+                if (time_cand(hb, fin[i], 16, &ms)) total[i] += ms / 3.f;
+            }
+        if (e != hipSuccess) break;
+        size_t win = 0;
+        for (size_t i = 1; i < fin.size(); i++)
+            if (total[i] < total[win] * 0.995f) win = i;
+        best_k[k] = fin[win];
+        best_k_ms[k] = total[win];
     }
     if (e == hipSuccess) {
         lat->cfg_hb = best_k[0];
