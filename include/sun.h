@@ -185,6 +185,15 @@ int sun_multihit(sun_lattice *in, sun_lattice *out, uint64_t seed, int nhit, voi
  * random numbers: that kernel's heatbath call is commented out); temporal links
  * copied from in (the reference leaves out's).  out as for sun_multihit. */
 int sun_multihit_space(sun_lattice *in, sun_lattice *out, int nhit, void *stream);
+/* ApplyMultiHitExt (multihitext.cu:395-459, LEN_SPACE 2, LEN_TIME 1): out = in with
+ * every temporal link U_3(x) replaced by the mean, over nhit >= 1 checkerboard heatbath
+ * sweeps of the 27 temporal links of the spatial box x + (i, j, k, 0), |i|,|j|,|k| <= 1
+ * (box parity (i+j+k)&1, 0 first; every other link frozen at its input value), of the
+ * box centre after each sweep (beta of `in`, RNG stream (seed, site, hit, box link));
+ * spatial links copied from in.  out as for sun_multihit.  The reference requires
+ * normal-order arrays and a halo-padded copy per sub-lattice pass; here one launch on
+ * the even/odd field, each site's box in LDS. */
+int sun_multihit_ext(sun_lattice *in, sun_lattice *out, uint64_t seed, int nhit, void *stream);
 /* WilsonLoop (wilsonloop.cu:494-516): out[2 (T + r (tmax+1)) + {0,1}] = Re, Im of
  * W(r, T) = sum_{x, mu<3} Tr(L_r(x) t1 L_r(x+T e_3)^+ t0^+) / (3 N V) for r <= rmax,
  * T <= tmax (rmax at most the smallest spatial extent, tmax at most the temporal). */

@@ -379,6 +379,11 @@ void ApplyMultiHitSpace(_gauge<Real>& in, _gauge<Real>& out, RNG& rng, int nhit)
     (void)rng;
     check(sun_multihit_space(in.handle(), out.handle(), nhit, nullptr), "ApplyMultiHitSpace");
 }
+/* ApplyMultiHitExt(in, out, RNG, nhit)   multihitext.cu:449-456; out must be SOA (18-real) */
+template <class Real>
+void ApplyMultiHitExt(_gauge<Real>& in, _gauge<Real>& out, RNG& rng, int nhit) {
+    check(sun_multihit_ext(in.handle(), out.handle(), rng.Seed(), nhit, nullptr), "ApplyMultiHitExt");
+}
 /* WilsonLoop<Real>(gauge, res, Rmax, Tmax)   wilsonloop.cu:494-516:
    res[it + r * (Tmax + 1)] = W(r, it), r <= Rmax, it <= Tmax */
 template <class Real>

@@ -76,6 +76,7 @@ def lib() -> ctypes.CDLL:
         "or_exp_iq": (None, [dp, dp]),
         "or_stout_smear": (ci, [ip, dp, cd, ci, ci]),
         "or_multihit_space": (ci, [ci, ip, ci, dp, dp]),
+        "or_multihit_ext": (ci, [ci, ip, cd, u64, ci, dp, dp]),
         "or_hyp_smear": (ci, [ci, ip, dp, cd, cd, cd, ci, ci, cd, ci]),
         "or_wilson_loops": (ci, [ci, ip, dp, ci, ci, dp]),
         "or_matmul": (None, [ci, dp, dp, dp]),
@@ -330,6 +331,12 @@ def hyp_smear(U, dims, alphas=(0.75, 0.6, 0.3), steps: int = 1, nhit: int = 10, 
 def multihit(U, dims, beta: float, seed: int, nhit: int):
     out = np.empty_like(U)
     _check(lib().or_multihit(U.shape[-1], _dims(dims), beta, seed, nhit, _dp(U), _dp(out)), "multihit")
+    return out
+
+
+def multihit_ext(U, dims, beta: float, seed: int, nhit: int):
+    out = np.empty_like(U)
+    _check(lib().or_multihit_ext(U.shape[-1], _dims(dims), beta, seed, nhit, _dp(U), _dp(out)), "multihit_ext")
     return out
 
 

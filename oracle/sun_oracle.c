@@ -25,6 +25,7 @@
 #define RNG_TAG_HOT  0x20000000u
 #define RNG_TAG_TEST 0x30000000u
 #define RNG_TAG_MH   0x40000000u
+#define RNG_TAG_MHX  0x50000000u
 
 /* ------------------------------------------------------------------------- */
 /* complex arithmetic in the reference's operation order (complex.h:82-230)    */
@@ -1346,6 +1347,98 @@ int or_multihit_space(int nc, const int X[4], int nhit, const double *Uin, doubl
                 for (int j = 0; j < nc; j++) acc.e[i][j] = cmk(acc.e[i][j].re / nhit, acc.e[i][j].im / nhit);
             mstore(nc, &acc, link_at_w(nc, Uout, V, mu, idd));
         }
+    }
+    return 0;
+}
+
+/* multihitext.cu:241-330 (kernel_multihitext) with :395-459 (ApplyMultiHitExt: mu = 3,
+ * LEN_SPACE 2, LEN_TIME 1).  For every site x, on a copy of the input: the 27 temporal
+ * links of the spatial box x + (i, j, k, 0), |i|, |j|, |k| <= 1 (time slice of x),
+ * receive nhit checkerboard heatbath sweeps -- parity (i + j + k) & 1 first 0 then 1,
+ * box order i fastest, as the reference's loops -- in staples whose other links keep
+ * their input values; the output temporal link at x is the mean of the centre link
+ * after each sweep.  Spatial links copied.  Box link b = (i+1) + 3 (j+1) + 9 (k+1)
+ * of site idd (even/odd index) in hit h draws from stream (seed, idd, h, TAG_MHX | b).
+ * The reference runs the sites in 128 launches over a halo-extended copy refreshed
+ * before each: a site's result depends only on the input and its own random numbers. */
+static int mhx_in_box(const int o[3]) {
+    return o[0] >= -1 && o[0] <= 1 && o[1] >= -1 && o[1] <= 1 && o[2] >= -1 && o[2] <= 1;
+}
+static int mhx_index(const int o[3]) { return (o[0] + 1) + 3 * (o[1] + 1) + 9 * (o[2] + 1); }
+
+int or_multihit_ext(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout) {
+    int rc = check_args(nc, X, OR_STORE_18);
+    if (rc) return rc;
+    if (nhit < 1) return -5;
+    long V = volume_of(X), half = V / 2;
+    memcpy(Uout, Uin, sizeof(double) * (size_t)(4 * V * nc * nc * 2));
+    double bnc = beta / (double)nc;
+#pragma omp parallel for schedule(static)
+    for (long idd = 0; idd < V; idd++) {
+        int parity = idd >= half ? 1 : 0;
+        int x[4];
+        or_eo_to_coords((int)(idd - parity * half), parity, X, x);
+        mat box[27];
+        for (int b = 0; b < 27; b++) {
+            int y[4] = {(x[0] + b % 3 - 1 + X[0]) % X[0], (x[1] + (b / 3) % 3 - 1 + X[1]) % X[1],
+                        (x[2] + b / 9 - 1 + X[2]) % X[2], x[3]};
+            mload(nc, link_at(nc, Uin, V, 3, or_coords_to_eo(y, X)), &box[b]);
+        }
+        mat acc;
+        mzero(nc, &acc);
+        for (int hit = 0; hit < nhit; hit++) {
+            for (int par = 0; par < 2; par++)
+                for (int b = 0; b < 27; b++) {
+                    int o[3] = {b % 3 - 1, (b / 3) % 3 - 1, b / 9 - 1};
+                    if (((o[0] + o[1] + o[2]) & 1) != par) continue;
+                    int y[4] = {x[0] + o[0], x[1] + o[1], x[2] + o[2], x[3]};
+                    mat staple, link, t, A, B, C;
+                    mzero(nc, &staple);
+                    for (int nu = 0; nu < 3; nu++) {
+                        int yp[4], yt[4], ym[4], ymt[4];
+                        for (int d = 0; d < 4; d++) yp[d] = yt[d] = ym[d] = ymt[d] = y[d];
+                        yp[nu] += 1;
+                        yt[3] += 1;
+                        ym[nu] -= 1;
+                        ymt[nu] -= 1;
+                        ymt[3] += 1;
+                        int op[3] = {o[0], o[1], o[2]}, om[3] = {o[0], o[1], o[2]};
+                        op[nu] += 1;
+                        om[nu] -= 1;
+                        int *pts[4] = {yp, yt, ym, ymt};
+                        for (int q = 0; q < 4; q++)
+                            for (int d = 0; d < 4; d++) pts[q][d] = (pts[q][d] + X[d]) % X[d];
+                        int yy[4] = {(y[0] + X[0]) % X[0], (y[1] + X[1]) % X[1], (y[2] + X[2]) % X[2], y[3]};
+                        /* up: U_nu(y) U_3(y+nu) U_nu(y+3)^dagger */
+                        mload(nc, link_at(nc, Uin, V, nu, or_coords_to_eo(yy, X)), &A);
+                        if (mhx_in_box(op)) B = box[mhx_index(op)];
+                        else mload(nc, link_at(nc, Uin, V, 3, or_coords_to_eo(yp, X)), &B);
+                        mload(nc, link_at(nc, Uin, V, nu, or_coords_to_eo(yt, X)), &C);
+                        link = mmul(nc, &A, &B);
+                        t = mdag(nc, &C);
+                        link = mmul(nc, &link, &t);
+                        madd_to(nc, &staple, &link);
+                        /* down: U_nu(y-nu)^dagger U_3(y-nu) U_nu(y-nu+3) */
+                        mload(nc, link_at(nc, Uin, V, nu, or_coords_to_eo(ym, X)), &A);
+                        if (mhx_in_box(om)) B = box[mhx_index(om)];
+                        else mload(nc, link_at(nc, Uin, V, 3, or_coords_to_eo(ym, X)), &B);
+                        mload(nc, link_at(nc, Uin, V, nu, or_coords_to_eo(ymt, X)), &C);
+                        link = mdag(nc, &A);
+                        link = mmul(nc, &link, &B);
+                        link = mmul(nc, &link, &C);
+                        madd_to(nc, &staple, &link);
+                    }
+                    mat F = mdag(nc, &staple);
+                    usrc st;
+                    usrc_stream(&st, seed, (uint32_t)idd, (uint32_t)hit, RNG_TAG_MHX | (uint32_t)b);
+                    heatbath_mat(nc, bnc, &box[b], &F, &st);
+                }
+            madd_to(nc, &acc, &box[13]);
+        }
+        for (int i = 0; i < nc; i++)
+            for (int j = 0; j < nc; j++)
+                acc.e[i][j] = cmk(acc.e[i][j].re / (double)nhit, acc.e[i][j].im / (double)nhit);
+        mstore(nc, &acc, link_at_w(nc, Uout, V, 3, idd));
     }
     return 0;
 }

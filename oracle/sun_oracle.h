@@ -116,6 +116,11 @@ int or_multihit(int nc, const int dims[4], double beta, uint64_t seed, int nhit,
 /* multihitsp.cu:115-161: Uout = Uin with every spatial link replaced by the mean of
  * nhit successive overrelaxation steps in its spatial staple */
 int or_multihit_space(int nc, const int dims[4], int nhit, const double *Uin, double *Uout);
+/* multihitext.cu:241-330, :395-459: Uout = Uin with each temporal link replaced by the
+ * mean, over nhit checkerboard heatbath sweeps of the 27 temporal links of its 3^3
+ * spatial box (other links frozen at their input values), of the box centre after
+ * each sweep (streams (seed, idd, hit, TAG_MHX | box index)) */
+int or_multihit_ext(int nc, const int dims[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout);
 /* wilsonloop.cu:44-121,494-516: out[2(T + r(tmax+1)) + {0,1}] = W(r, T), r <= rmax,
  * T <= tmax, averaged over sites and the three spatial directions, / (3 N V) */
 int or_wilson_loops(int nc, const int dims[4], const double *U, int rmax, int tmax, double *out);

@@ -44,7 +44,8 @@ EXPORTS = [
     "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
     "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
     "sun_link_monitor", "sun_link_monitor_async", "sun_link_trace", "sun_link_det",
-    "sun_ape_smear", "sun_stout_smear", "sun_hyp_smear", "sun_multihit", "sun_multihit_space", "sun_wilson_loops",
+    "sun_ape_smear", "sun_stout_smear", "sun_hyp_smear", "sun_multihit", "sun_multihit_space", "sun_multihit_ext",
+    "sun_wilson_loops",
     "sun_links_to_host", "sun_links_from_host",
     "sun_generate",
     "sun_batch_create", "sun_batch_destroy", "sun_batch_get_launch", "sun_batch_set_launch",
@@ -159,6 +160,7 @@ def lib() -> ctypes.CDLL:
         "sun_hyp_smear": (ci, [vp, cd, cd, cd, ci, ci, cd, ci, vp]),
         "sun_multihit": (ci, [vp, vp, u64, ci, vp]),
         "sun_multihit_space": (ci, [vp, vp, ci, vp]),
+        "sun_multihit_ext": (ci, [vp, vp, u64, ci, vp]),
         "sun_wilson_loops": (ci, [vp, ci, ci, dp, vp]),
         "sun_links_to_host": (ci, [vp, dp, ci]),
         "sun_links_from_host": (ci, [vp, dp, ci]),
@@ -369,6 +371,11 @@ class Lattice:
     def multihit_space(self, out: "Lattice", nhit: int, stream=None):
         """out (18-real) <- this field with spatial links averaged over nhit overrelaxation steps"""
         _check(lib().sun_multihit_space(self.h, out.h, nhit, stream), "sun_multihit_space")
+
+    def multihit_ext(self, out: "Lattice", seed: int, nhit: int, stream=None):
+        """out (18-real) <- this field with temporal links averaged over nhit heatbath sweeps of each site's 3^3
+        box of temporal links (multihitext.cu)"""
+        _check(lib().sun_multihit_ext(self.h, out.h, seed, nhit, stream), "sun_multihit_ext")
 
     def wilson_loops(self, rmax: int, tmax: int, stream=None) -> np.ndarray:
         """complex [rmax + 1, tmax + 1]: W(r, T)"""

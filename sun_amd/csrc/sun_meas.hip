@@ -1,8 +1,8 @@
 // sun_meas.hip -- the reference's measurement chain on a resident lattice (host
 // code): APE smearing (smear/ape.cu:198-264), stout smearing (smear/stout.cu:687-766),
 // HYP smearing (smear/hyp.cu:277-368), spatial multihit (smear/multihitsp.cu:246-266),
-// multihit of temporal links
-// (smear/multihit.cu:291-333) and Wilson loops (meas/wilsonloop.cu:494-516).
+// multihit of temporal links (smear/multihit.cu:291-333), extended multihit
+// (smear/multihitext.cu:395-459) and Wilson loops (meas/wilsonloop.cu:494-516).
 // Scratch (the ping-pong field of APE, the Wilson lines and partial sums) is
 // allocated per call: these are measurements between generation steps, not part
 // of the generation loop.
@@ -155,6 +155,16 @@ extern "C" int sun_multihit(sun_lattice* in, sun_lattice* out, uint64_t seed, in
     hipStream_t s = pick_stream(in, stream);
     HIP_TRY(hipSetDevice(in->ctx->device));
     HIP_TRY(in->ops->multihit(in->f, out->f, in->g, seed, nhit, in->d.beta / (double)in->d.nc, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SUN_OK;
+}
+
+extern "C" int sun_multihit_ext(sun_lattice* in, sun_lattice* out, uint64_t seed, int nhit, void* stream) {
+    SUN_TRY(check_average_output(in, out, "sun_multihit_ext"));
+    if (nhit < 1) return fail(SUN_ERR_INVALID, "sun_multihit_ext: nhit %d (the mean needs at least one)", nhit);
+    hipStream_t s = pick_stream(in, stream);
+    HIP_TRY(hipSetDevice(in->ctx->device));
+    HIP_TRY(in->ops->multihit_ext(in->f, out->f, in->g, seed, nhit, in->d.beta / (double)in->d.nc, s));
     HIP_TRY(hipStreamSynchronize(s));
     return SUN_OK;
 }

@@ -158,6 +158,10 @@ struct OpsTable {
                            hipStream_t);
     // spatial links -> mean of nhit overrelaxation steps, temporal copied (out 18-real)
     hipError_t (*multihit_space)(const Field& in, const Field& out, const Geom&, int nhit, hipStream_t);
+    // temporal links -> mean of the centre of nhit heatbath sweeps of each site's 3^3
+    // box of temporal links, spatial copied (out 18-real)
+    hipError_t (*multihit_ext)(const Field& in, const Field& out, const Geom&, uint64_t seed, int nhit, double bnc,
+                               hipStream_t);
     hipError_t (*wl_lines)(const Field& f, const Field& lines, const Geom&, int r, hipStream_t);
     hipError_t (*wl_loops)(const Field& f, const Field& lines, const Geom&, int r, int tmax, double2* partials,
                            hipStream_t);

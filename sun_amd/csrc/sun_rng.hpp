@@ -17,6 +17,7 @@ enum : uint32_t {
     kTagHotStart = 0x20000000u,
     kTagTest = 0x30000000u,
     kTagMultihit = 0x40000000u,
+    kTagMultihitExt = 0x50000000u,
 };
 
 struct Philox {

@@ -37,6 +37,7 @@ void use_all(_gauge<Real>& U, _gauge<Real>& out, RNG& rng) {
     ApplyHYPinTime<Real>(U, 1, 10, Real(1e-10), ParamHYP(1.f, 0.5f, 0.5f));
     ApplyMultiHit<Real>(U, out, rng, 4);
     ApplyMultiHitSpace<Real>(U, out, rng, 4);
+    ApplyMultiHitExt<Real>(U, out, rng, 2);
     std::complex<double> res[4];
     WilsonLoop<Real>(out, res, 1, 1);
     CWilsonLoop<Real>(out, res, 1, 1);

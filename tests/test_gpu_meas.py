@@ -1,7 +1,8 @@
 """The measurement chain on the GPU against the FP64 oracle: APE smearing
 (ape.cu), stout smearing (stout.cu, SU(3) variants), HYP smearing (hyp.cu; the
 device computes each decoration level once, the oracle recurses as the reference
-does), multihit (multihit.cu) and Wilson loops (wilsonloop.cu), every variant.
+does), multihit (multihit.cu), extended multihit (multihitext.cu) and Wilson loops
+(wilsonloop.cu), every variant.
 
 Bars: FP64 1e-12 per link (smearing, multihit) and 1e-13 on W(r, T); FP32 5e-5 per
 link plus four times the storage round trip of the oracle's result (8-real FP32
@@ -124,6 +125,43 @@ def test_multihit_space_parity(ctx, oracle, nc, storage, prec):
 
 
 @pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
+def test_multihit_ext_parity(ctx, oracle, nc, storage, prec):
+    """extended multihit (multihitext.cu): nhit checkerboard heatbath sweeps of each
+    site's 3^3 box of temporal links; the device runs 16 lanes per site with the box in
+    LDS, the oracle the sites one after another"""
+    beta, seed, nhit = BETAS[nc], 73, 3
+    with sun_amd.Lattice(ctx, nc, DIMS, beta, storage, prec) as lat, \
+            sun_amd.Lattice(ctx, nc, DIMS, beta, 18, prec) as out:
+        U = load_config(lat, thermalised(oracle, nc, DIMS, storage))
+        lat.multihit_ext(out, seed, nhit)
+        got = out.to_host()
+        with pytest.raises(sun_amd.SunError) as e:
+            lat.multihit_ext(out, seed, 0)
+        assert e.value.code == sun_amd.ERR_INVALID
+    ref = oracle.multihit_ext(U, DIMS, beta, seed, nhit)
+    assert np.abs(got[:3] - U[:3]).max() <= (0.0 if prec == 8 else store_tol(storage, prec))
+    d = per_link_diff(got[3:], ref[3:])
+    if prec == 8:
+        assert d.max() <= 1e-12, float(d.max())
+    else:
+        # FP32 heatbath chains: an accept/reject branch may flip against the oracle
+        assert int((d > 5e-5).sum()) <= max(2, d.size // 100), float(d.max())
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2, 4), (2, 4, 6, 2), (6, 2, 4, 4)])
+def test_multihit_ext_small_extents(ctx, oracle, dims):
+    """extents of 2 (box offsets -1 and +1 are one physical link, kept as two box
+    entries as the reference's padded copy does) and mixed extents, FP64 SU(3)"""
+    with sun_amd.Lattice(ctx, 3, dims, 5.8, 18, 8) as lat, sun_amd.Lattice(ctx, 3, dims, 5.8, 18, 8) as out:
+        lat.hot_start(9)
+        U = lat.to_host()
+        lat.multihit_ext(out, 11, 2)
+        got = out.to_host()
+    ref = oracle.multihit_ext(U, dims, 5.8, 11, 2)
+    assert per_link_diff(got, ref).max() <= 1e-12
+
+
+@pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
 def test_wilson_loops_parity(ctx, oracle, nc, storage, prec):
     rmax, tmax = 3, 5
     with sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], storage, prec) as lat:
@@ -152,9 +190,10 @@ def test_multihit_refuses_bad_outputs(ctx):
             sun_amd.Lattice(ctx, 3, DIMS, 6.0, 18, 8) as o64, sun_amd.Lattice(ctx, 3, (4, 6, 4, 6), 6.0, 18, 4) as od:
         lat.cold_start()
         for out in (lat, o12, o64, od):
-            with pytest.raises(sun_amd.SunError) as e:
-                lat.multihit(out, 1, 3)
-            assert e.value.code == sun_amd.ERR_INVALID
+            for call in (lambda: lat.multihit(out, 1, 3), lambda: lat.multihit_ext(out, 1, 3)):
+                with pytest.raises(sun_amd.SunError) as e:
+                    call()
+                assert e.value.code == sun_amd.ERR_INVALID
         with pytest.raises(sun_amd.SunError):
             lat.ape_smear(0.2, -1, 10)
 

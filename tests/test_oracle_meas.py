@@ -471,3 +471,98 @@ def test_multihit_space_closed_forms(oracle):
         assert np.abs(Mn[:3] - (O1[:3] + W[:3]) / 2).max() < 1e-14
     with pytest.raises(ValueError):
         oracle.multihit_space(W, dims, 0)
+
+
+# ---- extended multihit (multihitext.cu): local heatbath sweeps of a 3^3 box ----
+def eo_index(x, dims):
+    X0, X1, X2, X3 = dims
+    lex = x[0] + X0 * (x[1] + X1 * (x[2] + X2 * x[3]))
+    return lex // 2 + (sum(x) & 1) * (int(np.prod(dims)) // 2)
+
+
+def multihit_ext_extended_lattice(o, U, dims, beta, seed, nhit):
+    """The reference's own scheme (multihitext.cu:65-108, :122-144, :241-330, :395-429)
+    in numpy: a halo-padded copy (padding 2 in every direction) is refreshed from the
+    input before each of the 4*4*4*2 sub-lattice passes; in a pass every site of the
+    sub-lattice heatbaths the temporal links of its box in place in the copy.  The
+    single-link update is the pinned heatbath_link on stream
+    (seed, eo(x), hit, 0x50000000 | box index)."""
+    nc = U.shape[-1]
+    L = to_lex(U, dims)  # [mu, x3, x2, x1, x0]
+    lgrid, pad = (4, 4, 4, 2), 2
+    blocks = [(dims[d] + lgrid[d] - 1) // lgrid[d] for d in range(4)]
+    grid = [blocks[d] * lgrid[d] + 2 * pad for d in range(4)]
+    # extended index e holds the input at (e - pad) mod X
+    idx = [(np.arange(grid[d]) - pad) % dims[d] for d in range(4)]
+    out = np.array(L)
+    acc = np.zeros((dims[3], dims[2], dims[1], dims[0], nc, nc), dtype=complex)
+
+    def ext(C, mu, e):
+        return C[mu, e[3], e[2], e[1], e[0]]
+
+    for sub in range(lgrid[0] * lgrid[1] * lgrid[2] * lgrid[3]):
+        lid = (sub % 4, (sub // 4) % 4, (sub // 16) % 4, sub // 64)
+        C = L[:, idx[3]][:, :, idx[2]][:, :, :, idx[1]][:, :, :, :, idx[0]].copy()
+        for b3 in range(blocks[3]):
+            for b2 in range(blocks[2]):
+                for b1 in range(blocks[1]):
+                    for b0 in range(blocks[0]):
+                        x = [lid[0] + 4 * b0, lid[1] + 4 * b1, lid[2] + 4 * b2, lid[3] + 2 * b3]
+                        if any(x[d] >= dims[d] for d in range(4)):
+                            continue
+                        e = [x[d] + pad for d in range(4)]
+                        site = eo_index(x, dims)
+                        newU = np.zeros((nc, nc), dtype=complex)
+                        for hit in range(nhit):
+                            for parity in range(2):
+                                for k in (-1, 0, 1):
+                                    for j in (-1, 0, 1):
+                                        for i in (-1, 0, 1):
+                                            if ((i + j + k) & 1) != parity:
+                                                continue
+                                            y = [e[0] + i, e[1] + j, e[2] + k, e[3]]
+                                            S = np.zeros((nc, nc), dtype=complex)
+                                            for nu in range(3):
+                                                yn = list(y); yn[nu] += 1
+                                                yt = list(y); yt[3] += 1
+                                                ym = list(y); ym[nu] -= 1
+                                                ymt = list(ym); ymt[3] += 1
+                                                S += ext(C, nu, y) @ ext(C, 3, yn) @ dag(ext(C, nu, yt))
+                                                S += dag(ext(C, nu, ym)) @ ext(C, 3, ym) @ ext(C, nu, ymt)
+                                            b = (i + 1) + 3 * (j + 1) + 9 * (k + 1)
+                                            C[3, y[3], y[2], y[1], y[0]] = o.heatbath_link(
+                                                ext(C, 3, y).copy(), dag(S), beta, seed, c0=site, c2=hit,
+                                                c3=0x50000000 | b)
+                            newU += ext(C, 3, e)
+                        acc[x[3], x[2], x[1], x[0]] = newU / nhit
+    out[3] = acc
+    return from_lex(out, dims)
+
+
+@pytest.mark.parametrize("nc,dims", [(2, (4, 4, 4, 2)), (3, (4, 2, 4, 4)), (4, (2, 4, 4, 2))])
+def test_multihit_ext_matches_extended_lattice_restatement(oracle, nc, dims):
+    U = oracle.hot_start(nc, dims, 60 + nc)
+    beta = {2: 2.4, 3: 5.9, 4: 10.8}[nc]
+    M = oracle.multihit_ext(U, dims, beta, 77, 2)
+    R = multihit_ext_extended_lattice(oracle, U, dims, beta, 77, 2)
+    assert np.array_equal(M[:3], U[:3])
+    # the numpy staple sums in another order: rounding only, no accept/reject flips
+    assert np.abs(M[3] - R[3]).max() < 1e-12
+
+
+def test_multihit_ext_cold_large_beta_and_unitary_single_hit(oracle):
+    dims = (4, 4, 4, 4)
+    # cold field, beta huge: every box update stays at the staple's direction (1)
+    C = oracle.cold_start(3, dims)
+    M = oracle.multihit_ext(C, dims, 1e7, 5, 3)
+    assert np.abs(M[3] - np.eye(3)).max() < 1e-2 and np.array_equal(M[:3], C[:3])
+    # one hit: the output link is one heatbath result, in SU(N)
+    U = oracle.hot_start(3, dims, 8)
+    M1 = oracle.multihit_ext(U, dims, 6.0, 5, 1)
+    assert np.abs(M1[3] @ np.conj(np.swapaxes(M1[3], 1, 2)) - np.eye(3)).max() < 1e-12
+    assert np.abs(np.linalg.det(M1[3]) - 1).max() < 1e-12
+    # a site's output depends on the seed; different seeds differ
+    M2 = oracle.multihit_ext(U, dims, 6.0, 6, 1)
+    assert np.abs(M2[3] - M1[3]).max() > 1e-3
+    with pytest.raises(ValueError):
+        oracle.multihit_ext(U, dims, 6.0, 5, 0)
