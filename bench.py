@@ -390,7 +390,8 @@ def measurement_leg(sun_amd, ctx, lat, stream):
     """The reference's static-potential chain (test.cpp:170-230) on the generated 32^4
     FP32-12 field: link monitor (before and after a reunitarisation), APE smearing in
     space (w 0.2, 25 steps, 10 hits),
-    multihit (100 hits) into an 18-real FP32 lattice and Wilson loops r, T <= 16."""
+    multihit (100 hits) into an 18-real FP32 lattice, Wilson loops r, T <= 16, and the
+    extended multihit (10 heatbath sweeps of every site's 3^3 box of temporal links)."""
     V = int(np.prod(DIMS))
     reps = 20
     # the launch timings before this leg ran sweeps without reunitarisation
@@ -410,6 +411,9 @@ def measurement_leg(sun_amd, ctx, lat, stream):
         t0 = time.perf_counter()
         W = mh.wilson_loops(16, 16, stream=stream)
         wl_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        lat.multihit_ext(mh, 7, 10, stream=stream)
+        mhx_s = time.perf_counter() - t0
     return {
         "link_monitor": {"ms": mon_s * 1e3, "read_gbs": 4 * V * link_bytes(3, 12, 4) / mon_s / 1e9,
                          "max_unitarity_dev": float(mon[4]), "mean_det": [float(mon[2]), float(mon[3])],
@@ -418,6 +422,8 @@ def measurement_leg(sun_amd, ctx, lat, stream):
         "ape_space_25_steps_s": ape_s,
         "multihit_100_hits_s": mh_s,
         "wilson_loops_r16_t16_s": wl_s,
+        "multihit_ext_10_hits_s": mhx_s,
+        "multihit_ext_link_updates_per_s": 27 * 10 * V / mhx_s,
         "W_r_T4": [float(W[r, 4].real) for r in range(1, 9)],
         "W_r4_T": [float(W[4, t].real) for t in range(1, 9)],
     }

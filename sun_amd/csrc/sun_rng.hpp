@@ -29,7 +29,13 @@ struct Philox {
             uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
             uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
             uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+#ifdef __HIP_DEVICE_COMPILE__
+            // three-input XOR in one instruction (gfx950 v_bitop3_b32, truth table 0x96)
+            uint32_t n0 = __builtin_amdgcn_bitop3_b32(hi1, c1, k0, 0x96);
+            uint32_t n2 = __builtin_amdgcn_bitop3_b32(hi0, c3, k1, 0x96);
+#else
             uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+#endif
             c0 = n0;
             c1 = lo1;
             c2 = n2;
