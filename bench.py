@@ -348,7 +348,24 @@ def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
                 cs.wait(jid, 120.0)
                 st = cs.status(jid)
                 calib[(nc, dims)] = 20 * 5 * 4 * int(np.prod(dims)) / max(st.t_end - st.t_start, 1e-6)
+        # steady state of the served engine on the headline lattice: one SU(3) 32^4
+        # FP32-12 job of 220 iterations alone (the 20-iteration jobs above carry the
+        # job's set-up -- table upload, graph capture, start kernels -- in their time)
+        n_steady = 220
+        jid = cs.submit(3, (32, 32, 32, 32), 6.0, 2, n_steady, n_or=4, reunit_every=1, storage=12, precision=4,
+                        hot_start=False)
+        cs.wait(jid, 120.0)
+        st = cs.status(jid)
+        steady_s = max(st.t_end - st.t_start, 1e-6)
         cs.stop()
+    V32 = 32 ** 4
+    steady = {"iterations": n_steady, "s": steady_s,
+              "link_updates_per_s": n_steady * 5 * 4 * V32 / steady_s,
+              # HB + 4 OR (9 link transfers per update), reunitarisation (2 per link),
+              # plaquette (1) and Polyakov loop (1/4): 48.25 x 4V links of 48 B per iteration
+              "iteration_frac": n_steady * 4 * V32 * 48 * 48.25 / steady_s / (PEAK_HBM_GBS * 1e9),
+              "note": "one SU(3) 32^4 FP32-12 job alone through the job server (batched kernels, hipGraph), "
+                      "HB + 4 OR + reunit + plaquette + Polyakov per iteration, job start to end"}
     est = [j["n_iter"] * 5 * 4 * int(np.prod(j["dims"])) / calib[(j["nc"], j["dims"])] for j in jobs]
     rate = util / float(np.mean(est))
     gaps = rng.exponential(1.0 / rate, n_jobs)
@@ -376,6 +393,7 @@ def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
     return {"value": updates / span, "unit": "link-updates/s", "jobs": n_jobs,
             "busy_fraction": counters.busy_s / span,
             "service_model": {f"su{nc}_{dims[0]}^4": round(r / 1e9, 3) for (nc, dims), r in calib.items()},
+            "steady_su3_32x4": steady,
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "p50_queue_s": float(np.percentile(queue, 50)), "p99_queue_s": float(np.percentile(queue, 99)),
             "span_s": span, "arrival_rate_per_s": rate, "busy_s": counters.busy_s, "steps": int(counters.steps),

@@ -100,6 +100,34 @@ def test_mix_equals_alone_with_queueing(ctx):
         assert srv.stats().link_updates == expect
 
 
+LARGE_JOBS = [  # volume 2^17: groups of large volume (single-lattice update kernels)
+    (3, (16, 16, 16, 32), 6.0, 12, 4, True, 3, 4, 1, 31),
+    (3, (16, 16, 16, 32), 5.8, 12, 4, False, 2, 4, 1, 32),
+    (2, (16, 16, 16, 32), 2.4, 18, 4, True, 2, 2, 1, 33),
+    (3, (16, 16, 32, 16), 6.1, 18, 8, True, 2, 1, 1, 34),
+]
+
+
+@pytest.mark.gpu
+def test_large_volume_groups_equal_alone(ctx):
+    """Jobs of at least 2^17 sites share a group per volume whose updates run through
+    the single-lattice kernels (two SU(3) FP32-12 jobs in one group, SU(2) and FP64
+    groups beside it); each history equals its alone run bit for bit."""
+    budget = sum(job_bytes(j) for j in LARGE_JOBS) + (4 << 20)
+    with sun_amd.Server(ctx, budget, max_jobs=8) as srv:
+        ids = [submit(srv, j) for j in LARGE_JOBS]
+        steps = 0
+        while srv.stats().jobs_done < len(LARGE_JOBS):
+            srv.step()
+            steps += 1
+            assert steps < 50
+        for job, jid in zip(LARGE_JOBS, ids):
+            got = srv.observables(jid, job[6])
+            ref = alone(ctx, job)
+            assert np.array_equal(got, ref), (job, np.abs(got - ref).max())
+            srv.release(jid)
+
+
 @pytest.mark.gpu
 def test_arrivals_while_running(ctx):
     """Jobs submitted while others are mid-run join at the next step and still equal
