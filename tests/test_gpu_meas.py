@@ -216,3 +216,39 @@ def test_static_potential_chain(ctx, oracle, storage, prec, bar):
     assert np.abs(got - ref).max() <= bar, np.abs(got - ref).max()
     # smearing and multihit make the large loops measurable: W(4, 4) well above noise
     assert got[4, 4].real > 1e-3
+
+
+@pytest.mark.parametrize("ss", range(6), ids=["plain", "ape_s", "mhit", "ape_t", "ape_ts", "mhit_ape_s"])
+def test_reference_measurement_variants(ctx, oracle, ss):
+    """test.cpp:170-215: the six variants the reference's test program measures on every
+    configuration -- none, APE in space (w 0.2, 25 steps, 10 hits), multihit (100 hits),
+    APE in time (0.2, 1 step), APE in time then in space, multihit then APE in space
+    (0.4, 50 steps) -- each followed by W(r, T), r <= X/2, T <= T/2
+    (test.cpp:217-221); FP64 SU(3) 8^4 against the oracle's chain"""
+    dims, beta, seed = (8, 8, 8, 8), 6.0, 6
+    rmax, tmax = 4, 4
+    U0 = thermalised(oracle, 3, dims, 18)
+    with sun_amd.Lattice(ctx, 3, dims, beta, 18, 8) as lat, sun_amd.Lattice(ctx, 3, dims, beta, 18, 8) as mh:
+        U = load_config(lat, U0)
+        ref = U.copy()
+        meas = lat
+        if ss == 1:
+            lat.ape_smear(0.2, 25, 10, 1e-10)
+            oracle.ape_smear(ref, dims, 0.2, 25, 10, 1e-10)
+        elif ss in (2, 5):
+            lat.multihit(mh, seed, 100)
+            ref = oracle.multihit(ref, dims, beta, seed, 100)
+            meas = mh
+            if ss == 5:
+                mh.ape_smear(0.4, 50, 10, 1e-10)
+                oracle.ape_smear(ref, dims, 0.4, 50, 10, 1e-10)
+        elif ss in (3, 4):
+            lat.ape_smear(0.2, 1, 10, 1e-10, time_dir=True)
+            oracle.ape_smear(ref, dims, 0.2, 1, 10, 1e-10, time_dir=True)
+            if ss == 4:
+                lat.ape_smear(0.2, 25, 10, 1e-10)
+                oracle.ape_smear(ref, dims, 0.2, 25, 10, 1e-10)
+        got = meas.wilson_loops(rmax, tmax)
+    want = oracle.wilson_loops(ref, dims, rmax, tmax)
+    assert np.abs(got - want).max() <= 1e-10, np.abs(got - want).max()
+    assert np.abs(got[:, 0] - 1).max() < 1e-12  # W(r, 0) = 1: the spatial links stay in SU(3)
