@@ -105,6 +105,7 @@ struct Group {
         uint64_t seed = 0;
         double bnc = 0;
         const uint32_t* sweep_dev = nullptr;
+        double2* partials = nullptr;
         LaunchCfg cfg[2];
     };
     std::vector<Solo> solo;
@@ -227,7 +228,9 @@ int rebuild_tables(Group* g) {
         cp += p.nblocks;
         q.first_block = cl;
         q.nblocks = polyakov_blocks(L->g);
-        q.partials = L->partials;  // reused: the Polyakov launch follows the plaquette finalize
+        // reused: the Polyakov launch follows the plaquette finalize; large-volume groups
+        // measure both in one launch, Polyakov partials after the plaquette's
+        q.partials = g->vkey != 0 ? L->partials + plaquette_blocks(L->g) : L->partials;
         q.scale = 1.0 / ((double)L->d.nc * (double)L->g.X[0] * L->g.X[1] * L->g.X[2]);  // polyakovloop.cu:651
         cl += q.nblocks;
         tu[i] = a;
@@ -244,6 +247,7 @@ int rebuild_tables(Group* g) {
             m.g = a.g;
             m.seed = a.seed;
             m.bnc = a.bnc;
+            m.partials = tp[i].partials;
             m.sweep_dev = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(g->t_update + i) +
                                                             offsetof(BatchEntry, sweep));
             for (int k = 0; k < 2; k++) {
@@ -302,10 +306,18 @@ int group_iteration(Group* g) {
                                                  g->brick[1], g->lds[1], s));
     }
     if (g->reunit) HIP_TRY(g->ops->batch_reunitarize(g->t_reunit, n, g->blocks_reunit, s));
-    HIP_TRY(g->ops->batch_plaquette_partial(g->t_plaq, n, g->blocks_plaq, s));
-    HIP_TRY(launch_batch_finalize(g->t_plaq, n, g->d_plaq, s));
-    HIP_TRY(g->ops->batch_polyakov_partial(g->t_poly, n, g->blocks_poly, s));
-    HIP_TRY(launch_batch_finalize(g->t_poly, n, g->d_poly, s));
+    if (!g->solo.empty()) {
+        // plaquette and Polyakov partials of a member in one launch (the same partial
+        // sums as the two batched kernels)
+        for (const Group::Solo& m : g->solo) HIP_TRY(g->ops->measure_partial(m.f, m.g, m.partials, s));
+        HIP_TRY(launch_batch_finalize(g->t_plaq, n, g->d_plaq, s));
+        HIP_TRY(launch_batch_finalize(g->t_poly, n, g->d_poly, s));
+    } else {
+        HIP_TRY(g->ops->batch_plaquette_partial(g->t_plaq, n, g->blocks_plaq, s));
+        HIP_TRY(launch_batch_finalize(g->t_plaq, n, g->d_plaq, s));
+        HIP_TRY(g->ops->batch_polyakov_partial(g->t_poly, n, g->blocks_poly, s));
+        HIP_TRY(launch_batch_finalize(g->t_poly, n, g->d_poly, s));
+    }
     HIP_TRY(launch_batch_record(g->t_plaq, n, g->d_plaq, g->d_poly, s));
     HIP_TRY(launch_batch_advance(g->t_update, n, s));
     return SUN_OK;
