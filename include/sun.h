@@ -85,6 +85,15 @@ int sun_lattice_destroy(sun_lattice *lat);
 /* _gauge<Real>::Copy / Backup: device-to-device copy of the whole link field; the two
  * lattices must share N, extents, storage and precision   include/gaugearray.h:279, :282-314 */
 int sun_lattice_copy(sun_lattice *dst, const sun_lattice *src, void *stream);
+/* Copy of a field into a lattice of the same N and extents but any storage and
+ * precision (FP64-18 -> FP32-12, ...): each link decoded as the update kernels read it,
+ * widened to FP64 and encoded in dst's storage (12-real: rows 0 and 1; 8-real: the
+ * reference's 8-parameter record) and precision.  Device to device through an FP64
+ * staging buffer allocated for the call; synchronous.  The reference converts
+ * between storage types only through its load/save paths (device_load_save.h:64-322,
+ * io_gauge.cpp); here lockstep precision comparisons start both precisions from one
+ * field without a host round trip. */
+int sun_lattice_convert(sun_lattice *dst, const sun_lattice *src, void *stream);
 /* device bytes of the link field (gaugearray.h:447-464 Bytes()) */
 size_t sun_lattice_bytes(const sun_lattice_desc *desc);
 
@@ -290,6 +299,8 @@ typedef struct {
 #define SUN_JOB_DONE 2
 #define SUN_JOB_CANCELLED 3
 #define SUN_JOB_SUSPENDED 4
+#define SUN_JOB_FAILED 5    /* observables not finite or out of range, or its suspend file
+                               could not be written: sun_server_job_error says which */
 
 typedef struct {
     uint64_t id;
@@ -306,9 +317,15 @@ typedef struct {
 } sun_server_counters;
 
 int sun_server_create(sun_ctx *ctx, size_t budget_bytes, int max_jobs, sun_server **out);
+/* Checks a job description as sun_server_submit does, without a server or a device,
+ * and gives the server memory the job occupies while resident (link field, reduction
+ * scratch, observable history; arena-rounded): for sizing a budget.  Same status
+ * codes as submit, except SUN_ERR_BUDGET. */
+int sun_job_bytes(const sun_job_desc *job, size_t *bytes);
 int sun_server_destroy(sun_server *srv);
 /* SUN_ERR_BUDGET if the job alone exceeds the budget; SUN_ERR_UNSUPPORTED for a cold
- * 8-real start or reunit_every > 1 */
+ * 8-real start, reunit_every > 1, or 12-real links with reunit_every = 0 (they leave
+ * SU(3) geometrically and overflow within about a dozen iterations) */
 int sun_server_submit(sun_server *srv, const sun_job_desc *job, uint64_t *id);
 int sun_server_cancel(sun_server *srv, uint64_t id);
 int sun_server_step(sun_server *srv);                 /* one step on the calling thread  */
@@ -319,8 +336,15 @@ int sun_server_status(sun_server *srv, uint64_t id, sun_job_status *st);
 /* rows {plaq_spatial, plaq_temporal, poly_re, poly_im} of every iteration of a done job */
 int sun_server_observables(sun_server *srv, uint64_t id, double *out, int max_rows);
 int sun_server_release(sun_server *srv, uint64_t id);  /* forget a finished job          */
+/* Why a job is in state SUN_JOB_FAILED (empty otherwise): its latest observables were
+ * not finite or outside [-1, 1] (checked after every group launch and over the whole
+ * history when it completes), or its suspend file could not be written.  The
+ * reference stops the process on such errors (errorCULQCD, comm_mpi.h:116-120); the
+ * server fails the one job and keeps running the others. */
+int sun_server_job_error(sun_server *srv, uint64_t id, char *buf, int len);
 /* Takes a job off the device into a file: a pending job at once, a running one at its
- * next iteration boundary (wait for state SUN_JOB_SUSPENDED).  The file holds the job
+ * next iteration boundary (wait for state SUN_JOB_SUSPENDED; the path must be creatable
+ * when this is called, else SUN_ERR_INVALID and the job runs on).  The file holds the job
  * description, the iterations done, their observable rows and a native checkpoint of
  * the lattice (SUN_FILE_NATIVE layout).  No counterpart in the reference; its
  * checkpoint format is io_gauge.cpp:429-700. */

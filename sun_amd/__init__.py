@@ -39,6 +39,7 @@ _lib = None
 EXPORTS = [
     "sun_init", "sun_finalize", "sun_last_error", "sun_version",
     "sun_lattice_create", "sun_lattice_destroy", "sun_lattice_bytes", "sun_lattice_copy",
+    "sun_lattice_convert",
     "sun_lattice_get_launch", "sun_lattice_set_launch", "sun_lattice_save", "sun_lattice_load",
     "sun_cold_start", "sun_hot_start",
     "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
@@ -54,7 +55,7 @@ EXPORTS = [
     "sun_server_create", "sun_server_destroy", "sun_server_submit", "sun_server_cancel",
     "sun_server_step", "sun_server_start", "sun_server_stop", "sun_server_wait", "sun_server_status",
     "sun_server_observables", "sun_server_release", "sun_server_stats", "sun_server_clock",
-    "sun_server_suspend", "sun_server_resume",
+    "sun_server_suspend", "sun_server_resume", "sun_server_job_error", "sun_job_bytes",
     "sun_stream_create", "sun_stream_destroy", "sun_stream_synchronize",
     "sun_event_create", "sun_event_destroy", "sun_event_record", "sun_event_elapsed_ms",
     "sun_device_name",
@@ -113,7 +114,7 @@ class ServerCounters(ctypes.Structure):
                 ("running", ctypes.c_int), ("pending", ctypes.c_int)]
 
 
-JOB_PENDING, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_SUSPENDED = 0, 1, 2, 3, 4
+JOB_PENDING, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_SUSPENDED, JOB_FAILED = 0, 1, 2, 3, 4, 5
 
 
 def lib() -> ctypes.CDLL:
@@ -136,6 +137,7 @@ def lib() -> ctypes.CDLL:
         "sun_lattice_destroy": (ci, [vp]),
         "sun_lattice_bytes": (ctypes.c_size_t, [ctypes.POINTER(LatticeDesc)]),
         "sun_lattice_copy": (ci, [vp, vp, vp]),
+        "sun_lattice_convert": (ci, [vp, vp, vp]),
         "sun_lattice_get_launch": (ci, [vp, ci, ctypes.POINTER(Launch)]),
         "sun_lattice_set_launch": (ci, [vp, ci, ctypes.POINTER(Launch)]),
         "sun_lattice_save": (ci, [vp, ctypes.c_char_p, ci, ci, u64, u32]),
@@ -189,6 +191,8 @@ def lib() -> ctypes.CDLL:
         "sun_server_clock": (cd, [vp]),
         "sun_server_suspend": (ci, [vp, u64, ctypes.c_char_p]),
         "sun_server_resume": (ci, [vp, ctypes.c_char_p, ctypes.POINTER(u64)]),
+        "sun_server_job_error": (ci, [vp, u64, ctypes.c_char_p, ci]),
+        "sun_job_bytes": (ci, [ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_size_t)]),
         "sun_stream_create": (ci, [ctypes.POINTER(vp)]),
         "sun_stream_destroy": (ci, [vp]),
         "sun_stream_synchronize": (ci, [vp]),
@@ -273,6 +277,10 @@ class Lattice:
     def copy_from(self, other: "Lattice", stream=None):
         """device-to-device copy of other's links (_gauge::Copy)"""
         _check(lib().sun_lattice_copy(self.h, other.h, stream), "sun_lattice_copy")
+
+    def convert_from(self, other: "Lattice", stream=None):
+        """this lattice's field := other's, in this lattice's storage and precision"""
+        _check(lib().sun_lattice_convert(self.h, other.h, stream), "sun_lattice_convert")
 
     @property
     def bytes(self) -> int:
@@ -459,6 +467,17 @@ class Batch:
         return obs[:n_iter]
 
 
+def job_bytes(nc: int, dims, beta: float, n_iter: int, n_or: int = 4, reunit_every: int = 1,
+              storage: int = STORE_18, precision: int = F64, hot_start: bool = False, seed: int = 0) -> int:
+    """server memory a job occupies while resident (sun_job_bytes); raises SunError
+    for a description the server refuses.  Host only: needs no device."""
+    d = JobDesc(nc, (ctypes.c_int * 4)(*dims), beta, storage, precision, seed, int(hot_start), n_iter, n_or,
+                reunit_every)
+    b = ctypes.c_size_t()
+    _check(lib().sun_job_bytes(ctypes.byref(d), ctypes.byref(b)), "sun_job_bytes")
+    return int(b.value)
+
+
 class Server:
     """Continuous-batching job server (sun_server_*): jobs are admitted while their
     memory fits the HBM budget reserved at creation; each scheduling step runs one
@@ -536,6 +555,12 @@ class Server:
         jid = ctypes.c_uint64()
         _check(lib().sun_server_resume(self.h, os.fsencode(path), ctypes.byref(jid)), "sun_server_resume")
         return int(jid.value)
+
+    def job_error(self, jid: int) -> str:
+        """why a job is JOB_FAILED ('' otherwise)"""
+        buf = ctypes.create_string_buffer(512)
+        _check(lib().sun_server_job_error(self.h, jid, buf, len(buf)), "sun_server_job_error")
+        return buf.value.decode()
 
 
 class Timer:

@@ -96,10 +96,12 @@ int sun::check_desc(const sun_lattice_desc* d) {
         if (d->dims[i] < 2 || (d->dims[i] & 1))
             return fail(SUN_ERR_INVALID, "dims[%d]=%d: extents must be even and >= 2", i, d->dims[i]);
         V *= d->dims[i];
+        // a plane holds 4V records of 16 bytes addressed by 32-bit byte offsets; checked
+        // after every factor, so the product (at most 2^26 * 2^31) cannot overflow
+        if (4 * V * 16 > 0xFFFFFFFFll)
+            return fail(SUN_ERR_INVALID, "volume too large at dims[%d]=%d (4V x 16 bytes must fit in 32 bits)", i,
+                        d->dims[i]);
     }
-    // a plane holds 4V records of 16 bytes addressed by 32-bit byte offsets
-    if (4 * V * 16 > 0xFFFFFFFFll)
-        return fail(SUN_ERR_INVALID, "volume %lld too large (4V x 16 bytes must fit in 32 bits)", V);
     if (d->storage != SUN_STORE_18 && d->storage != SUN_STORE_12 && d->storage != SUN_STORE_8)
         return fail(SUN_ERR_INVALID, "storage=%d: use 18, 12 or 8", d->storage);
     if (d->nc != 3 && d->storage != SUN_STORE_18)
@@ -265,6 +267,28 @@ extern "C" int sun_lattice_copy(sun_lattice* dst, const sun_lattice* src, void* 
         return fail(SUN_ERR_INVALID, "sun_lattice_copy: lattices differ in N, extents, storage or precision");
     HIP_TRY(hipMemcpyAsync(dst->buf, src->buf, src->bytes, hipMemcpyDeviceToDevice, pick_stream(dst, stream)));
     return SUN_OK;
+}
+
+extern "C" int sun_lattice_convert(sun_lattice* dst, const sun_lattice* src, void* stream) {
+    if (!dst || !src) return fail(SUN_ERR_INVALID, "sun_lattice_convert: NULL lattice");
+    if (dst->d.nc != src->d.nc || memcmp(dst->d.dims, src->d.dims, sizeof(dst->d.dims)) != 0)
+        return fail(SUN_ERR_INVALID, "sun_lattice_convert: lattices differ in N or extents");
+    if (dst->ctx->device != src->ctx->device)
+        return fail(SUN_ERR_INVALID, "sun_lattice_convert: lattices on different devices");
+    hipStream_t s = pick_stream(dst, stream);
+    // every link through the canonical FP64 form: the source's stored rows decoded
+    // (12/8-real reconstructed) and widened, then encoded in the destination's storage
+    // and precision -- the same two kernels as a host round trip, without the host
+    const size_t n = 4 * (size_t)src->g.V * src->d.nc * src->d.nc * 2;
+    double* tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, n * sizeof(double)));
+    int rc = SUN_OK;
+    hipError_t e = src->ops->to_canonical(src->f, 4 * src->g.V, tmp, s);
+    if (e == hipSuccess) e = dst->ops->from_canonical(dst->f, 4 * dst->g.V, tmp, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = fail(SUN_ERR_HIP, "sun_lattice_convert: %s", hipGetErrorString(e));
+    hipFree(tmp);
+    return rc;
 }
 
 extern "C" int sun_lattice_get_launch(const sun_lattice* lat, int kernel, sun_launch* out) {

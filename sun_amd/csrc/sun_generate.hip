@@ -78,6 +78,10 @@ static int sun_generate_body(sun_lattice* lat, const sun_generate_desc* d, void*
     if (!lat || !d) return fail(SUN_ERR_INVALID, "sun_generate: NULL argument");
     if (d->n_iter < 0 || d->n_or < 0 || d->reunit_every < 0)
         return fail(SUN_ERR_INVALID, "sun_generate: negative count");
+    if (lat->d.storage == SUN_STORE_12 && d->reunit_every == 0)
+        return fail(SUN_ERR_UNSUPPORTED,
+                    "sun_generate: 12-real links need reunit_every >= 1 (row 2 is rebuilt from rows 0 and 1, whose "
+                    "drift from SU(3) grows about 4x per iteration without reunitarisation)");
     if (d->n_iter == 0) return SUN_OK;
     hipStream_t s = pick_stream(lat, stream);
     const bool measure = d->measure != 0;
@@ -116,6 +120,14 @@ static int sun_generate_body(sun_lattice* lat, const sun_generate_desc* d, void*
             obs[4 * it + 2] = h[2 * it + 1].x;
             obs[4 * it + 3] = h[2 * it + 1].y;
         }
+        // a field that has overflowed or left SU(N) (the reference stops the process,
+        // errorCULQCD, comm_mpi.h:116-120): the rows are returned and the call fails
+        for (int it = 0; it < d->n_iter; it++)
+            if (!observables_in_range(h[2 * it], h[2 * it + 1]))
+                return fail(SUN_ERR_STATE,
+                            "sun_generate: iteration %d: plaquette (%g, %g), Polyakov loop (%g, %g) not finite or "
+                            "outside [-1, 1]",
+                            it, h[2 * it].x, h[2 * it].y, h[2 * it + 1].x, h[2 * it + 1].y);
     } else {
         HIP_TRY(hipStreamSynchronize(s));
     }

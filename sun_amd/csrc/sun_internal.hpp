@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -172,4 +173,15 @@ int read_native(FILE* f, const sun_lattice_desc& d, size_t bytes, std::vector<un
                 uint64_t* seed, uint32_t* next_sweep, const char* path);
 bool write_native_bytes(FILE* f, const sun_job_desc& d, const std::vector<unsigned char>& payload, uint64_t seed,
                         uint32_t next_sweep);
+
+// Observables a generation run may report: finite, plaquette components and
+// |Polyakov loop| at most 1 (Re Tr U / N of a unitary U; the slack covers FP32 links
+// between reunitarisations).  A field that has overflowed to NaN or drifted off
+// SU(N) fails this.
+constexpr double kObsSlack = 1e-5;
+inline bool observables_in_range(double2 plaq, double2 poly) {
+    const double lim = 1.0 + kObsSlack;
+    return std::isfinite(plaq.x) && std::isfinite(plaq.y) && std::isfinite(poly.x) && std::isfinite(poly.y) &&
+           std::fabs(plaq.x) <= lim && std::fabs(plaq.y) <= lim && std::hypot(poly.x, poly.y) <= lim;
+}
 }  // namespace sun

@@ -245,9 +245,9 @@ int load_reference(sun_lattice* L, const char* path, bool header, int save_prec)
         int found = 0;
         double beta = 0.0;
         uint64_t confprec = 0;
+        unsigned char b[8] = {0};
         for (int bsize = 4; bsize <= 8 && !found; bsize += 4) {
             fseek(fi.f, 16, SEEK_SET);
-            unsigned char b[8];
             uint64_t cp = 0;
             if (fread(b, 1, bsize, fi.f) != (size_t)bsize || fread(&cp, sizeof(cp), 1, fi.f) != 1) continue;
             if ((cp != 4 && cp != 8) || size != (long)(16 + bsize + 8 + 4 * V * per_link * cp)) continue;
@@ -262,7 +262,15 @@ int load_reference(sun_lattice* L, const char* path, bool header, int save_prec)
             found = bsize;
         }
         if (!found) return fail(SUN_ERR_INVALID, "%s: header/size do not describe a %d-colour lattice", path, L->d.nc);
-        const bool beta_ok = found == 4 ? (float)beta == (float)L->d.beta : beta == L->d.beta;
+        // A 4-byte beta is (float)beta as written here, or -- from the reference's
+        // SaveBin_Gauge with Real = float, which writes sizeof(float) bytes starting at
+        // the double PARAMS::Beta (io_gauge.cpp:457) -- the double's first (low) 4 bytes
+        bool beta_ok = found == 4 ? (float)beta == (float)L->d.beta : beta == L->d.beta;
+        if (found == 4 && !beta_ok) {
+            const double lb = L->d.beta;
+            beta_ok = memcmp(b, &lb, 4) == 0;
+            if (beta_ok) beta = lb;
+        }
         if (!beta_ok) return fail(SUN_ERR_INVALID, "%s: beta %.17g does not match the lattice's %.17g", path, beta,
                                   L->d.beta);
         save_prec = (int)confprec;

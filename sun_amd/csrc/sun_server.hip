@@ -26,6 +26,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <cstdio>
 #include <condition_variable>
 #include <cstddef>
 #include <cstring>
@@ -72,7 +74,13 @@ struct Job {
     // iterations, copied to its memory at admission (kept until it retires)
     std::vector<unsigned char> resume_field;
     std::vector<double2> resume_hist;
+    std::string error;  // why the job failed (state SUN_JOB_FAILED)
 };
+
+bool finished(const Job* J) {
+    return J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED || J->state == SUN_JOB_SUSPENDED ||
+           J->state == SUN_JOB_FAILED;
+}
 
 struct Group {
     const OpsTable* ops = nullptr;
@@ -355,7 +363,10 @@ int launch_group(sun_server* S, Group* g, int max_iters) {
     }
     int k = max_iters;
     for (const Job* J : g->members) k = std::min(k, J->d.n_iter - J->iters_done);
-    k = std::max(k, 1);
+    // a member with no iteration left is retired at the completion that finished it and
+    // resume refuses finished jobs, so k >= 1 here; were it not, launching would write
+    // past that member's observable history
+    if (k < 1) return SUN_OK;
     for (int i = 0; i < k; i++) HIP_TRY(hipGraphLaunch(g->exec, g->stream));
     HIP_TRY(hipEventRecord(g->done, g->stream));
     g->inflight = k;
@@ -472,17 +483,35 @@ int write_job_file(const Job* J, const char* path, const std::vector<double2>& h
     return rc;
 }
 
-// under S->mu; the job's group is idle and the job is one of its members
+void set_job_error(Job* J, int iteration, double2 plaq, double2 poly) {
+    char buf[192];
+    snprintf(buf, sizeof(buf), "iteration %d: plaquette (%g, %g), Polyakov loop (%g, %g) not finite or outside [-1, 1]",
+             iteration, plaq.x, plaq.y, poly.x, poly.y);
+    J->error = buf;
+}
+
+// under S->mu; the job's group is idle and the job is one of its members.  A job file
+// that cannot be written fails the suspended job (its state and message say why)
+// instead of stopping the scheduler.
 int retire(sun_server* S, Job* J, bool done) {
     Group* g = J->group;
-    if (done) {
+    bool failed = !J->error.empty();
+    if (done && !failed) {
         J->hist.resize((size_t)2 * J->d.n_iter);
         HIP_TRY(hipMemcpy(J->hist.data(), J->lat->hist, sizeof(double2) * J->hist.size(), hipMemcpyDeviceToHost));
-    } else if (J->suspend) {
+        for (int it = 0; it < J->d.n_iter && !failed; it++)
+            if (!observables_in_range(J->hist[2 * it], J->hist[2 * it + 1])) {
+                set_job_error(J, it, J->hist[2 * it], J->hist[2 * it + 1]);
+                failed = true;
+            }
+    } else if (J->suspend && !failed) {
         std::vector<double2> rows((size_t)2 * J->iters_done);
         if (!rows.empty())
             HIP_TRY(hipMemcpy(rows.data(), J->lat->hist, sizeof(double2) * rows.size(), hipMemcpyDeviceToHost));
-        SUN_TRY(write_job_file(J, J->suspend_path.c_str(), rows));
+        if (write_job_file(J, J->suspend_path.c_str(), rows) != SUN_OK) {
+            J->error = std::string("suspend: ") + sun_last_error();
+            failed = true;
+        }
     }
     g->members.erase(std::find(g->members.begin(), g->members.end(), J));
     g->dirty = true;
@@ -490,28 +519,38 @@ int retire(sun_server* S, Job* J, bool done) {
     J->lat = nullptr;
     S->arena.release(J->off);
     J->off = SIZE_MAX;
-    J->state = done ? SUN_JOB_DONE : J->suspend ? SUN_JOB_SUSPENDED : SUN_JOB_CANCELLED;
+    J->state = failed ? SUN_JOB_FAILED : done ? SUN_JOB_DONE : J->suspend ? SUN_JOB_SUSPENDED : SUN_JOB_CANCELLED;
+    if (failed) J->hist.clear();
     J->t_end = now_since(S->t0);
     J->resume_field.clear();
     J->resume_field.shrink_to_fit();
     S->running.erase(std::find(S->running.begin(), S->running.end(), J));
     S->admit_needed = true;
-    if (done) S->jobs_done++;
+    if (done && !failed) S->jobs_done++;
     return SUN_OK;
 }
 
-// a group's launched iterations have completed (its event has fired)
+// a group's launched iterations have completed (its event has fired): the members'
+// latest observables are checked, then finished, failed, cancelled and suspended
+// members are retired
 int complete_group(sun_server* S, Group* g) {
     const int k = g->inflight;
     g->inflight = 0;
     g->iter_ms = (now_since(S->t0) - g->t_launch) * 1e3 / k;
+    const int n = (int)g->members.size();
+    std::vector<double2> plaq(n), poly(n);
+    HIP_TRY(hipMemcpyAsync(plaq.data(), g->d_plaq, sizeof(double2) * n, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(poly.data(), g->d_poly, sizeof(double2) * n, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
     std::lock_guard<std::mutex> lk(S->mu);
-    const std::vector<Job*> members = g->members;
-    for (Job* J : members) {
+    const std::vector<Job*> members = g->members;  // the order of the device tables
+    for (int i = 0; i < n; i++) {
+        Job* J = members[i];
         J->iters_done += k;
         S->link_updates += (uint64_t)k * (uint64_t)(1 + J->d.n_or) * 4ull * (uint64_t)J->lat->g.V;
+        if (!observables_in_range(plaq[i], poly[i])) set_job_error(J, J->iters_done - 1, plaq[i], poly[i]);
         const bool done = J->iters_done >= J->d.n_iter;
-        if (done || J->cancel || J->suspend) SUN_TRY(retire(S, J, done));
+        if (done || J->cancel || J->suspend || !J->error.empty()) SUN_TRY(retire(S, J, done));
     }
     S->steps++;
     S->cv.notify_all();
@@ -685,20 +724,34 @@ extern "C" int sun_server_destroy(sun_server* S) {
 }
 
 // checks of a job description (submitted, or read from a job file: with_state when
-// the file carries the job's lattice) and the server memory it needs
-int check_job(const sun_server* S, const sun_job_desc* jd, bool with_state, size_t* bytes) {
+// the file carries the job's lattice) and the server memory it needs; host only
+int check_job_desc(const sun_job_desc* jd, bool with_state, size_t* bytes) {
     sun_lattice_desc d = {jd->nc, {jd->dims[0], jd->dims[1], jd->dims[2], jd->dims[3]}, jd->beta, jd->storage,
                           jd->precision};
     SUN_TRY(check_desc(&d));
     if (jd->n_iter < 1 || jd->n_or < 0) return fail(SUN_ERR_INVALID, "job: n_iter >= 1 and n_or >= 0 required");
     if (jd->reunit_every != 0 && jd->reunit_every != 1)
         return fail(SUN_ERR_UNSUPPORTED, "job: reunit_every must be 0 or 1 in the server");
+    if (jd->storage == SUN_STORE_12 && jd->reunit_every == 0)
+        return fail(SUN_ERR_UNSUPPORTED,
+                    "job: 12-real links need reunit_every = 1 (row 2 is rebuilt from rows 0 and 1, whose drift "
+                    "from SU(3) grows about 4x per iteration without reunitarisation)");
     if (!with_state && !jd->hot_start && jd->storage == SUN_STORE_8)
         return fail(SUN_ERR_UNSUPPORTED, "job: 8-real storage cannot hold a cold start (gaugearray.cu:293-297)");
     *bytes = Arena::round_up(lattice_footprint(&d, jd->n_iter));
+    return SUN_OK;
+}
+
+int check_job(const sun_server* S, const sun_job_desc* jd, bool with_state, size_t* bytes) {
+    SUN_TRY(check_job_desc(jd, with_state, bytes));
     if (*bytes > S->arena.capacity())
         return fail(SUN_ERR_BUDGET, "job needs %zu bytes, the server budget is %zu", *bytes, S->arena.capacity());
     return SUN_OK;
+}
+
+extern "C" int sun_job_bytes(const sun_job_desc* jd, size_t* bytes) {
+    if (!jd || !bytes) return fail(SUN_ERR_INVALID, "sun_job_bytes: NULL argument");
+    return check_job_desc(jd, false, bytes);
 }
 
 extern "C" int sun_server_submit(sun_server* S, const sun_job_desc* jd, uint64_t* id) {
@@ -725,7 +778,7 @@ extern "C" int sun_server_cancel(sun_server* S, uint64_t id) {
     std::lock_guard<std::mutex> lk(S->mu);
     Job* J = find_job(S, id);
     if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    if (J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED || J->state == SUN_JOB_SUSPENDED) return SUN_OK;
+    if (finished(J)) return SUN_OK;
     J->cancel = true;
     if (J->state == SUN_JOB_PENDING) {
         S->pending.erase(std::find(S->pending.begin(), S->pending.end(), J));
@@ -765,13 +818,10 @@ extern "C" int sun_server_wait(sun_server* S, uint64_t id, double timeout_s) {
     std::unique_lock<std::mutex> lk(S->mu);
     Job* J = find_job(S, id);
     if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    auto finished = [&] {
-        return J->state == SUN_JOB_DONE || J->state == SUN_JOB_CANCELLED || J->state == SUN_JOB_SUSPENDED ||
-               S->worker_rc != SUN_OK;
-    };
-    if (!S->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), finished))
+    auto ended = [&] { return finished(J) || S->worker_rc != SUN_OK; };
+    if (!S->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), ended))
         return fail(SUN_ERR_STATE, "job %llu not finished after %.1f s", (unsigned long long)id, timeout_s);
-    if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED && J->state != SUN_JOB_SUSPENDED)
+    if (!finished(J))
         return fail(SUN_ERR_STATE, "the scheduling thread stopped on an error before job %llu finished",
                     (unsigned long long)id);
     return SUN_OK;
@@ -820,8 +870,7 @@ extern "C" int sun_server_release(sun_server* S, uint64_t id) {
     std::lock_guard<std::mutex> lk(S->mu);
     Job* J = find_job(S, id);
     if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
-    if (J->state != SUN_JOB_DONE && J->state != SUN_JOB_CANCELLED && J->state != SUN_JOB_SUSPENDED)
-        return fail(SUN_ERR_STATE, "job %llu is still pending or running", (unsigned long long)id);
+    if (!finished(J)) return fail(SUN_ERR_STATE, "job %llu is still pending or running", (unsigned long long)id);
     S->jobs.erase(id);
     delete J;
     return SUN_OK;
@@ -864,8 +913,22 @@ extern "C" int sun_server_suspend(sun_server* S, uint64_t id, const char* path) 
         S->cv.notify_all();
         return SUN_OK;
     }
+    // the file is written at the group's next completion; a path that cannot be
+    // created is refused now, while the job can still run on
+    FILE* probe = fopen(path, "wb");
+    if (!probe) return fail(SUN_ERR_INVALID, "cannot open %s for writing", path);
+    fclose(probe);
     J->suspend = true;  // retired, with its file written, at the group's next completion
     S->cv.notify_all();
+    return SUN_OK;
+}
+
+extern "C" int sun_server_job_error(sun_server* S, uint64_t id, char* buf, int len) {
+    if (!S || !buf || len < 1) return fail(SUN_ERR_INVALID, "sun_server_job_error: bad argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    const Job* J = find_job(S, id);
+    if (!J) return fail(SUN_ERR_STATE, "unknown job %llu", (unsigned long long)id);
+    snprintf(buf, (size_t)len, "%s", J->error.c_str());
     return SUN_OK;
 }
 
@@ -889,7 +952,9 @@ extern "C" int sun_server_resume(sun_server* S, const char* path, uint64_t* id) 
          get(f, d.n_iter) && get(f, d.n_or) && get(f, d.reunit_every) && get(f, iters_done);
     if (!ok) return fail(SUN_ERR_INVALID, "%s: not a job file", path);
     sun_lattice_desc ld = {d.nc, {d.dims[0], d.dims[1], d.dims[2], d.dims[3]}, d.beta, d.storage, d.precision};
-    if (iters_done < 0 || iters_done > d.n_iter || (iters_done > 0 && !has_lattice) || has_lattice > 1)
+    // a job file never holds a finished job (a job that completes is retired as done,
+    // not suspended), so iters_done < n_iter
+    if (iters_done < 0 || iters_done >= d.n_iter || (iters_done > 0 && !has_lattice) || has_lattice > 1)
         return fail(SUN_ERR_INVALID, "%s: inconsistent job state", path);
     // every size below follows from a description that fits the budget
     size_t bytes = 0;

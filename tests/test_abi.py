@@ -62,3 +62,36 @@ def test_descriptor_limits():
     for bad in [(3, (4, 4, 4, 5), 18, 8), (3, (0, 4, 4, 4), 18, 8), (5, (4, 4, 4, 4), 18, 8),
                 (2, (4, 4, 4, 4), 12, 4), (3, (4, 4, 4, 4), 10, 4), (3, (4, 4, 4, 4), 18, 2)]:
         assert sun_amd.lattice_bytes(*bad) == 0, bad
+    # extents whose product overflows 64 bits are refused before the product wraps
+    # (65536^4 = 2^64 would wrap to 0 and pass a check made after the loop)
+    assert sun_amd.lattice_bytes(3, (65536, 65536, 65536, 65536), 18, 8) == 0
+    assert sun_amd.lattice_bytes(3, (2 ** 30, 2 ** 30, 2, 2), 18, 8) == 0
+
+
+def test_job_validation_on_host():
+    """sun_job_bytes applies the server's job checks without a device: the memory a
+    job occupies, and the refusals (12-real links without reunitarisation every
+    iteration, cold 8-real starts, reunit_every > 1, bad extents and counts)"""
+    import sun_amd
+    b = sun_amd.job_bytes(3, (8, 8, 8, 8), 6.0, 50, storage=12, precision=4)
+    assert b >= sun_amd.lattice_bytes(3, (8, 8, 8, 8), 12, 4) and b % 256 == 0
+    # the observable history grows with the iterations
+    assert sun_amd.job_bytes(3, (8, 8, 8, 8), 6.0, 500, storage=12, precision=4) > b
+    assert sun_amd.job_bytes(3, (8, 8, 8, 8), 6.0, 50, storage=18, precision=4, reunit_every=0) > 0
+    assert sun_amd.job_bytes(3, (8, 8, 8, 8), 6.0, 50, storage=8, precision=4, hot_start=True, reunit_every=0) > 0
+    cases = [
+        (dict(storage=12, precision=4, reunit_every=0), sun_amd.ERR_UNSUPPORTED),
+        (dict(storage=12, precision=8, reunit_every=0), sun_amd.ERR_UNSUPPORTED),
+        (dict(storage=8, precision=4, hot_start=False), sun_amd.ERR_UNSUPPORTED),
+        (dict(reunit_every=2), sun_amd.ERR_UNSUPPORTED),
+        (dict(n_iter=0), sun_amd.ERR_INVALID),
+        (dict(n_or=-1), sun_amd.ERR_INVALID),
+        (dict(dims=(8, 8, 8, 7)), sun_amd.ERR_INVALID),
+        (dict(nc=5), sun_amd.ERR_UNSUPPORTED),
+    ]
+    for kw, code in cases:
+        args = dict(nc=3, dims=(8, 8, 8, 8), beta=6.0, n_iter=10)
+        args.update(kw)
+        with pytest.raises(sun_amd.SunError) as e:
+            sun_amd.job_bytes(**args)
+        assert e.value.code == code, (kw, e.value.code, str(e.value))

@@ -179,6 +179,9 @@ def test_budget_rejection_and_bad_requests(ctx):
         with pytest.raises(sun_amd.SunError) as e:
             srv.submit(3, (4, 4, 4, 4), 6.0, 1, 2, reunit_every=2)
         assert e.value.code == sun_amd.ERR_UNSUPPORTED
+        with pytest.raises(sun_amd.SunError) as e:  # 12-real links need reunitarisation
+            srv.submit(3, (4, 4, 4, 4), 6.0, 1, 20, reunit_every=0, storage=12, precision=4)
+        assert e.value.code == sun_amd.ERR_UNSUPPORTED
         with pytest.raises(sun_amd.SunError) as e:  # odd extent
             srv.submit(3, (4, 4, 4, 5), 6.0, 1, 2)
         assert e.value.code == sun_amd.ERR_INVALID
@@ -338,6 +341,9 @@ def test_resume_rejects_bad_job_files(ctx, tmp_path):
         (dict(has_lattice=1, iters_done=3, rows=3), sun_amd.ERR_INVALID),  # no checkpoint after them
         (dict(has_lattice=1, n_iter=2 ** 31 - 1, iters_done=2 ** 30), sun_amd.ERR_BUDGET),
         (dict(storage=8, hot=0), sun_amd.ERR_UNSUPPORTED),
+        (dict(storage=12, reunit=0), sun_amd.ERR_UNSUPPORTED),
+        # a finished job (no iteration left): launching it would write past its history
+        (dict(has_lattice=1, n_iter=3, iters_done=3, rows=3), sun_amd.ERR_INVALID),
     ]
     with sun_amd.Server(ctx, 1 << 30) as srv:
         for fields, code in cases:
@@ -348,3 +354,80 @@ def test_resume_rejects_bad_job_files(ctx, tmp_path):
         jid = srv.resume(job_file(p, n_iter=2, seed=9))
         run_until(srv, jid, sun_amd.JOB_DONE)
         assert srv.status(jid).iters_done == 2
+
+
+@pytest.mark.gpu
+def test_suspend_to_unwritable_path_keeps_jobs_running(ctx, tmp_path):
+    """A suspend request whose file cannot be created is refused at the call; the
+    job and its neighbours run on and finish with their alone results."""
+    a, b = JOBS[7], JOBS[0]
+    with sun_amd.Server(ctx, 1 << 30) as srv:
+        ja, jb = submit(srv, a), submit(srv, b)
+        srv.step()
+        with pytest.raises(sun_amd.SunError) as e:
+            srv.suspend(ja, str(tmp_path / "no" / "such" / "dir" / "job.bin"))
+        assert e.value.code == sun_amd.ERR_INVALID
+        run_until(srv, ja, sun_amd.JOB_DONE)
+        run_until(srv, jb, sun_amd.JOB_DONE)
+        assert srv.job_error(ja) == ""
+        assert np.array_equal(srv.observables(ja, a[6]), alone(ctx, a))
+        assert np.array_equal(srv.observables(jb, b[6]), alone(ctx, b))
+
+
+def resumable_job_file(ctx, path, U, job, tmp_path):
+    """a job file (no iterations done) whose lattice is the host field U"""
+    nc, dims, beta, storage, prec, hot, n_iter, n_or, reunit, seed = job
+    ck = str(tmp_path / "field.ckpt")
+    with sun_amd.Lattice(ctx, nc, dims, beta, storage, prec) as lat:
+        lat.from_host(U)
+        lat.save(ck, sun_amd.FILE_NATIVE, seed=seed, next_sweep=0)
+    job_file(path, has_lattice=1, nc=nc, dims=dims, beta=beta, storage=storage, prec=prec, seed=seed,
+             hot=int(hot), n_iter=n_iter, n_or=n_or, reunit=reunit)
+    with open(path, "ab") as f:
+        f.write(open(ck, "rb").read())
+    return path
+
+
+@pytest.mark.gpu
+def test_observable_guard_fails_only_the_bad_job(ctx, tmp_path):
+    """A job whose field is not finite (set from the host, resumed from a job file)
+    or far from SU(N) (every link doubled: plaquette 16) ends in JOB_FAILED with the
+    iteration and values in its message, while a job beside it in the same group
+    finishes with its alone result; sun_generate reports such a field as an error."""
+    good = (3, (8, 8, 8, 8), 6.0, 18, 4, True, 6, 4, 0, 31)  # the same group as the bad jobs
+    nc, dims = good[0], good[1]
+    with sun_amd.Lattice(ctx, nc, dims, 6.0, 18, 4) as lat:
+        lat.hot_start(5)
+        U = lat.to_host()
+    nan_field = U.copy()
+    nan_field[2, 17] = np.nan
+    big_field = np.zeros_like(U)  # twice the unit matrix: every plaquette is 16
+    big_field[..., np.arange(nc), np.arange(nc)] = 2.0
+    bad_nan = (3, dims, 6.0, 18, 4, True, 6, 4, 0, 41)
+    bad_big = (3, dims, 6.0, 18, 4, True, 6, 4, 0, 42)
+    p1 = resumable_job_file(ctx, str(tmp_path / "nan.bin"), nan_field, bad_nan, tmp_path)
+    p2 = resumable_job_file(ctx, str(tmp_path / "big.bin"), big_field, bad_big, tmp_path)
+    with sun_amd.Server(ctx, 1 << 30) as srv:
+        jg = submit(srv, good)
+        j1, j2 = srv.resume(p1), srv.resume(p2)
+        run_until(srv, jg, sun_amd.JOB_DONE)
+        for j in (j1, j2):
+            run_until(srv, j, sun_amd.JOB_FAILED)
+            assert srv.status(j).iters_done >= 1
+            with pytest.raises(sun_amd.SunError):
+                srv.observables(j, 1)
+        assert "not finite or outside [-1, 1]" in srv.job_error(j1) and "nan" in srv.job_error(j1)
+        assert "iteration 0" in srv.job_error(j2), srv.job_error(j2)
+        assert srv.job_error(jg) == ""
+        assert np.array_equal(srv.observables(jg, good[6]), alone(ctx, good))
+        assert srv.stats().bytes_in_use == 0 and srv.stats().jobs_done == 1
+    with sun_amd.Lattice(ctx, nc, dims, 6.0, 18, 4) as lat:
+        lat.from_host(nan_field)
+        with pytest.raises(sun_amd.SunError) as e:
+            lat.generate(seed=1, n_iter=2, n_or=1, reunit_every=1)
+        assert e.value.code == sun_amd.ERR_STATE and "iteration 0" in str(e.value)
+    with sun_amd.Lattice(ctx, 3, dims, 6.0, 12, 4) as lat:
+        lat.cold_start()
+        with pytest.raises(sun_amd.SunError) as e:
+            lat.generate(seed=1, n_iter=2, n_or=1, reunit_every=0)
+        assert e.value.code == sun_amd.ERR_UNSUPPORTED
