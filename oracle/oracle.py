@@ -61,6 +61,7 @@ def lib() -> ctypes.CDLL:
         "or_heatbath_sub": (ci, [ci, ip, cd, u64, u32, ci, dp, ci, ci]),
         "or_overrelax_sub": (ci, [ci, ip, ci, dp, ci, ci]),
         "or_heatbath_sweep": (ci, [ci, ip, cd, u64, u32, ci, dp]),
+        "or_heatbath_sweep_margins": (ci, [ci, ip, cd, u64, u32, ci, dp, dp]),
         "or_overrelax_sweep": (ci, [ci, ip, ci, dp]),
         "or_overrelax_link": (None, [ci, dp, dp]),
         "or_heatbath_link": (None, [ci, cd, dp, dp, u64, u32, u32, u32]),
@@ -73,6 +74,8 @@ def lib() -> ctypes.CDLL:
         "or_su3_project": (None, [dp, ci]),
         "or_ape_smear": (ci, [ci, ip, dp, cd, ci, ci, cd, ci]),
         "or_multihit": (ci, [ci, ip, cd, u64, ci, dp, dp]),
+        "or_multihit_margins": (ci, [ci, ip, cd, u64, ci, dp, dp, dp]),
+        "or_multihit_ext_margins": (ci, [ci, ip, cd, u64, ci, dp, dp, dp]),
         "or_exp_iq": (None, [dp, dp]),
         "or_stout_smear": (ci, [ip, dp, cd, ci, ci]),
         "or_multihit_space": (ci, [ci, ip, ci, dp, dp]),
@@ -235,6 +238,16 @@ def heatbath_sweep(U, dims, beta, seed, sweep, storage=STORE_18):
     return U
 
 
+def heatbath_sweep_margins(U, dims, beta, seed, sweep, storage=STORE_18):
+    """heatbath sweep of U in place; returns, per link (shape (4, V)), the smallest
+    distance from a tie among the sampler decisions of its update (inf if none)"""
+    nc = U.shape[-1]
+    m = np.empty(U.shape[:2], dtype=np.float64)
+    _check(lib().or_heatbath_sweep_margins(nc, _dims(dims), beta, seed, sweep, storage, _dp(U), _dp(m)),
+           "heatbath")
+    return m
+
+
 def overrelax_sweep(U, dims, storage=STORE_18):
     nc = U.shape[-1]
     _check(lib().or_overrelax_sweep(nc, _dims(dims), storage, _dp(U)), "overrelax")
@@ -338,6 +351,25 @@ def multihit_ext(U, dims, beta: float, seed: int, nhit: int):
     out = np.empty_like(U)
     _check(lib().or_multihit_ext(U.shape[-1], _dims(dims), beta, seed, nhit, _dp(U), _dp(out)), "multihit_ext")
     return out
+
+
+def multihit_margins(U, dims, beta: float, seed: int, nhit: int):
+    """(multihit result, per-link smallest distance from a sampler tie; temporal links)"""
+    out = np.empty_like(U)
+    m = np.full(U.shape[:2], np.inf)
+    _check(lib().or_multihit_margins(U.shape[-1], _dims(dims), beta, seed, nhit, _dp(U), _dp(out), _dp(m)),
+           "multihit")
+    return out, m
+
+
+def multihit_ext_margins(U, dims, beta: float, seed: int, nhit: int):
+    """(extended multihit result, per temporal link the smallest distance from a
+    sampler tie over every update of its box)"""
+    out = np.empty_like(U)
+    m = np.full(U.shape[:2], np.inf)
+    _check(lib().or_multihit_ext_margins(U.shape[-1], _dims(dims), beta, seed, nhit, _dp(U), _dp(out), _dp(m)),
+           "multihit_ext")
+    return out, m
 
 
 def multihit_space(U, dims, nhit: int):

@@ -5,11 +5,15 @@
  *
  * Arithmetic follows the reference's operation order (complex.h:150-152,
  * matrixsun.h:298-347) and is built with -ffp-contract=off, so every multiply-add
- * is rounded the way the reference's source writes it.  Transcendentals come from
- * the host libm (the reference uses CUDA libm; that boundary is unpinned and
- * tolerances of the parity tests absorb it).
+ * is rounded the way the reference's source writes it.  The sampler's and the Haar
+ * start's log, exp and sin/cos of 2 pi u are sun_libm.h's (the engine's own portable
+ * implementations, compiled here as on the device and pinned against glibc by the
+ * CPU tests), so FP64 chains agree bit for bit with the device; other
+ * transcendentals come from the host libm.  The reference uses CUDA libm, a
+ * boundary no reference test pins.
  */
 #include "sun_oracle.h"
+#include "sun_libm.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -19,7 +23,6 @@
 #endif
 
 #define NMAX 4
-#define PII 6.2831853071795864769252867665590   /* 2*pi, include/constants.h:27 */
 
 #define RNG_TAG_HB   0x10000000u
 #define RNG_TAG_HOT  0x20000000u
@@ -216,41 +219,51 @@ static inline void q_mul_uv_dagger(const double v[4], const double u[4], double 
 }
 
 /* device_PHB_OVR.h:162-223 (MILC Kennedy-Pendleton / Creutz generator) */
-static void gen_su2_milc(double al, usrc *st, double a[4]) {
+/* margin (may be NULL): lowered to the smallest |distance from a tie| among the
+ * decisions this call takes -- the branch test al > 2, each Kennedy-Pendleton test
+ * (1 - d/2) - xr4^2 and each Creutz test (1 - a0^2) - xr2^2.  Arithmetic of another
+ * precision can only decide differently where that distance is at its rounding
+ * level; the FP32 parity tests use it to attribute links that differ by O(1). */
+#define TIE(m) do { if (margin) { double t_ = fabs(m); if (t_ < *margin) *margin = t_; } } while (0)
+static void gen_su2_milc(double al, usrc *st, double a[4], double *margin) {
     double xr1, xr2, xr3, xr4, d, r;
     int k;
     double a0 = 0.0;
     xr1 = unext(st);
-    xr1 = log(xr1 + 1.e-10);
+    xr1 = sun_log(xr1 + 1.e-10);
     xr2 = unext(st);
-    xr2 = log(xr2 + 1.e-10);
+    xr2 = sun_log(xr2 + 1.e-10);
     xr3 = unext(st);
     xr4 = unext(st);
-    xr3 = cos(PII * xr3);
+    xr3 = sun_cos2pi(xr3);
     d = -(xr2 + xr1 * xr3 * xr3) / al;
     int nacd = 0;
+    TIE(al - 2.0);
+    TIE((1.00 - 0.5 * d) - xr4 * xr4);
     if ((1.00 - 0.5 * d) > xr4 * xr4) nacd = 1;
     if (nacd == 0 && al > 2.0) {
         for (k = 0; k < 20; k++) {
             xr1 = unext(st);
-            xr1 = log(xr1 + 1.e-10);
+            xr1 = sun_log(xr1 + 1.e-10);
             xr2 = unext(st);
-            xr2 = log(xr2 + 1.e-10);
+            xr2 = sun_log(xr2 + 1.e-10);
             xr3 = unext(st);
             xr4 = unext(st);
-            xr3 = cos(PII * xr3);
+            xr3 = sun_cos2pi(xr3);
             d = -(xr2 + xr1 * xr3 * xr3) / al;
+            TIE((1.00 - 0.5 * d) - xr4 * xr4);
             if ((1.00 - 0.5 * d) > xr4 * xr4) break;
         }
     }
     if (nacd == 0 && al <= 2.0) {
-        xr3 = exp(-2.0 * al);
+        xr3 = sun_exp(-2.0 * al);
         xr4 = 1.0 - xr3;
         for (k = 0; k < 20; k++) {
             xr1 = unext(st);
             xr2 = unext(st);
             r = xr3 + xr4 * xr1;
-            a0 = 1.00 + log(r) / al;
+            a0 = 1.00 + sun_log(r) / al;
+            TIE((1.0 - a0 * a0) - xr2 * xr2);
             if ((1.0 - a0 * a0) > xr2 * xr2) break;
         }
         d = 1.0 - a0;
@@ -263,16 +276,18 @@ static void gen_su2_milc(double al, usrc *st, double a[4]) {
     xr1 = xr3 - a[3] * a[3];
     xr1 = fabs(xr1);
     xr1 = sqrt(xr1);
-    xr2 = PII * unext(st);
-    a[1] = xr1 * cos(xr2);
-    a[2] = xr1 * sin(xr2);
+    /* sin/cos of PII * u (device_PHB_OVR.h:217-221) with the angle in revolutions */
+    double sn, cs;
+    sun_sincos2pi(unext(st), &sn, &cs);
+    a[1] = xr1 * cs;
+    a[2] = xr1 * sn;
 }
 
 int or_generate_su2_milc(double al, const double *u, int nu, double a[4]) {
     usrc s;
     memset(&s, 0, sizeof(s));
     s.list = u; s.nlist = nu; s.pos = 0;
-    gen_su2_milc(al, &s, a);
+    gen_su2_milc(al, &s, a, NULL);
     return s.pos;
 }
 
@@ -282,7 +297,7 @@ void or_sample_a0(double al, uint64_t seed, int n, double *a0_out) {
         usrc s;
         double a[4];
         usrc_stream(&s, seed, (uint32_t)i, 0u, RNG_TAG_TEST);
-        gen_su2_milc(al, &s, a);
+        gen_su2_milc(al, &s, a, NULL);
         a0_out[i] = a[0];
     }
 }
@@ -341,12 +356,15 @@ static void reconstruct8(mat *m, cx theta) {
     p2 = 1.0 / p1;
     diff = 1.0 - p1;
     p1 = sqrt(diff >= 0 ? diff : 0.0);
-    m->e[0][0].re = p1 * cos(theta.re);
-    m->e[0][0].im = p1 * sin(theta.re);
+    double s0, c0, s1, c1;
+    sun_sincos(theta.re, &s0, &c0);
+    sun_sincos(theta.im, &s1, &c1);
+    m->e[0][0].re = p1 * c0;
+    m->e[0][0].im = p1 * s0;
     diff = 1.0 - cabs2(m->e[0][0]) - cabs2(m->e[1][0]);
     p1 = sqrt(diff >= 0 ? diff : 0.0);
-    m->e[2][0].re = p1 * cos(theta.im);
-    m->e[2][0].im = p1 * sin(theta.im);
+    m->e[2][0].re = p1 * c1;
+    m->e[2][0].im = p1 * s1;
     m->e[1][1] = cconj(cmul(m->e[2][0], m->e[0][2]));
     m->e[2][2] = cmul(cconj(m->e[0][0]), m->e[1][0]);
     m->e[1][1] = cadd(m->e[1][1], cmul(m->e[2][2], m->e[0][1]));
@@ -381,8 +399,8 @@ void or_encode8(const double *link, double *p8) {
     p8[0] = m.e[0][1].re; p8[1] = m.e[0][1].im;
     p8[2] = m.e[0][2].re; p8[3] = m.e[0][2].im;
     p8[4] = m.e[1][0].re; p8[5] = m.e[1][0].im;
-    p8[6] = atan2(m.e[0][0].im, m.e[0][0].re);
-    p8[7] = atan2(m.e[2][0].im, m.e[2][0].re);
+    p8[6] = sun_atan2(m.e[0][0].im, m.e[0][0].re);
+    p8[7] = sun_atan2(m.e[2][0].im, m.e[2][0].re);
 }
 void or_decode8(const double *p8, double *link) {
     mat m;
@@ -457,8 +475,9 @@ static void reunit_mat(int nc, mat *U) {
         for (int c = 0; c < nc; ++c) U->e[c][j] = cscale(U->e[c][j], t1);
     }
     t2 = mdet(nc, U);
-    double ai = atan2(t2.im, t2.re);
-    t2 = cmk(cos(ai), -sin(ai));
+    double ai = sun_atan2(t2.im, t2.re), sn, cs;
+    sun_sincos(ai, &sn, &cs);
+    t2 = cmk(cs, -sn);
     for (int c = 0; c < nc; ++c) U->e[c][nc - 1] = cmul(U->e[c][nc - 1], t2);
 }
 
@@ -507,9 +526,10 @@ void or_hot_start(int nc, const int X[4], uint64_t seed, int storage, double *U)
         for (int i = 0; i < nc; i++)
             for (int j = 0; j < nc; j++) {
                 double u1 = unext(&st), u2 = unext(&st);
-                double rr = sqrt(-2.0 * log(u1));
-                double ph = PII * u2;
-                m.e[i][j] = cmk(rr * cos(ph), rr * sin(ph));
+                double rr = sqrt(-2.0 * sun_log(u1));
+                double sn, cs;
+                sun_sincos2pi(u2, &sn, &cs);
+                m.e[i][j] = cmk(rr * cs, rr * sn);
             }
         reunit_mat(nc, &m);
         double *dst = link_at_w(nc, U, V, mu, s);
@@ -619,7 +639,7 @@ static void block_proj_uf(const mat *U, const mat *F, int p, int q, double r[4])
 }
 
 /* device_PHB_OVR.h:361-485 */
-static void heatbath_mat(int nc, double beta_over_nc, mat *U, const mat *F, usrc *st) {
+static void heatbath_mat(int nc, double beta_over_nc, mat *U, const mat *F, usrc *st, double *margin) {
     if (nc == 3) {
         for (int block = 0; block < 3; block++) {
             int p, q;
@@ -630,7 +650,7 @@ static void heatbath_mat(int nc, double beta_over_nc, mat *U, const mat *F, usrc
             double ap = beta_over_nc * k;
             k = 1.0 / k;
             for (int i = 0; i < 4; i++) r[i] *= k;
-            gen_su2_milc(ap, st, a);
+            gen_su2_milc(ap, st, a, margin);
             q_mul_uv_dagger(a, r, rr);
             cx a0 = cmk(rr[0], rr[3]), a1 = cmk(rr[2], rr[1]);
             cx a2 = cmk(-rr[2], rr[1]), a3 = cmk(rr[0], -rr[3]);
@@ -652,7 +672,7 @@ static void heatbath_mat(int nc, double beta_over_nc, mat *U, const mat *F, usrc
         double ap = beta_over_nc * k;
         k = 1.0 / k;
         for (int i = 0; i < 4; i++) r[i] *= k;
-        gen_su2_milc(ap, st, a);
+        gen_su2_milc(ap, st, a, margin);
         q_mul_uv_dagger(a, r, rr);
         mul_block_sun(nc, rr, U, x, y);
         mul_block_sun(nc, rr, &M, x, y);
@@ -708,7 +728,7 @@ void or_heatbath_link(int nc, double beta, double *U, const double *F,
     usrc_stream(&st, seed, c0, c2, c3);
     mload(nc, U, &u);
     mload(nc, F, &f);
-    heatbath_mat(nc, beta / (double)nc, &u, &f, &st);
+    heatbath_mat(nc, beta / (double)nc, &u, &f, &st, NULL);
     mstore(nc, &u, U);
 }
 
@@ -723,8 +743,8 @@ static int check_args(int nc, const int X[4], int storage) {
 
 /* monte.cu:44-88: one (parity, mu) heatbath launch.  RNG stream of the site:
  * key = seed, counter = (id, draw, sweep, TAG_HB | parity<<4 | mu). */
-int or_heatbath_sub(int nc, const int X[4], double beta, uint64_t seed, uint32_t sweep,
-                    int storage, double *U, int parity, int mu) {
+static int heatbath_sub(int nc, const int X[4], double beta, uint64_t seed, uint32_t sweep,
+                        int storage, double *U, int parity, int mu, double *margins) {
     int rc = check_args(nc, X, storage);
     if (rc) return rc;
     long V = volume_of(X);
@@ -739,11 +759,18 @@ int or_heatbath_sub(int nc, const int X[4], double beta, uint64_t seed, uint32_t
         double *l = link_at_w(nc, U, V, mu, id + parity * half);
         mat u;
         mload(nc, l, &u);
-        heatbath_mat(nc, bnc, &u, &F, &st);
+        double *m = margins ? &margins[(long)mu * V + id + parity * half] : NULL;
+        if (m) *m = INFINITY;
+        heatbath_mat(nc, bnc, &u, &F, &st, m);
         mstore(nc, &u, l);
         project_link(nc, storage, l);
     }
     return 0;
+}
+
+int or_heatbath_sub(int nc, const int X[4], double beta, uint64_t seed, uint32_t sweep,
+                    int storage, double *U, int parity, int mu) {
+    return heatbath_sub(nc, X, beta, seed, sweep, storage, U, parity, mu, NULL);
 }
 
 /* ovr.cu:39-80 */
@@ -771,6 +798,16 @@ int or_heatbath_sweep(int nc, const int X[4], double beta, uint64_t seed, uint32
     for (int parity = 0; parity < 2; parity++)
         for (int mu = 0; mu < 4; mu++) {
             int rc = or_heatbath_sub(nc, X, beta, seed, sweep, storage, U, parity, mu);
+            if (rc) return rc;
+        }
+    return 0;
+}
+
+int or_heatbath_sweep_margins(int nc, const int X[4], double beta, uint64_t seed, uint32_t sweep,
+                              int storage, double *U, double *margins) {
+    for (int parity = 0; parity < 2; parity++)
+        for (int mu = 0; mu < 4; mu++) {
+            int rc = heatbath_sub(nc, X, beta, seed, sweep, storage, U, parity, mu, margins);
             if (rc) return rc;
         }
     return 0;
@@ -1291,7 +1328,8 @@ int or_hyp_smear(int nc, const int X[4], double *U, double a1, double a2, double
  * itself and nhit successive heatbath updates in its (fixed) staple, all from the
  * input field; spatial links copied.  Hit k of the link (parity, id) draws from the
  * stream (seed, id, k, TAG_MH | parity<<4 | 3). */
-int or_multihit(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout) {
+static int multihit(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout,
+                    double *margins) {
     int rc = check_args(nc, X, OR_STORE_18);
     if (rc) return rc;
     long V = volume_of(X), half = V / 2;
@@ -1307,10 +1345,12 @@ int or_multihit(int nc, const int X[4], double beta, uint64_t seed, int nhit, co
         mat u, acc;
         mload(nc, link_at(nc, Uin, V, 3, idd), &u);
         acc = u;
+        double *m = margins ? &margins[3 * V + idd] : NULL;
+        if (m) *m = INFINITY;
         for (int hit = 0; hit < nhit; hit++) {
             usrc st;
             usrc_stream(&st, seed, (uint32_t)id, (uint32_t)hit, RNG_TAG_MH | ((uint32_t)parity << 4) | 3u);
-            heatbath_mat(nc, bnc, &u, &F, &st);
+            heatbath_mat(nc, bnc, &u, &F, &st, m);
             madd_to(nc, &acc, &u);
         }
         for (int i = 0; i < nc; i++)
@@ -1319,6 +1359,15 @@ int or_multihit(int nc, const int X[4], double beta, uint64_t seed, int nhit, co
         mstore(nc, &acc, link_at_w(nc, Uout, V, 3, idd));
     }
     return 0;
+}
+
+int or_multihit(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout) {
+    return multihit(nc, X, beta, seed, nhit, Uin, Uout, NULL);
+}
+
+int or_multihit_margins(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin,
+                        double *Uout, double *margins) {
+    return multihit(nc, X, beta, seed, nhit, Uin, Uout, margins);
 }
 
 /* multihitsp.cu:115-161: spatial links -> mean of nhit successive overrelaxation
@@ -1366,7 +1415,8 @@ static int mhx_in_box(const int o[3]) {
 }
 static int mhx_index(const int o[3]) { return (o[0] + 1) + 3 * (o[1] + 1) + 9 * (o[2] + 1); }
 
-int or_multihit_ext(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout) {
+static int multihit_ext(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin,
+                        double *Uout, double *margins) {
     int rc = check_args(nc, X, OR_STORE_18);
     if (rc) return rc;
     if (nhit < 1) return -5;
@@ -1386,6 +1436,8 @@ int or_multihit_ext(int nc, const int X[4], double beta, uint64_t seed, int nhit
         }
         mat acc;
         mzero(nc, &acc);
+        double *m = margins ? &margins[3 * V + idd] : NULL;
+        if (m) *m = INFINITY;
         for (int hit = 0; hit < nhit; hit++) {
             for (int par = 0; par < 2; par++)
                 for (int b = 0; b < 27; b++) {
@@ -1431,7 +1483,7 @@ int or_multihit_ext(int nc, const int X[4], double beta, uint64_t seed, int nhit
                     mat F = mdag(nc, &staple);
                     usrc st;
                     usrc_stream(&st, seed, (uint32_t)idd, (uint32_t)hit, RNG_TAG_MHX | (uint32_t)b);
-                    heatbath_mat(nc, bnc, &box[b], &F, &st);
+                    heatbath_mat(nc, bnc, &box[b], &F, &st, m);
                 }
             madd_to(nc, &acc, &box[13]);
         }
@@ -1441,6 +1493,15 @@ int or_multihit_ext(int nc, const int X[4], double beta, uint64_t seed, int nhit
         mstore(nc, &acc, link_at_w(nc, Uout, V, 3, idd));
     }
     return 0;
+}
+
+int or_multihit_ext(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout) {
+    return multihit_ext(nc, X, beta, seed, nhit, Uin, Uout, NULL);
+}
+
+int or_multihit_ext_margins(int nc, const int X[4], double beta, uint64_t seed, int nhit, const double *Uin,
+                            double *Uout, double *margins) {
+    return multihit_ext(nc, X, beta, seed, nhit, Uin, Uout, margins);
 }
 
 /* wilsonloop.cu:44-121,494-516 (WilsonLineSP + WilsonLoopSP): W(r, T) = sum over x and

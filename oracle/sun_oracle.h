@@ -69,6 +69,11 @@ int or_overrelax_sub(int nc, const int dims[4], int storage, double *U, int pari
 int or_heatbath_sweep(int nc, const int dims[4], double beta, uint64_t seed, uint32_t sweep,
                       int storage, double *U);
 int or_overrelax_sweep(int nc, const int dims[4], int storage, double *U);
+/* the heatbath sweep, also writing margins[4V]: for every updated link the smallest
+ * distance from a tie among its sampler's decisions (branch al > 2, accept tests);
+ * +inf for links the sweep does not update.  For attributing FP32 accept/reject flips */
+int or_heatbath_sweep_margins(int nc, const int dims[4], double beta, uint64_t seed, uint32_t sweep,
+                              int storage, double *U, double *margins);
 
 /* ---- single-link updates (exposed for closed-form tests) --------------------- */
 void or_overrelax_link(int nc, double *U, const double *F);
@@ -113,6 +118,10 @@ int or_hyp_smear(int nc, const int dims[4], double *U, double a1, double a2, dou
 /* multihit.cu:112-147: Uout = Uin with each temporal link replaced by the mean of
  * itself and nhit heatbath updates in its staple (stream (seed, id, hit, TAG_MH|p<<4|3)) */
 int or_multihit(int nc, const int dims[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout);
+/* the same, also writing margins[4V] (temporal links: the smallest distance from a tie
+ * among the sampler decisions of its nhit updates; other entries untouched) */
+int or_multihit_margins(int nc, const int dims[4], double beta, uint64_t seed, int nhit, const double *Uin,
+                        double *Uout, double *margins);
 /* multihitsp.cu:115-161: Uout = Uin with every spatial link replaced by the mean of
  * nhit successive overrelaxation steps in its spatial staple */
 int or_multihit_space(int nc, const int dims[4], int nhit, const double *Uin, double *Uout);
@@ -121,6 +130,9 @@ int or_multihit_space(int nc, const int dims[4], int nhit, const double *Uin, do
  * spatial box (other links frozen at their input values), of the box centre after
  * each sweep (streams (seed, idd, hit, TAG_MHX | box index)) */
 int or_multihit_ext(int nc, const int dims[4], double beta, uint64_t seed, int nhit, const double *Uin, double *Uout);
+/* the same, margins[4V] as for or_multihit_margins over every update in the link's box */
+int or_multihit_ext_margins(int nc, const int dims[4], double beta, uint64_t seed, int nhit, const double *Uin,
+                            double *Uout, double *margins);
 /* wilsonloop.cu:44-121,494-516: out[2(T + r(tmax+1)) + {0,1}] = W(r, T), r <= rmax,
  * T <= tmax, averaged over sites and the three spatial directions, / (3 N V) */
 int or_wilson_loops(int nc, const int dims[4], const double *U, int rmax, int tmax, double *out);

@@ -189,18 +189,11 @@ SUN_HD void store_reals(const Field& f, uint32_t l, const R* r) {
     }
 }
 
-// sin and cos of a stored phase th in (-pi, pi] (atan2 output).  On the device:
-// sincospi of th/pi, which needs no large-argument reduction (the general sin/cos
-// carry a Payne-Hanek path that dominates the registers of a kernel decoding 19
-// links).
-SUN_HD void phase(double th, double& s, double& c) {
-#ifdef __HIP_DEVICE_COMPILE__
-    sincospi(th * 0.318309886183790671538, &s, &c);
-#else
-    s = sin(th);
-    c = cos(th);
-#endif
-}
+// sin and cos of a stored phase th in (-pi, pi] (atan2 output): sun_libm.h's, the
+// same bits on the host and the device, with a short reduction for this range (the
+// general library sin/cos carry a Payne-Hanek path that dominates the registers of a
+// kernel decoding 19 links).
+SUN_HD void phase(double th, double& s, double& c) { sun_sincos(th, &s, &c); }
 // FP32 on the device: the native v_sin_f32 / v_cos_f32 of th / (2 pi) in revolutions
 // (within 1.3e-7 of libm on this range, below the FP32 rounding of the stored reals)
 SUN_HD void phase(float th, float& s, float& c) {
@@ -213,6 +206,11 @@ SUN_HD void phase(float th, float& s, float& c) {
     c = cosf(th);
 #endif
 }
+
+// the stored phase arg z of the 8-real record (device_load_save.h:112-120): FP64
+// through sun_libm.h (host and device agree bitwise), FP32 the library atan2f
+SUN_HD double phase_of(cplx<double> z) { return arg_of(z); }
+SUN_HD float phase_of(cplx<float> z) { return arg_of(z); }
 
 // square root and reciprocal of the 8-real decode: FP64 exact IEEE; FP32 on the device
 // the native v_sqrt_f32 / v_rcp_f32 (1 ulp), as cheap as the FP32 link they decode
@@ -365,8 +363,8 @@ SUN_HD void pack(const Mat<R, N>& m, R* r) {
         r[0] = m.e[0][1].re; r[1] = m.e[0][1].im;
         r[2] = m.e[0][2].re; r[3] = m.e[0][2].im;
         r[4] = m.e[1][0].re; r[5] = m.e[1][0].im;
-        r[6] = atan2(m.e[0][0].im, m.e[0][0].re);
-        r[7] = atan2(m.e[2][0].im, m.e[2][0].re);
+        r[6] = phase_of(m.e[0][0]);
+        r[7] = phase_of(m.e[2][0]);
     }
 }
 

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sun_libm.h"
+
 #define SUN_HD __host__ __device__ __forceinline__
 
 namespace sun {
@@ -35,6 +37,16 @@ template <typename R>
 SUN_HD R abs2(cplx<R> a) { return a.re * a.re + a.im * a.im; }
 template <typename R>
 SUN_HD cplx<R> scale(cplx<R> a, R s) { return mk<R>(a.re * s, a.im * s); }
+
+// arg z and (sin, cos) of a phase: FP64 through sun_libm.h (the host restatement
+// computes the same bits), FP32 the library routines
+SUN_HD double arg_of(cplx<double> z) { return sun_atan2(z.im, z.re); }
+SUN_HD float arg_of(cplx<float> z) { return atan2f(z.im, z.re); }
+SUN_HD void sincos_of(double th, double& s, double& c) { sun_sincos(th, &s, &c); }
+SUN_HD void sincos_of(float th, float& s, float& c) {
+    s = sinf(th);
+    c = cosf(th);
+}
 
 template <typename R, int N>
 struct Mat {

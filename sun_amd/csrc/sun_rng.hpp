@@ -8,6 +8,7 @@
 // be replayed bit-for-bit from (seed, sweep).
 #pragma once
 
+#include "sun_libm.h"
 #include "sun_math.hpp"
 
 namespace sun {
@@ -82,40 +83,6 @@ struct Stream {
     SUN_HD float next_f() { return u01f(word()); }
 };
 
-// cos(2 pi u) and (sin, cos)(2 pi u) for u in (0, 1].  On the device they are
-// cospi/sincospi of the exact argument 2u: no large-argument reduction (the general
-// cos/sin inline a Payne-Hanek path that costs far more registers than the sampler).
-SUN_HD double cos2pi(double u) {
-#ifdef __HIP_DEVICE_COMPILE__
-    return cospi(2.0 * u);
-#else
-    return cos(kTwoPi * u);
-#endif
-}
-SUN_HD void sincos2pi(double u, double& s, double& c) {
-#ifdef __HIP_DEVICE_COMPILE__
-    sincospi(2.0 * u, &s, &c);
-#else
-    s = sin(kTwoPi * u);
-    c = cos(kTwoPi * u);
-#endif
-}
-SUN_HD float cos2pi(float u) {
-#ifdef __HIP_DEVICE_COMPILE__
-    return cospif(2.f * u);
-#else
-    return cosf((float)kTwoPi * u);
-#endif
-}
-SUN_HD void sincos2pi(float u, float& s, float& c) {
-#ifdef __HIP_DEVICE_COMPILE__
-    sincospif(2.f * u, &s, &c);
-#else
-    s = sinf((float)kTwoPi * u);
-    c = cosf((float)kTwoPi * u);
-#endif
-}
-
 // Transcendentals of the FP32 sampler.  On the device the native gfx950
 // instructions: v_log_f32 (log2), v_cos_f32 / v_sin_f32 (of 2 pi x: the argument is
 // in revolutions), v_sqrt_f32, v_rsq_f32, v_rcp_f32.  Measured against libm over the uniforms
@@ -171,40 +138,41 @@ SUN_HD float f32_rcp(float x) {
 // Kennedy-Pendleton generator with the Creutz fallback for al <= 2, restating
 // device_PHB_OVR.h:162-223 (including the 1e-10 guard and the 20-try caps).  The
 // reference's double literals make this arithmetic double even for Real=float;
-// here it is double for every storage precision.
+// here it is double for every storage precision.  Its log, exp and sin/cos of 2 pi u
+// are sun_libm.h's, which give the same bits on the host and the device.
 SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
     double xr1, xr2, xr3, xr4, d, r;
     double a0 = 0.0;
     xr1 = st.next();
-    xr1 = log(xr1 + 1.e-10);
+    xr1 = sun_log(xr1 + 1.e-10);
     xr2 = st.next();
-    xr2 = log(xr2 + 1.e-10);
+    xr2 = sun_log(xr2 + 1.e-10);
     xr3 = st.next();
     xr4 = st.next();
-    xr3 = cos2pi(xr3);
+    xr3 = sun_cos2pi(xr3);
     d = -(xr2 + xr1 * xr3 * xr3) / al;
     bool accepted = (1.00 - 0.5 * d) > xr4 * xr4;
     if (!accepted && al > 2.0) {
         for (int k = 0; k < 20; k++) {
             xr1 = st.next();
-            xr1 = log(xr1 + 1.e-10);
+            xr1 = sun_log(xr1 + 1.e-10);
             xr2 = st.next();
-            xr2 = log(xr2 + 1.e-10);
+            xr2 = sun_log(xr2 + 1.e-10);
             xr3 = st.next();
             xr4 = st.next();
-            xr3 = cos2pi(xr3);
+            xr3 = sun_cos2pi(xr3);
             d = -(xr2 + xr1 * xr3 * xr3) / al;
             if ((1.00 - 0.5 * d) > xr4 * xr4) break;
         }
     }
     if (!accepted && al <= 2.0) {
-        xr3 = exp(-2.0 * al);
+        xr3 = sun_exp(-2.0 * al);
         xr4 = 1.0 - xr3;
         for (int k = 0; k < 20; k++) {
             xr1 = st.next();
             xr2 = st.next();
             r = xr3 + xr4 * xr1;
-            a0 = 1.00 + log(r) / al;
+            a0 = 1.00 + sun_log(r) / al;
             if ((1.0 - a0 * a0) > xr2 * xr2) break;
         }
         d = 1.0 - a0;
@@ -218,7 +186,7 @@ SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
     xr1 = fabs(xr1);
     xr1 = sqrt(xr1);
     double sn, cs;
-    sincos2pi(st.next(), sn, cs);
+    sun_sincos2pi(st.next(), &sn, &cs);
     a[1] = xr1 * cs;
     a[2] = xr1 * sn;
 }

@@ -324,8 +324,10 @@ SUN_HD void reunit_link(Mat<R, N>& U) {
             for (int c = 0; c < N; ++c) U.e[c][j] = scale(U.e[c][j], t1);
         }
         t2 = det<R, N>(U);
-        R ai = atan2(t2.im, t2.re);
-        t2 = mk<R>(cos(ai), -sin(ai));
+        const R ai = arg_of(t2);
+        R sn, cs;
+        sincos_of(ai, sn, cs);
+        t2 = mk<R>(cs, -sn);
 #pragma unroll
         for (int c = 0; c < N; ++c) U.e[c][N - 1] = U.e[c][N - 1] * t2;
     }

@@ -2,7 +2,7 @@
 
 * sun_generate on the reference's own small case (SU(3) 8^4, beta=6.0, Haar hot
   start seed 1, FP64 18-real, 10 x (1 heatbath + 4 overrelaxation), plaquette every
-  iteration) tracks the FP64 oracle iteration by iteration;
+  iteration) runs free beside the FP64 oracle's chain and agrees link by link;
 * a captured-and-replayed hipGraph produces bit-identical links and observables to
   eager launches, on every replay;
 * a lattice gives bit-identical results alone and inside a batch of lattices of
@@ -17,38 +17,31 @@ import sun_amd
 pytestmark = pytest.mark.gpu
 
 
-def self_sensitivity(fn, U, eps):
-    rng = np.random.default_rng(7)
-    A = U.copy()
-    B = U * (1 + eps * rng.standard_normal(U.shape))
-    fn(A)
-    fn(B)
-    return float(np.abs(A - B).max())
-
-
-def test_generate_config1_lockstep(ctx, oracle):
-    """Each of the 10 iterations, from the same input on both sides, matches the
-    oracle link by link within the conditioning of that iteration (Haar start:
-    ~1e-6 per 1e-15 of input at iteration 0, ~1e-12 once thermalised)."""
-    dims, beta, seed = (8, 8, 8, 8), 6.0, 1
+def test_generate_config1_chain(ctx, oracle):
+    """The reference's own small case, run free on both sides: SU(3) 8^4, beta 6.0,
+    Haar start from seed 1, FP64 18-real, 10 iterations of 1 heatbath + 4
+    overrelaxation sweeps through one sun_generate call (hipGraph replay), against
+    the oracle's chain from the same start.  The device runs the oracle's FP64
+    operations in the oracle's order with the same portable transcendentals, so the
+    links agree to 1e-12 after 10 chaotic iterations (a sweep amplifies a 1e-15
+    input difference ~100x) and every iteration's plaquette and Polyakov loop to
+    1e-13."""
+    dims, beta, seed, n_iter = (8, 8, 8, 8), 6.0, 1, 10
+    U = oracle.hot_start(3, dims, seed)
     with sun_amd.Lattice(ctx, 3, dims, beta, 18, 8) as lat:
         lat.hot_start(seed)
-        for it in range(10):
-            U = lat.to_host()
-            obs = lat.generate(seed=seed, n_iter=1, n_or=4, reunit_every=0, first_sweep=it)
-            G = lat.to_host()
-
-            def one_iteration(A):
-                oracle.heatbath_sweep(A, dims, beta, seed, it)
-                for _ in range(4):
-                    oracle.overrelax_sweep(A, dims)
-
-            sens = self_sensitivity(one_iteration, U, 2.0 ** -52)
-            one_iteration(U)
-            bound = max(1e-12, 50 * sens)
-            assert np.abs(G - U).max() <= bound, (it, np.abs(G - U).max(), sens)
-            assert np.abs(obs[0, :2] - oracle.plaquette(U, dims)).max() <= max(1e-13, bound)
-            assert abs(complex(obs[0, 2], obs[0, 3]) - oracle.polyakov(U, dims)) <= max(1e-13, bound)
+        assert np.abs(lat.to_host() - U).max() <= 1e-15
+        obs = lat.generate(seed=seed, n_iter=n_iter, n_or=4, reunit_every=0, first_sweep=0)
+        G = lat.to_host()
+    for it in range(n_iter):
+        oracle.heatbath_sweep(U, dims, beta, seed, it)
+        for _ in range(4):
+            oracle.overrelax_sweep(U, dims)
+        assert np.abs(obs[it, :2] - oracle.plaquette(U, dims)).max() <= 1e-13, it
+        assert abs(complex(obs[it, 2], obs[it, 3]) - oracle.polyakov(U, dims)) <= 1e-13, it
+    d = float(np.abs(G - U).max())
+    print(f"config 1 chain: max link difference after {n_iter} iterations {d:.3e}")
+    assert d <= 1e-12
 
 
 @pytest.mark.parametrize("nc,beta,storage", [(3, 6.0, 12), (2, 2.3, 18), (4, 11.0, 18)])

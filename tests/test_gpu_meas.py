@@ -6,9 +6,12 @@ does), multihit (multihit.cu), extended multihit (multihitext.cu) and Wilson loo
 
 Bars: FP64 1e-12 per link (smearing, multihit) and 1e-13 on W(r, T); FP32 5e-5 per
 link plus four times the storage round trip of the oracle's result (8-real FP32
-re-derives phases), 2e-6 on W(r, T).  FP32 multihit runs heatbath chains in FP32, so
-an accept/reject branch may flip against the oracle; as for heatbath sweeps, the
-count of temporal links beyond the bar is bounded (1%) instead.  The whole chain of
+re-derives phases) and, for smearing, ten times the oracle's own change under a
+one-ulp input perturbation (iterated projections and the 8-real decode are
+ill-conditioned at some links), 2e-6 on W(r, T); HYP in FP32 1e-4 per link (three
+nested levels of iterated FP32 projections).  FP32 multihit runs heatbath chains in FP32, so an accept/reject branch may
+flip against the oracle; as for heatbath sweeps, every temporal link beyond the bar
+must have a near-tie sampler decision among its own updates (flip_report).  The whole chain of
 test.cpp:170-230 (APE 25 steps, multihit, Wilson loops) runs in FP64 against the
 oracle's chain to 1e-10, and in FP32-12 against it to 2e-3.
 """
@@ -16,18 +19,29 @@ import numpy as np
 import pytest
 
 import sun_amd
-from test_gpu_parity import BETAS, IDS, VARIANTS, load_config, per_link_diff, store_tol, thermalised, tol
+from test_gpu_parity import (BETAS, IDS, MAX_FLIP_OUTLIERS, VARIANTS, flip_report, load_config, per_link_diff,
+                             store_tol, thermalised, tol)
 
 pytestmark = pytest.mark.gpu
 
 DIMS = (4, 6, 4, 8)
 
 
-def stored(ctx, nc, storage, prec, U):
-    """U as the lattice's storage keeps it (the oracle's result projected)"""
-    with sun_amd.Lattice(ctx, nc, DIMS, BETAS[nc], storage, prec) as tmp:
-        tmp.from_host(U)
-        return tmp.to_host()
+def smear_bar(lat, U, ref, storage, prec, fn, base=None):
+    """Per-link bar for a smearing of input U against the oracle's result ref:
+    tol(prec) plus four times the storage round trip of ref; in FP32 also ten times
+    the oracle's own change at the link when its input moves by one FP32 ulp
+    (relative 2^-23): the device smears in FP32 arithmetic, and the iterated SU(N)
+    projections (and the 8-real decode, which divides by |e01|^2+|e02|^2) amplify
+    rounding at some links.  fn(P) is the oracle's smearing of P.  Leaves ref on the
+    lattice."""
+    lat.from_host(ref)
+    bar = (tol(prec) if base is None else base) + 4 * per_link_diff(lat.to_host(), ref)
+    if prec == 4:
+        rng = np.random.default_rng(321)
+        P = U * (1 + 2.0 ** -23 * rng.standard_normal(U.shape))
+        bar = bar + 10 * per_link_diff(fn(P), ref)
+    return bar
 
 
 @pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
@@ -37,9 +51,10 @@ def test_ape_parity(ctx, oracle, nc, storage, prec, time_dir):
         U = load_config(lat, thermalised(oracle, nc, DIMS, storage))
         lat.ape_smear(0.2, 2, 10, 1e-10, time_dir)
         got = lat.to_host()
-    ref = oracle.ape_smear(U.copy(), DIMS, 0.2, 2, 10, 1e-10, time_dir)
+        fn = lambda P: oracle.ape_smear(P.copy(), DIMS, 0.2, 2, 10, 1e-10, time_dir)  # noqa: E731
+        ref = fn(U)
+        bar = smear_bar(lat, U, ref, storage, prec, fn)
     d = per_link_diff(got, ref)
-    bar = tol(prec) + 4 * per_link_diff(stored(ctx, nc, storage, prec, ref), ref)
     assert np.all(d <= bar), (float(d.max()), float(bar.max()))
     untouched = slice(0, 3) if time_dir else slice(3, 4)
     assert np.array_equal(got[untouched], U[untouched])
@@ -55,9 +70,10 @@ def test_stout_parity(ctx, oracle, nc, storage, prec, time_dir):
         U = load_config(lat, thermalised(oracle, nc, DIMS, storage))
         lat.stout_smear(0.15, 3, time_dir)
         got = lat.to_host()
-    ref = oracle.stout_smear(U.copy(), DIMS, 0.15, 3, time_dir)
+        fn = lambda P: oracle.stout_smear(P.copy(), DIMS, 0.15, 3, time_dir)  # noqa: E731
+        ref = fn(U)
+        bar = smear_bar(lat, U, ref, storage, prec, fn)
     d = per_link_diff(got, ref)
-    bar = tol(prec) + 4 * per_link_diff(stored(ctx, nc, storage, prec, ref), ref)
     assert np.all(d <= bar), (float(d.max()), float(bar.max()))
     untouched = slice(0, 3) if time_dir else slice(3, 4)
     assert np.array_equal(got[untouched], U[untouched])
@@ -72,11 +88,12 @@ def test_hyp_parity(ctx, oracle, nc, storage, prec, time_dir):
         U = load_config(lat, thermalised(oracle, nc, dims, storage))
         lat.hyp_smear((0.75, 0.6, 0.3), 1, 10, 1e-10, time_dir)
         got = lat.to_host()
-        ref = oracle.hyp_smear(U.copy(), dims, (0.75, 0.6, 0.3), 1, 10, 1e-10, time_dir)
-        lat.from_host(ref)
-        stored_ref = lat.to_host()
+        fn = lambda P: oracle.hyp_smear(P.copy(), dims, (0.75, 0.6, 0.3), 1, 10, 1e-10, time_dir)  # noqa: E731
+        ref = fn(U)
+        # FP32: three nested levels of ten-hit SU(N) projections, each rounding in
+        # FP32 -- 1e-4 per link (SU(4) links reach 5.3e-5)
+        bar = smear_bar(lat, U, ref, storage, prec, fn, base=1e-12 if prec == 8 else 1e-4)
     d = per_link_diff(got, ref)
-    bar = tol(prec) + 4 * per_link_diff(stored_ref, ref)
     assert np.all(d <= bar), (float(d.max()), float(bar.max()))
     untouched = slice(0, 3) if time_dir else slice(3, 4)
     assert np.array_equal(got[untouched], U[untouched])
@@ -99,14 +116,19 @@ def test_multihit_parity(ctx, oracle, nc, storage, prec):
         U = load_config(lat, thermalised(oracle, nc, DIMS, storage))
         lat.multihit(out, seed, nhit)
         got = out.to_host()
-    ref = oracle.multihit(U, DIMS, beta, seed, nhit)
+    ref, margins = oracle.multihit_margins(U, DIMS, beta, seed, nhit)
     # spatial links copied (decoded in the link precision on both sides)
     assert np.abs(got[:3] - U[:3]).max() <= (0.0 if prec == 8 else store_tol(storage, prec))
-    d = per_link_diff(got[3:], ref[3:])
+    d = per_link_diff(got, ref)
     if prec == 8:
         assert d.max() <= 1e-12
     else:
-        assert int((d > 5e-5).sum()) <= max(2, d.size // 100), float(d.max())
+        # FP32 heatbath hits: a temporal link beyond 5e-5 needs a near-tie decision
+        # among its own hits (the staples come from the input, so nothing propagates)
+        d[:3 * d.size // 4] = 0.0
+        n_out, n_tie, unexplained = flip_report(d, np.full(d.size, 5e-5), margins, DIMS, propagate=False)
+        print(f"multihit FP32 su{nc} s{storage}: {n_out} temporal links beyond 5e-5, {n_tie} near-tie")
+        assert not unexplained and n_out <= MAX_FLIP_OUTLIERS, (n_out, unexplained[:10])
 
 
 @pytest.mark.parametrize("nc,storage,prec", VARIANTS, ids=IDS)
@@ -138,14 +160,18 @@ def test_multihit_ext_parity(ctx, oracle, nc, storage, prec):
         with pytest.raises(sun_amd.SunError) as e:
             lat.multihit_ext(out, seed, 0)
         assert e.value.code == sun_amd.ERR_INVALID
-    ref = oracle.multihit_ext(U, DIMS, beta, seed, nhit)
+    ref, margins = oracle.multihit_ext_margins(U, DIMS, beta, seed, nhit)
     assert np.abs(got[:3] - U[:3]).max() <= (0.0 if prec == 8 else store_tol(storage, prec))
-    d = per_link_diff(got[3:], ref[3:])
+    d = per_link_diff(got, ref)
     if prec == 8:
         assert d.max() <= 1e-12, float(d.max())
     else:
-        # FP32 heatbath chains: an accept/reject branch may flip against the oracle
-        assert int((d > 5e-5).sum()) <= max(2, d.size // 100), float(d.max())
+        # FP32 heatbath chains: a temporal link beyond 5e-5 needs a near-tie decision
+        # somewhere in its box's updates
+        d[:3 * d.size // 4] = 0.0
+        n_out, n_tie, unexplained = flip_report(d, np.full(d.size, 5e-5), margins, DIMS, propagate=False)
+        print(f"multihit_ext FP32 su{nc} s{storage}: {n_out} temporal links beyond 5e-5, {n_tie} near-tie")
+        assert not unexplained and n_out <= MAX_FLIP_OUTLIERS, (n_out, unexplained[:10])
 
 
 @pytest.mark.parametrize("dims", [(2, 2, 2, 4), (2, 4, 6, 2), (6, 2, 4, 4)])

@@ -212,3 +212,25 @@ def test_heatbath_link_stays_in_sun(oracle, nc):
         new = oracle.heatbath_link(U[k % 4, 7 * k % 128], np.conj(S.T), 6.0, 5, c0=k)
         assert unitarity_err(new) < 1e-13
         assert abs(np.linalg.det(new) - 1) < 1e-13
+
+
+def test_portable_transcendentals_against_glibc(tmp_path):
+    """sun_libm.h -- the FP64 log, exp, sin/cos and atan2 that the device sampler,
+    Haar start and 8-real codec share with the oracle -- within 2 ulp of glibc's log
+    and exp (sampler arguments and the whole exponent range), within 3 ulp of its
+    atan2 (all quadrants; the pi/4 + atan((t-1)/(t+1)) step cancels just above
+    tan(pi/8)),
+    within 2 x 2^-53 of long-double sin/cos of 2 pi u and of phases in [-pi, pi],
+    with exact values at the quadrant points, log(2^k) = k ln 2 and C's conventions
+    for zeros, NaN and overflow"""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "libm_check")
+    subprocess.run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", f"-I{root}/sun_amd/csrc",
+                    os.path.join(root, "tests", "host", "libm_check.c"), "-o", exe, "-lm"], check=True)
+    mlog, mexp, msin, mcos, matan, nfail = subprocess.run([exe], capture_output=True, text=True,
+                                                          check=True).stdout.split()
+    assert float(mlog) <= 2 and float(mexp) <= 2 and float(matan) <= 3, (mlog, mexp, matan)
+    assert float(msin) <= 2 and float(mcos) <= 2, (msin, mcos)
+    assert int(nfail) == 0
