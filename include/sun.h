@@ -207,6 +207,17 @@ int sun_multihit_ext(sun_lattice *in, sun_lattice *out, uint64_t seed, int nhit,
  * W(r, T) = sum_{x, mu<3} Tr(L_r(x) t1 L_r(x+T e_3)^+ t0^+) / (3 N V) for r <= rmax,
  * T <= tmax (rmax at most the smallest spatial extent, tmax at most the temporal). */
 int sun_wilson_loops(sun_lattice *lat, int rmax, int tmax, double *out, void *stream);
+/* sun_wilson_loops without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under stream/graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and results (bit for bit), but the
+ * Wilson-line field and the partial sums live in a fixed 32 MiB device array (one per
+ * GPU, held until the process ends) and W(r, T) goes to DEVICE memory dev_out (16-byte
+ * aligned, 2 (rmax+1)(tmax+1) doubles, same layout as out), asynchronously on the
+ * stream.  Needs 3V x 2N^2 x precision bytes for the line field plus 48 (tmax+1)
+ * ceil(V/256) for the partials: SU(3) up to 16^4 in FP64 and 16^3x32 in FP32;
+ * larger requests fail with SUN_ERR_BUDGET.  Calls on one GPU share the array: calls
+ * that overlap in time on one GPU (other streams, other threads) race. */
+int sun_wilson_loops_noalloc(sun_lattice *lat, int rmax, int tmax, double *dev_out, void *stream);
 
 /* ---- host exchange (gaugearray.h:521-596 Get/Set, canonical FP64 layout) ---------------- */
 int sun_links_to_host(sun_lattice *lat, double *host, int layout);

@@ -46,7 +46,7 @@ EXPORTS = [
     "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
     "sun_link_monitor", "sun_link_monitor_async", "sun_link_trace", "sun_link_det",
     "sun_ape_smear", "sun_stout_smear", "sun_hyp_smear", "sun_multihit", "sun_multihit_space", "sun_multihit_ext",
-    "sun_wilson_loops",
+    "sun_wilson_loops", "sun_wilson_loops_noalloc",
     "sun_links_to_host", "sun_links_from_host",
     "sun_generate",
     "sun_batch_create", "sun_batch_destroy", "sun_batch_get_launch", "sun_batch_set_launch",
@@ -164,6 +164,7 @@ def lib() -> ctypes.CDLL:
         "sun_multihit_space": (ci, [vp, vp, ci, vp]),
         "sun_multihit_ext": (ci, [vp, vp, u64, ci, vp]),
         "sun_wilson_loops": (ci, [vp, ci, ci, dp, vp]),
+        "sun_wilson_loops_noalloc": (ci, [vp, ci, ci, vp, vp]),
         "sun_links_to_host": (ci, [vp, dp, ci]),
         "sun_links_from_host": (ci, [vp, dp, ci]),
         "sun_generate": (ci, [vp, ctypes.POINTER(GenerateDesc), vp, dp]),
@@ -390,6 +391,13 @@ class Lattice:
         out = np.zeros((rmax + 1) * (tmax + 1) * 2)
         _check(lib().sun_wilson_loops(self.h, rmax, tmax, _dp(out), stream), "sun_wilson_loops")
         return (out[0::2] + 1j * out[1::2]).reshape(rmax + 1, tmax + 1)
+
+    def wilson_loops_noalloc(self, rmax: int, tmax: int, dev_out: int, stream=None):
+        """W(r, T) into DEVICE memory at address dev_out (2 (rmax+1)(tmax+1) doubles,
+        16-byte aligned), asynchronously on stream, without allocating: the line field
+        and partial sums use the device's fixed scratch array"""
+        _check(lib().sun_wilson_loops_noalloc(self.h, rmax, tmax, ctypes.c_void_p(dev_out), stream),
+               "sun_wilson_loops_noalloc")
 
     # -- host exchange (canonical layout, complex128 [4, V, N, N])
     def to_host(self) -> np.ndarray:

@@ -39,6 +39,34 @@ extern "C" const char* sun_version(void) { return "sun-mi355x 0.1 (gfx950)"; }
 // ---------------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------------
+// Scratch of the allocation-free entry points (sun_wilson_loops_noalloc): a device
+// array of fixed size instead of hipMalloc, for callers that cannot allocate (code
+// under stream capture, serving loops).  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address on each GPU is looked up once, at
+// the first sun_init on that device, and only that address is used afterwards.
+static __device__ __align__(256) unsigned char scratch[sun::kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// the device address of this device's scratch array; the device is current
+int lookup_scratch(int device, void** ptr) {
+    if (device >= kMaxDevices) return fail(SUN_ERR_UNSUPPORTED, "sun_init: device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SUN_ERR_HIP, "sun_init: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_dev[device] = p;
+    }
+    *ptr = g_scratch_dev[device];
+    return SUN_OK;
+}
+}  // namespace
+
 extern "C" int sun_init(int device, int verbosity, int tune, sun_ctx** out) {
     if (!out) return fail(SUN_ERR_INVALID, "sun_init: out is NULL");
     *out = nullptr;
@@ -56,6 +84,11 @@ extern "C" int sun_init(int device, int verbosity, int tune, sun_ctx** out) {
         c->name = prop.name;
         c->gcn_arch = prop.gcnArchName;
         c->num_cu = prop.multiProcessorCount;
+    }
+    const int rc = lookup_scratch(device, &c->scratch);
+    if (rc != SUN_OK) {
+        delete c;
+        return rc;
     }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {

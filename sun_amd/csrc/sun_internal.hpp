@@ -91,9 +91,16 @@ struct sun_ctx {
     hipStream_t stream = nullptr;
     std::mutex mu;
     sun::Tuner tuner;
+    // this device's copy of the allocation-free entry points' scratch array (its
+    // device address, looked up once per device at sun_init)
+    void* scratch = nullptr;
 };
 
 namespace sun {
+// size of the device scratch array of the allocation-free entry points (one per
+// device, held until the process ends): sun_wilson_loops_noalloc keeps its Wilson-line
+// field (3V links in 18-real storage) and its partial sums there
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
 // device result slots of a lattice: plaquette and Polyakov of sun_generate, or the
 // three of the link monitor
 constexpr int kResSlots = 3;

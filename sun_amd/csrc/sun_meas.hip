@@ -207,3 +207,41 @@ static int sun_wilson_loops_body(sun_lattice* lat, int rmax, int tmax, double* o
 extern "C" int sun_wilson_loops(sun_lattice* lat, int rmax, int tmax, double* out, void* stream) {
     return guard_alloc("sun_wilson_loops", [&] { return sun_wilson_loops_body(lat, rmax, tmax, out, stream); });
 }
+
+// The same kernels with the line field and the partial sums in the device's scratch
+// array (its address from sun_init) and the results written straight to the caller's
+// device buffer: no allocation and no host synchronisation.  The line field is set to
+// the identity by the r = 0 pass and every partial is written before it is read, so
+// nothing left in the array by an earlier call is used.
+extern "C" int sun_wilson_loops_noalloc(sun_lattice* lat, int rmax, int tmax, double* dev_out, void* stream) {
+    SUN_TRY(check_lattice(lat, "sun_wilson_loops_noalloc"));
+    if (!dev_out || (reinterpret_cast<uintptr_t>(dev_out) & 15u) != 0)
+        return fail(SUN_ERR_INVALID, "sun_wilson_loops_noalloc: dev_out must be a 16-byte aligned device pointer");
+    const Geom& g = lat->g;
+    if (rmax < 0 || tmax < 0 || rmax > g.X[0] || rmax > g.X[1] || rmax > g.X[2] || tmax > g.X[3])
+        return fail(SUN_ERR_INVALID, "sun_wilson_loops_noalloc: rmax %d, tmax %d outside the lattice", rmax, tmax);
+    if (!lat->ctx->scratch) return fail(SUN_ERR_STATE, "sun_wilson_loops_noalloc: no scratch array on this device");
+    const size_t full = (size_t)2 * lat->d.nc * lat->d.nc * (size_t)lat->d.precision;  // 18-real record
+    const size_t nline = 3 * (size_t)g.V;
+    const int nb = monitor_blocks(g);
+    const size_t line_bytes = (nline * full + 255) & ~(size_t)255;
+    const size_t part_bytes = (size_t)3 * (tmax + 1) * nb * sizeof(double2);
+    if (line_bytes + part_bytes > kNoAllocScratchBytes)
+        return fail(SUN_ERR_BUDGET,
+                    "sun_wilson_loops_noalloc: needs %zu bytes of scratch (line field %zu + partials %zu), the "
+                    "array holds %zu; use sun_wilson_loops",
+                    line_bytes + part_bytes, line_bytes, part_bytes, kNoAllocScratchBytes);
+    hipStream_t s = pick_stream(lat, stream);
+    HIP_TRY(hipSetDevice(lat->ctx->device));
+    unsigned char* base = static_cast<unsigned char*>(lat->ctx->scratch);
+    const Field lf = make_field(base, nline, full);
+    double2* part = reinterpret_cast<double2*>(base + line_bytes);
+    double2* res = reinterpret_cast<double2*>(dev_out);
+    const double scale = 1.0 / (3.0 * lat->d.nc * (double)g.V);  // wilsonloop.cu:512
+    for (int r = 0; r <= rmax; r++) {
+        HIP_TRY(lat->ops->wl_lines(lat->f, lf, g, r, s));
+        HIP_TRY(lat->ops->wl_loops(lat->f, lf, g, r, tmax, part, s));
+        HIP_TRY(launch_wl_finalize(part, nb, tmax, scale, res + (size_t)r * (tmax + 1), s));
+    }
+    return SUN_OK;
+}
