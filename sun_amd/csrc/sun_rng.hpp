@@ -197,7 +197,6 @@ SUN_HD void generate_su2_milc(double al, Stream& st, double a[4]) {
 // instructions above (the library routines cost ~120 more instructions per call).
 SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
     float xr1, xr2, xr3, xr4, d, r;
-    float a0 = 0.f;
     const float inv_al = f32_rcp(al);
     xr1 = st.next_f();
     xr1 = f32_log(xr1 + 1.e-10f);
@@ -228,19 +227,23 @@ SUN_HD void generate_su2_milc_f32(float al, Stream& st, float a[4]) {
             xr1 = st.next_f();
             xr2 = st.next_f();
             r = xr3 + xr4 * xr1;
-            a0 = 1.00f + f32_log(r) * inv_al;
-            if ((1.f - a0 * a0) > xr2 * xr2) break;
+            d = -f32_log(r) * inv_al;  // 1 - a0 for a0 = 1 + log(r) / al
+            if (d * (2.f - d) > xr2 * xr2) break;
         }
-        d = 1.f - a0;
     }
+    // The reference's 1 - a0^2 and 1 - a0^2 - a3^2 (in double there) are formed here
+    // without differences of nearly equal FP32 numbers: 1 - a0^2 = d (2 - d) (at
+    // a0 ~ 1, 1 - a0*a0 in FP32 loses every significant bit: errors of 6e-5 in a link
+    // at 32^4), and with a3 = (2u - 1) r, 1 - a0^2 - a3^2 = r^2 * 4u(1 - u), where u and
+    // 1 - u are each converted from the Philox word.
     a[0] = 1.f - d;
-    xr3 = 1.f - a[0] * a[0];
-    xr3 = fabsf(xr3);
+    xr3 = fabsf(d * (2.f - d));
     r = f32_sqrt(xr3);
-    a[3] = (2.f * st.next_f() - 1.f) * r;
-    xr1 = xr3 - a[3] * a[3];
-    xr1 = fabsf(xr1);
-    xr1 = f32_sqrt(xr1);
+    const uint32_t w = st.word();
+    const float u = fmaf((float)w, 0x1p-32f, 0x1p-32f);  // (w + 1) 2^-32, the draw as next_f()
+    const float v = (float)(~w) * 0x1p-32f;              // 1 - u = (2^32 - 1 - w) 2^-32
+    a[3] = (u - v) * r;
+    xr1 = f32_sqrt(xr3 * (4.f * u * v));
     float sn, cs;
     f32_sincos2pi(st.next_f(), sn, cs);
     a[1] = xr1 * cs;
