@@ -162,36 +162,73 @@ double or_u01(uint32_t w) {
     return ((double)w + 1.0) * 0x1p-32;
 }
 
-/* uniform source: a Philox stream, or an explicit list (tests).  Draw k of stream
+/* sequential uniform stream (the Haar start, or_stream_uniforms): draw k of stream
  * (seed, c0, c2, c3) is word k%4 of Philox4x32-10((c0, k/4, c2, c3), seed). */
 typedef struct {
     uint32_t key[2];
     uint32_t c0, c2, c3, j;
     uint32_t w[4];
     int nw;
-    const double *list;
-    int nlist, pos;
 } usrc;
 
 static void usrc_stream(usrc *s, uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3) {
     s->key[0] = (uint32_t)seed; s->key[1] = (uint32_t)(seed >> 32);
-    s->c0 = c0; s->c2 = c2; s->c3 = c3; s->j = 0; s->nw = 0; s->list = NULL;
-    s->nlist = 0; s->pos = 0;
+    s->c0 = c0; s->c2 = c2; s->c3 = c3; s->j = 0; s->nw = 0;
 }
 static inline double unext(usrc *s) {
-    if (s->list) {
-        double v = (s->pos < s->nlist) ? s->list[s->pos] : 1.0;
-        s->pos++;
-        return v;
-    }
     if (s->nw == 0) {
         uint32_t ctr[4] = { s->c0, s->j, s->c2, s->c3 };
         or_philox4x32_10(ctr, s->key, s->w);
         s->j++;
         s->nw = 4;
     }
-    s->pos++;
     return or_u01(s->w[4 - s->nw--]);
+}
+
+/* Uniforms of one SU(2) subgroup update, block b of a link update with counter words
+ * (c0, c2, c3): attempt k (k = 0 the first Kennedy-Pendleton attempt, k >= 1 its
+ * retries or the Creutz tries) reads the words of Philox((c0, 32 b + k, c2, c3), seed),
+ * the final two draws words 0 and 1 of Philox((c0, 32 b + 31, c2, c3)) -- the engine's
+ * layout (sun_rng.hpp), a choice of this build (the reference streams cuRAND
+ * sequentially).  An explicit list (tests) is read in order instead: 4 uniforms per
+ * Kennedy-Pendleton attempt, 2 per Creutz try, then the 2 final ones. */
+typedef struct {
+    uint32_t key[2];
+    uint32_t c0, c2, c3, cbase;
+    const double *list;
+    int nlist, pos;
+} bsrc;
+
+static void bsrc_block(bsrc *b, uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int block) {
+    b->key[0] = (uint32_t)seed; b->key[1] = (uint32_t)(seed >> 32);
+    b->c0 = c0; b->c2 = c2; b->c3 = c3; b->cbase = 32u * (uint32_t)block;
+    b->list = NULL; b->nlist = 0; b->pos = 0;
+}
+static double blist(bsrc *b) {
+    double v = (b->pos < b->nlist) ? b->list[b->pos] : 1.0;
+    b->pos++;
+    return v;
+}
+/* the n (4 or 2) uniforms of attempt k */
+static void battempt(bsrc *b, int k, int n, double *u) {
+    if (b->list) {
+        for (int i = 0; i < n; i++) u[i] = blist(b);
+        return;
+    }
+    uint32_t ctr[4] = { b->c0, b->cbase + (uint32_t)k, b->c2, b->c3 }, w[4];
+    or_philox4x32_10(ctr, b->key, w);
+    for (int i = 0; i < n; i++) u[i] = or_u01(w[i]);
+}
+static void bfinal(bsrc *b, double u[2]) {
+    if (b->list) {
+        u[0] = blist(b);
+        u[1] = blist(b);
+        return;
+    }
+    uint32_t ctr[4] = { b->c0, b->cbase + 31u, b->c2, b->c3 }, w[4];
+    or_philox4x32_10(ctr, b->key, w);
+    u[0] = or_u01(w[0]);
+    u[1] = or_u01(w[1]);
 }
 
 void or_stream_uniforms(uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3, int n, double *out) {
@@ -225,16 +262,17 @@ static inline void q_mul_uv_dagger(const double v[4], const double u[4], double 
  * precision can only decide differently where that distance is at its rounding
  * level; the FP32 parity tests use it to attribute links that differ by O(1). */
 #define TIE(m) do { if (margin) { double t_ = fabs(m); if (t_ < *margin) *margin = t_; } } while (0)
-static void gen_su2_milc(double al, usrc *st, double a[4], double *margin) {
+static void gen_su2_milc(double al, bsrc *st, double a[4], double *margin) {
     double xr1, xr2, xr3, xr4, d, r;
     int k;
-    double a0 = 0.0;
-    xr1 = unext(st);
+    double a0 = 0.0, u[4];
+    battempt(st, 0, 4, u);
+    xr1 = u[0];
     xr1 = sun_log(xr1 + 1.e-10);
-    xr2 = unext(st);
+    xr2 = u[1];
     xr2 = sun_log(xr2 + 1.e-10);
-    xr3 = unext(st);
-    xr4 = unext(st);
+    xr3 = u[2];
+    xr4 = u[3];
     xr3 = sun_cos2pi(xr3);
     d = -(xr2 + xr1 * xr3 * xr3) / al;
     int nacd = 0;
@@ -242,13 +280,14 @@ static void gen_su2_milc(double al, usrc *st, double a[4], double *margin) {
     TIE((1.00 - 0.5 * d) - xr4 * xr4);
     if ((1.00 - 0.5 * d) > xr4 * xr4) nacd = 1;
     if (nacd == 0 && al > 2.0) {
-        for (k = 0; k < 20; k++) {
-            xr1 = unext(st);
+        for (k = 1; k <= 20; k++) {
+            battempt(st, k, 4, u);
+            xr1 = u[0];
             xr1 = sun_log(xr1 + 1.e-10);
-            xr2 = unext(st);
+            xr2 = u[1];
             xr2 = sun_log(xr2 + 1.e-10);
-            xr3 = unext(st);
-            xr4 = unext(st);
+            xr3 = u[2];
+            xr4 = u[3];
             xr3 = sun_cos2pi(xr3);
             d = -(xr2 + xr1 * xr3 * xr3) / al;
             TIE((1.00 - 0.5 * d) - xr4 * xr4);
@@ -258,9 +297,10 @@ static void gen_su2_milc(double al, usrc *st, double a[4], double *margin) {
     if (nacd == 0 && al <= 2.0) {
         xr3 = sun_exp(-2.0 * al);
         xr4 = 1.0 - xr3;
-        for (k = 0; k < 20; k++) {
-            xr1 = unext(st);
-            xr2 = unext(st);
+        for (k = 1; k <= 20; k++) {
+            battempt(st, k, 2, u);
+            xr1 = u[0];
+            xr2 = u[1];
             r = xr3 + xr4 * xr1;
             a0 = 1.00 + sun_log(r) / al;
             TIE((1.0 - a0 * a0) - xr2 * xr2);
@@ -272,19 +312,21 @@ static void gen_su2_milc(double al, usrc *st, double a[4], double *margin) {
     xr3 = 1.0 - a[0] * a[0];
     xr3 = fabs(xr3);
     r = sqrt(xr3);
-    a[3] = (2.0 * unext(st) - 1.0) * r;
+    double fin[2];
+    bfinal(st, fin);
+    a[3] = (2.0 * fin[0] - 1.0) * r;
     xr1 = xr3 - a[3] * a[3];
     xr1 = fabs(xr1);
     xr1 = sqrt(xr1);
     /* sin/cos of PII * u (device_PHB_OVR.h:217-221) with the angle in revolutions */
     double sn, cs;
-    sun_sincos2pi(unext(st), &sn, &cs);
+    sun_sincos2pi(fin[1], &sn, &cs);
     a[1] = xr1 * cs;
     a[2] = xr1 * sn;
 }
 
 int or_generate_su2_milc(double al, const double *u, int nu, double a[4]) {
-    usrc s;
+    bsrc s;
     memset(&s, 0, sizeof(s));
     s.list = u; s.nlist = nu; s.pos = 0;
     gen_su2_milc(al, &s, a, NULL);
@@ -294,9 +336,9 @@ int or_generate_su2_milc(double al, const double *u, int nu, double a[4]) {
 void or_sample_a0(double al, uint64_t seed, int n, double *a0_out) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) {
-        usrc s;
+        bsrc s;
         double a[4];
-        usrc_stream(&s, seed, (uint32_t)i, 0u, RNG_TAG_TEST);
+        bsrc_block(&s, seed, (uint32_t)i, 0u, RNG_TAG_TEST, 0);
         gen_su2_milc(al, &s, a, NULL);
         a0_out[i] = a[0];
     }
@@ -639,7 +681,9 @@ static void block_proj_uf(const mat *U, const mat *F, int p, int q, double r[4])
 }
 
 /* device_PHB_OVR.h:361-485 */
-static void heatbath_mat(int nc, double beta_over_nc, mat *U, const mat *F, usrc *st, double *margin) {
+/* counters (seed, c0, c2, c3) of the link; block b draws as bsrc_block describes */
+static void heatbath_mat(int nc, double beta_over_nc, mat *U, const mat *F, uint64_t seed, uint32_t c0, uint32_t c2,
+                         uint32_t c3, double *margin) {
     if (nc == 3) {
         for (int block = 0; block < 3; block++) {
             int p, q;
@@ -650,7 +694,9 @@ static void heatbath_mat(int nc, double beta_over_nc, mat *U, const mat *F, usrc
             double ap = beta_over_nc * k;
             k = 1.0 / k;
             for (int i = 0; i < 4; i++) r[i] *= k;
-            gen_su2_milc(ap, st, a, margin);
+            bsrc st;
+            bsrc_block(&st, seed, c0, c2, c3, block);
+            gen_su2_milc(ap, &st, a, margin);
             q_mul_uv_dagger(a, r, rr);
             cx a0 = cmk(rr[0], rr[3]), a1 = cmk(rr[2], rr[1]);
             cx a2 = cmk(-rr[2], rr[1]), a3 = cmk(rr[0], -rr[3]);
@@ -672,7 +718,9 @@ static void heatbath_mat(int nc, double beta_over_nc, mat *U, const mat *F, usrc
         double ap = beta_over_nc * k;
         k = 1.0 / k;
         for (int i = 0; i < 4; i++) r[i] *= k;
-        gen_su2_milc(ap, st, a, margin);
+        bsrc st;
+        bsrc_block(&st, seed, c0, c2, c3, block);
+        gen_su2_milc(ap, &st, a, margin);
         q_mul_uv_dagger(a, r, rr);
         mul_block_sun(nc, rr, U, x, y);
         mul_block_sun(nc, rr, &M, x, y);
@@ -724,11 +772,9 @@ void or_overrelax_link(int nc, double *U, const double *F) {
 void or_heatbath_link(int nc, double beta, double *U, const double *F,
                       uint64_t seed, uint32_t c0, uint32_t c2, uint32_t c3) {
     mat u, f;
-    usrc st;
-    usrc_stream(&st, seed, c0, c2, c3);
     mload(nc, U, &u);
     mload(nc, F, &f);
-    heatbath_mat(nc, beta / (double)nc, &u, &f, &st, NULL);
+    heatbath_mat(nc, beta / (double)nc, &u, &f, seed, c0, c2, c3, NULL);
     mstore(nc, &u, U);
 }
 
@@ -754,14 +800,13 @@ static int heatbath_sub(int nc, const int X[4], double beta, uint64_t seed, uint
     for (long id = 0; id < half; id++) {
         mat S = staple_at(nc, X, U, parity, mu, (int)id);
         mat F = mdag(nc, &S);
-        usrc st;
-        usrc_stream(&st, seed, (uint32_t)id, sweep, RNG_TAG_HB | ((uint32_t)parity << 4) | (uint32_t)mu);
+        const uint32_t tag = RNG_TAG_HB | ((uint32_t)parity << 4) | (uint32_t)mu;
         double *l = link_at_w(nc, U, V, mu, id + parity * half);
         mat u;
         mload(nc, l, &u);
         double *m = margins ? &margins[(long)mu * V + id + parity * half] : NULL;
         if (m) *m = INFINITY;
-        heatbath_mat(nc, bnc, &u, &F, &st, m);
+        heatbath_mat(nc, bnc, &u, &F, seed, (uint32_t)id, sweep, tag, m);
         mstore(nc, &u, l);
         project_link(nc, storage, l);
     }
@@ -1348,9 +1393,7 @@ static int multihit(int nc, const int X[4], double beta, uint64_t seed, int nhit
         double *m = margins ? &margins[3 * V + idd] : NULL;
         if (m) *m = INFINITY;
         for (int hit = 0; hit < nhit; hit++) {
-            usrc st;
-            usrc_stream(&st, seed, (uint32_t)id, (uint32_t)hit, RNG_TAG_MH | ((uint32_t)parity << 4) | 3u);
-            heatbath_mat(nc, bnc, &u, &F, &st, m);
+            heatbath_mat(nc, bnc, &u, &F, seed, (uint32_t)id, (uint32_t)hit, RNG_TAG_MH | ((uint32_t)parity << 4) | 3u, m);
             madd_to(nc, &acc, &u);
         }
         for (int i = 0; i < nc; i++)
@@ -1481,9 +1524,7 @@ static int multihit_ext(int nc, const int X[4], double beta, uint64_t seed, int 
                         madd_to(nc, &staple, &link);
                     }
                     mat F = mdag(nc, &staple);
-                    usrc st;
-                    usrc_stream(&st, seed, (uint32_t)idd, (uint32_t)hit, RNG_TAG_MHX | (uint32_t)b);
-                    heatbath_mat(nc, bnc, &box[b], &F, &st, m);
+                    heatbath_mat(nc, bnc, &box[b], &F, seed, (uint32_t)idd, (uint32_t)hit, RNG_TAG_MHX | (uint32_t)b, m);
                 }
             madd_to(nc, &acc, &box[13]);
         }

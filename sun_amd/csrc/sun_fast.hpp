@@ -517,7 +517,9 @@ SUN_HD void overrelax_su3(F3& U, const F3& S) {
     }
 }
 
-SUN_HD void heatbath_su3(F3& U, const F3& S, double bnc, Stream& st) {
+SUN_HD void heatbath_su3(F3& U, const F3& S, double bnc, const DrawKey& key) {
+    // the counters of every block's first attempt and final draws are known now
+    const BlockDraws bd[3] = {BlockDraws(key, 0), BlockDraws(key, 1), BlockDraws(key, 2)};
 #pragma unroll
     for (int block = 0; block < 3; block++) {
         const int p = block_p(3, block), q = block_q(3, block);
@@ -529,7 +531,7 @@ SUN_HD void heatbath_su3(F3& U, const F3& S, double bnc, Stream& st) {
 #pragma unroll
         for (int i = 0; i < 4; i++) r.a[i] *= k;
         Quat<float> a;
-        generate_su2_milc_f32(ap, st, a.a);
+        generate_su2_milc_f32(ap, key, block, bd[block], a.a);
         rotate_rows(U, mul_uv_dagger(a, r), p, q);
     }
 }

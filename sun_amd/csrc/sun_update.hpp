@@ -107,7 +107,7 @@ SUN_HD void mul_block(const Quat<R>& u, Mat<R, N>& link, int x, int y) {
 // one SU(2)-subgroup heatbath step of the N != 3 path on rows (x, y) of block B:
 // U <- r U, M <- r M with r = a (M-block)^+ normalised (device_PHB_OVR.h:430-482)
 template <typename R, int N, int B>
-SUN_HD void heatbath_block(Mat<R, N>& U, Mat<R, N>& M, double beta_over_nc, Stream& st) {
+SUN_HD void heatbath_block(Mat<R, N>& U, Mat<R, N>& M, double beta_over_nc, const DrawKey& key) {
     constexpr int x = block_p(N, B), y = block_q(N, B);
     Quat<R> r = get_block_su2<R, N>(M, x, y);
     R k = qabs(r);
@@ -116,12 +116,13 @@ SUN_HD void heatbath_block(Mat<R, N>& U, Mat<R, N>& M, double beta_over_nc, Stre
 #pragma unroll
     for (int i = 0; i < 4; i++) r.a[i] *= k;
     Quat<R> a;
+    const BlockDraws bd(key, B);
     if constexpr (std::is_same<R, float>::value) {
         // FP32 links: the FP32 sampler (as the SU(3) FP32 path)
-        generate_su2_milc_f32(ap, st, a.a);
+        generate_su2_milc_f32(ap, key, B, bd, a.a);
     } else {
         double ad[4];
-        generate_su2_milc((double)ap, st, ad);
+        generate_su2_milc((double)ap, key, B, bd, ad);
 #pragma unroll
         for (int i = 0; i < 4; i++) a.a[i] = R(ad[i]);
     }
@@ -131,13 +132,13 @@ SUN_HD void heatbath_block(Mat<R, N>& U, Mat<R, N>& M, double beta_over_nc, Stre
 }
 
 template <typename R, int N, int... B>
-SUN_HD void heatbath_blocks(Mat<R, N>& U, Mat<R, N>& M, double beta_over_nc, Stream& st,
+SUN_HD void heatbath_blocks(Mat<R, N>& U, Mat<R, N>& M, double beta_over_nc, const DrawKey& key,
                             std::integer_sequence<int, B...>) {
-    (heatbath_block<R, N, B>(U, M, beta_over_nc, st), ...);
+    (heatbath_block<R, N, B>(U, M, beta_over_nc, key), ...);
 }
 
 template <typename R, int N>
-SUN_HD void heatbath_link(Mat<R, N>& U, const Mat<R, N>& F, double beta_over_nc, Stream& st) {
+SUN_HD void heatbath_link(Mat<R, N>& U, const Mat<R, N>& F, double beta_over_nc, const DrawKey& key) {
     if constexpr (N == 3) {
 #pragma unroll
         for (int block = 0; block < 3; block++) {
@@ -149,7 +150,7 @@ SUN_HD void heatbath_link(Mat<R, N>& U, const Mat<R, N>& F, double beta_over_nc,
 #pragma unroll
             for (int i = 0; i < 4; i++) r.a[i] *= k;
             double ad[4];
-            generate_su2_milc((double)ap, st, ad);
+            generate_su2_milc((double)ap, key, block, BlockDraws(key, block), ad);
             Quat<R> a;
 #pragma unroll
             for (int i = 0; i < 4; i++) a.a[i] = R(ad[i]);
@@ -167,7 +168,7 @@ SUN_HD void heatbath_link(Mat<R, N>& U, const Mat<R, N>& F, double beta_over_nc,
         Mat<R, N> M = U * F;
         // one instance per block: the block's rows are compile-time constants (a
         // rolled loop would index U and M at run time, i.e. from scratch memory)
-        heatbath_blocks<R, N>(U, M, beta_over_nc, st, std::make_integer_sequence<int, N * (N - 1) / 2>{});
+        heatbath_blocks<R, N>(U, M, beta_over_nc, key, std::make_integer_sequence<int, N * (N - 1) / 2>{});
     }
 }
 
