@@ -219,6 +219,50 @@ int sun_wilson_loops(sun_lattice *lat, int rmax, int tmax, double *out, void *st
  * that overlap in time on one GPU (other streams, other threads) race. */
 int sun_wilson_loops_noalloc(sun_lattice *lat, int rmax, int tmax, double *dev_out, void *stream);
 
+/* ---- operator-basis Wilson loops (src/wl/, include/meas/wlop.h, wloopex.h) ------------------
+ * An operator field holds, for one spatial direction mu and radius r, nop gauge-covariant
+ * sums of paths from x to x + r mu at every site (18-real links in the lattice's
+ * precision): family SUN_WLOP_A0 (calcop_dg_A0.cu:46-135 with mop/mo0.cuh) has 10 slots,
+ * the straight line, MO0(l) for l = 1..8 (the four staples of height l over the line,
+ * / 2) and a zero slot; SUN_WLOP_33 (calcop_dg_33.cu:95-860) has 25, the line and three
+ * families of staple-shaped paths combined with the reference's phase tables (slots
+ * 1-8 invariant under a 90-degree rotation about mu, 9-16 times i, 17-24 times -1,
+ * except slot 18, whose table entries mix the rotation phases as the reference writes
+ * them).  FP64 fields equal the CPU restatement bit for bit.  The correlator adds, for
+ * every slot pair (i, j) and T = 0..tmax, sum_x Re Tr(O_i(x) t1 O_j(x + T e3)^+ t0^+)
+ * (t0, t1 the T temporal links from x and x + r mu) to a device accumulator
+ * acc[(i nop + j)(tmax + 1) + T], in a fixed summation order. */
+enum { SUN_WLOP_A0 = 0, SUN_WLOP_33 = 1 };
+typedef struct sun_wlop sun_wlop;
+/* Sigma_g_plus<Real> (wloopex.h:13-36; for SUN_WLOP_33 the fieldOp/wloop pair of
+ * wlop.h): operator field (nop V links) and accumulator (nop^2 (tmax+1) doubles, zero)
+ * on the lattice's device; tmax <= the temporal extent */
+int sun_wlop_create(sun_lattice *lat, int family, int tmax, sun_wlop **out);
+int sun_wlop_destroy(sun_wlop *h);
+int sun_wlop_nops(const sun_wlop *h);
+/* CalcWLOPs_A0 (calcop_dg_A0.cu:307-343) / CalcWLOPs_dg_33 (calcop_dg_33.cu:966-987):
+ * the operator field of direction mu (0..2) at radius r (1..X_mu), asynchronous */
+int sun_wlop_fields(sun_wlop *h, int r, int mu, void *stream);
+/* CalcWilsonLoop_A0 (wilsonloop_dg_A0.cu:57-105, 219-254) / CalcWilsonLoop_dg
+ * (wilsonloop_dg.cu:82-129, 227-266): adds the correlators of the current operator
+ * field (t1 at distance r along mu) to the accumulator, asynchronous */
+int sun_wlop_correlate(sun_wlop *h, int r, int mu, void *stream);
+/* cudaMemset(arg.wloop, 0) of test.cpp:280 */
+int sun_wlop_clear(sun_wlop *h, void *stream);
+/* the accumulator, nop^2 (tmax + 1) doubles (synchronous) */
+int sun_wlop_read(sun_wlop *h, double *out, void *stream);
+/* the accumulator in the component-table order of wilsonloop_dg.cu:101-122
+ * (DEV_Ops / DEV_OpComps / DEV_OpPos, which no reference file sets; here arguments):
+ * for component c, pairs (pos[c] + a, pos[c] + b), a, b < comps[c], a outer;
+ * out[id (tmax + 1) + T] with id counting those pairs */
+int sun_wlop_read_blocks(sun_wlop *h, int ncomp, const int *comps, const int *pos, double *out, void *stream);
+/* the operator field in the canonical FP64 order: out[((k V) + s) N^2 2], s the
+ * even/odd site (synchronous) */
+int sun_wlop_fields_to_host(sun_wlop *h, double *out);
+/* test.cpp:273-298: for r = 2..rmax, the accumulator after clearing it and adding
+ * mu = 0, 1, 2, divided by 3 N V: out[(((r - 2) nop^2 + i nop + j)(tmax + 1)) + T] */
+int sun_wilson_loops_ops(sun_lattice *lat, int family, int rmax, int tmax, double *out, void *stream);
+
 /* ---- host exchange (gaugearray.h:521-596 Get/Set, canonical FP64 layout) ---------------- */
 int sun_links_to_host(sun_lattice *lat, double *host, int layout);
 int sun_links_from_host(sun_lattice *lat, const double *host, int layout);

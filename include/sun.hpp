@@ -24,6 +24,10 @@
  *   PlaquetteCUB<Real>             include/meas/plaquette.h:120-176
  *   OnePolyakovLoop<Real>          include/meas/polyakovloop.h:15-73
  *   SaveBin_Gauge / ReadBin_Gauge  include/io_gauge.h, src/io_gauge.cpp:429-700
+ *   Sigma_g_plus, CalcWLOPs_A0,    include/meas/wloopex.h:13-36, src/wl/calcop_dg_A0.cu,
+ *   CalcWilsonLoop_A0              src/wl/wilsonloop_dg_A0.cu
+ *   CalcWLOPs_dg_33,               src/wl/calcop_dg_33.cu, src/wl/wilsonloop_dg.cu,
+ *   CalcWilsonLoop_dg              include/meas/wlop.h
  *
  * Differences, on purpose: errors throw sun::culqcd::Error (a std::runtime_error
  * carrying sun_last_error()) instead of exiting; SETPARAMS only records the
@@ -396,6 +400,94 @@ void WilsonLoop(_gauge<Real>& U, std::complex<double>* res, int radius, int Tmax
 template <class Real>
 void CWilsonLoop(_gauge<Real>& U, std::complex<double>* res, int radius, int Tmax) {
     WilsonLoop<Real>(U, res, radius, Tmax);
+}
+
+/* Sigma_g_plus<Real>(Rmax, Tmax)   wloopex.h:13-36: the A0 operator basis (opN = 10
+   operators, totalOpN = 100 pairs) with its correlator accumulator.  The device objects
+   are created on the first CalcWLOPs_A0 with the gauge field it is given.  The
+   reference's test clears and fetches the accumulator with raw cudaMemset / cudaMemcpy
+   on arg.wloop (test.cpp:280, :285); here Clear() and Fetch() (which fills wloop_h,
+   totalOpN (Tmax + 1) values, pair i opN + j at [it + pair (Tmax + 1)]). */
+template <class Real>
+class Sigma_g_plus {
+   public:
+    int Rmax, Tmax, opN, totalOpN;
+    std::vector<Real> wloop_h;
+    size_t wloop_size;
+    Sigma_g_plus(int Rmax_, int Tmax_) : Rmax(Rmax_), Tmax(Tmax_), opN(10), totalOpN(100) {
+        wloop_h.assign((size_t)totalOpN * (Tmax + 1), Real(0));
+        wloop_size = wloop_h.size() * sizeof(Real);
+    }
+    ~Sigma_g_plus() {
+        if (h_) sun_wlop_destroy(h_);
+    }
+    Sigma_g_plus(const Sigma_g_plus&) = delete;
+    Sigma_g_plus& operator=(const Sigma_g_plus&) = delete;
+    void Clear() {
+        if (h_) check(sun_wlop_clear(h_, nullptr), "Sigma_g_plus::Clear");
+    }
+    void Fetch() {
+        std::vector<double> v(wloop_h.size(), 0.0);
+        if (h_) check(sun_wlop_read(h_, v.data(), nullptr), "Sigma_g_plus::Fetch");
+        for (size_t i = 0; i < v.size(); i++) wloop_h[i] = (Real)v[i];
+    }
+    sun_wlop* attach(sun_lattice* lat) {
+        if (!h_) check(sun_wlop_create(lat, SUN_WLOP_A0, Tmax, &h_), "Sigma_g_plus");
+        return h_;
+    }
+
+   private:
+    sun_wlop* h_ = nullptr;
+};
+/* CalcWLOPs_A0(gauge, &arg, radius, mu)   calcop_dg_A0.cu:307-343 */
+template <class Real>
+void CalcWLOPs_A0(_gauge<Real>& U, Sigma_g_plus<Real>* arg, int radius, int mu) {
+    check(sun_wlop_fields(arg->attach(U.handle()), radius, mu, nullptr), "CalcWLOPs_A0");
+}
+/* CalcWilsonLoop_A0(gauge, &arg, radius, mu)   wilsonloop_dg_A0.cu:219-254 */
+template <class Real>
+void CalcWilsonLoop_A0(_gauge<Real>& U, Sigma_g_plus<Real>* arg, int radius, int mu) {
+    check(sun_wlop_correlate(arg->attach(U.handle()), radius, mu, nullptr), "CalcWilsonLoop_A0");
+}
+/* The fieldOp of CalcWLOPs_dg_33 (calcop_dg_33.cu:966-987): the 25 operators of the
+   33 family and the accumulator that CalcWilsonLoop_dg adds to. */
+template <class Real>
+class WLOpField33 {
+   public:
+    WLOpField33(_gauge<Real>& U, int Tmax) : Tmax(Tmax) {
+        check(sun_wlop_create(U.handle(), SUN_WLOP_33, Tmax, &h_), "WLOpField33");
+    }
+    ~WLOpField33() { sun_wlop_destroy(h_); }
+    WLOpField33(const WLOpField33&) = delete;
+    WLOpField33& operator=(const WLOpField33&) = delete;
+    sun_wlop* handle() const { return h_; }
+    void Clear() { check(sun_wlop_clear(h_, nullptr), "WLOpField33::Clear"); }
+    int Tmax;
+
+   private:
+    sun_wlop* h_ = nullptr;
+};
+template <class Real>
+void CalcWLOPs_dg_33(_gauge<Real>& U, WLOpField33<Real>& fieldOp, int radius, int mu) {
+    (void)U;
+    check(sun_wlop_fields(fieldOp.handle(), radius, mu, nullptr), "CalcWLOPs_dg_33");
+}
+/* CalcWilsonLoop_dg(gauge, fieldOp, wloop, radius, Tmax, mu, opN)   wilsonloop_dg.cu:227-266:
+   the reference reads its component table from __constant__ DEV_Ops / DEV_OpComps /
+   DEV_OpPos, which no reference file sets; here the table is an argument.  Adds this
+   (radius, mu) to the accumulator and returns the accumulated component-block pairs in
+   wloop[it + idOp (Tmax + 1)]. */
+template <class Real>
+void CalcWilsonLoop_dg(_gauge<Real>& U, WLOpField33<Real>& fieldOp, Real* wloop, int radius, int mu,
+                       const std::vector<int>& comps, const std::vector<int>& pos) {
+    (void)U;
+    check(sun_wlop_correlate(fieldOp.handle(), radius, mu, nullptr), "CalcWilsonLoop_dg");
+    size_t n = 0;
+    for (int c : comps) n += (size_t)c * c;
+    std::vector<double> v(n * (fieldOp.Tmax + 1));
+    check(sun_wlop_read_blocks(fieldOp.handle(), (int)comps.size(), comps.data(), pos.data(), v.data(), nullptr),
+          "CalcWilsonLoop_dg");
+    for (size_t i = 0; i < v.size(); i++) wloop[i] = (Real)v[i];
 }
 
 /* SaveBin_Gauge<Real, RealSaveConf>(gauge, filename, withheader) / ReadBin_Gauge:

@@ -82,6 +82,8 @@ def lib() -> ctypes.CDLL:
         "or_multihit_ext": (ci, [ci, ip, cd, u64, ci, dp, dp]),
         "or_hyp_smear": (ci, [ci, ip, dp, cd, cd, cd, ci, ci, cd, ci]),
         "or_wilson_loops": (ci, [ci, ip, dp, ci, ci, dp]),
+        "or_wlop_fields": (ci, [ci, ip, dp, ci, ci, ci, dp]),
+        "or_wlop_correlate": (ci, [ci, ip, dp, dp, ci, ci, ci, ci, dp]),
         "or_matmul": (None, [ci, dp, dp, dp]),
         "or_det": (None, [ci, dp, dp]),
         "or_num_threads": (ci, []),
@@ -383,6 +385,46 @@ def wilson_loops(U, dims, rmax: int, tmax: int) -> np.ndarray:
     out = np.zeros((rmax + 1) * (tmax + 1) * 2)
     _check(lib().or_wilson_loops(U.shape[-1], _dims(dims), _dp(U), rmax, tmax, _dp(out)), "wilson_loops")
     return (out[0::2] + 1j * out[1::2]).reshape(rmax + 1, tmax + 1)
+
+
+# ---- operator-basis Wilson loops (src/wl/*.cu) ------------------------------------
+WLOP_A0, WLOP_33 = 0, 1
+WLOP_NOPS = {WLOP_A0: 10, WLOP_33: 25}
+
+
+def wlop_fields(U, dims, family: int, r: int, mu: int) -> np.ndarray:
+    """operator fields [nop, V, N, N] of direction mu at radius r (calcop_dg_A0.cu,
+    calcop_dg_33.cu)"""
+    nc = U.shape[-1]
+    ops = np.zeros((WLOP_NOPS[family], volume(dims), nc, nc), dtype=np.complex128)
+    _check(lib().or_wlop_fields(nc, _dims(dims), _dp(U), family, r, mu, _dp(ops)), "wlop_fields")
+    return ops
+
+
+def wlop_correlate(U, dims, ops, r: int, mu: int, tmax: int, out=None) -> np.ndarray:
+    """adds sum_x Re Tr(O_i(x) t1 O_j(x + T e3)^+ t0^+) to out[i nop + j, T]
+    (wilsonloop_dg_A0.cu:57-105) and returns out"""
+    nop = ops.shape[0]
+    if out is None:
+        out = np.zeros((nop * nop, tmax + 1))
+    assert out.shape == (nop * nop, tmax + 1) and out.flags.c_contiguous
+    ops = np.ascontiguousarray(ops)
+    _check(lib().or_wlop_correlate(U.shape[-1], _dims(dims), _dp(U), _dp(ops), nop, r, mu, tmax, _dp(out)),
+           "wlop_correlate")
+    return out
+
+
+def wilson_loops_ops(U, dims, rmax: int, tmax: int, family: int = WLOP_A0) -> np.ndarray:
+    """test.cpp:273-298: for r = 2..rmax the sum over mu < 3 of the correlators of
+    every operator pair, / (3 N V); [rmax - 1, nop^2, tmax + 1]"""
+    nc = U.shape[-1]
+    nop = WLOP_NOPS[family]
+    res = np.zeros((max(rmax - 1, 0), nop * nop, tmax + 1))
+    for r in range(2, rmax + 1):
+        for mu in range(3):
+            wlop_correlate(U, dims, wlop_fields(U, dims, family, r, mu), r, mu, tmax, res[r - 2])
+        res[r - 2] /= volume(dims) * 3 * nc
+    return res
 
 
 def matmul(A, B):

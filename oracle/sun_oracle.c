@@ -1614,6 +1614,316 @@ int or_wilson_loops(int nc, const int X[4], const double *U, int rmax, int tmax,
     return 0;
 }
 
+/* ------------------------------------------------------------------------- */
+/* operator-basis Wilson loops: the operator fields of src/wl/calcop_dg_A0.cu   */
+/* (with include/mop/mo0.cuh) and src/wl/calcop_dg_33.cu, and the correlator of  */
+/* src/wl/wilsonloop_dg_A0.cu / wilsonloop_dg.cu                                */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int nc;
+    const int *X;
+    long V;
+    const double *U;
+} lctx;
+
+/* one step of a path at y: code = +(d+1) takes U_d(y) and moves to y + d; code =
+ * -(d+1) moves to y - d and takes U_d(y - d)^+ */
+static mat path_step(const lctx *c, int y[4], int code) {
+    mat a;
+    int d = (code > 0 ? code : -code) - 1;
+    if (code < 0) y[d] = (y[d] - 1 + c->X[d]) % c->X[d];
+    mload(c->nc, link_at(c->nc, c->U, c->V, d, or_coords_to_eo(y, c->X)), &a);
+    if (code > 0) {
+        y[d] = (y[d] + 1) % c->X[d];
+        return a;
+    }
+    return mdag(c->nc, &a);
+}
+
+/* ordered product of the steps st[0..n) from x, left to right; it starts from the
+ * identity when ident (mo0.cuh:22, calcop_dg_A0.cu:60), from the first link
+ * otherwise (calcop_dg_33.cu: link0 = GAUGE_LOAD(...) then *=) */
+static mat path_walk(const lctx *c, const int x[4], const int *st, int n, int ident) {
+    int y[4] = {x[0], x[1], x[2], x[3]};
+    mat m, a;
+    if (ident) munit(c->nc, &m);
+    for (int k = 0; k < n; k++) {
+        a = path_step(c, y, st[k]);
+        if (k == 0 && !ident)
+            m = a;
+        else
+            m = mmul(c->nc, &m, &a);
+    }
+    return m;
+}
+
+/* append `count` copies of `code` */
+static int path_seg(int *st, int n, int code, int count) {
+    for (int k = 0; k < count; k++) st[n++] = code;
+    return n;
+}
+
+/* acc += i^q L (q = 0..3), the reference's  mop += link / += timesI(link) / -= link
+ * / -= timesI(link)  (matrixsun.h:968-975: timesI(a) = (-Im a, Re a)) */
+static void acc_phase(int nc, mat *acc, const mat *L, int q) {
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) {
+            cx a = acc->e[i][j], l = L->e[i][j];
+            if (q == 0) a = cadd(a, l);
+            else if (q == 1) a = cmk(a.re + (-l.im), a.im + l.re);
+            else if (q == 2) a = csub(a, l);
+            else a = cmk(a.re - (-l.im), a.im - l.re);
+            acc->e[i][j] = a;
+        }
+}
+
+static void mat_div(int nc, mat *m, double s) {
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) m->e[i][j] = cmk(m->e[i][j].re / s, m->e[i][j].im / s);
+}
+static void mat_mulr(int nc, mat *m, double s) {
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) m->e[i][j] = cmk(m->e[i][j].re * s, m->e[i][j].im * s);
+}
+
+/* Phase tables of the 33 family: q[p][k] is the power of i with which path p enters
+ * output k (the first path of a family initialises every output).  Paths of family
+ * I and III: p = 4 part + 2 (down) + il; family II: p = 8 part + 2 type + il, type 0
+ * up-forward, 1 up-backward, 2 down-forward, 3 down-backward; il = 0 steps first
+ * along mu+1, il = 1 along mu+2 (mod 3).  calcop_dg_33.cu:150-260 (I), :319-672
+ * (II), :734-852 (III). */
+static const signed char WLOP_Q1[8][6] = {
+    {0, 0, 0, 0, 0, 0}, {0, 1, 2, 0, 1, 2}, {0, 2, 0, 0, 2, 0}, {0, 3, 2, 0, 3, 2},
+    {0, 0, 0, 2, 2, 2}, {0, 1, 2, 2, 3, 0}, {0, 2, 0, 2, 0, 2}, {0, 3, 2, 2, 1, 0}};
+static const signed char WLOP_Q2[16][12] = {
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, {0, 2, 0, 2, 1, 3, 1, 3, 2, 0, 2, 0},
+    {0, 2, 0, 2, 0, 2, 0, 2, 0, 2, 0, 2}, {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2},
+    {0, 2, 0, 2, 2, 0, 2, 0, 0, 2, 0, 2}, {0, 0, 0, 0, 3, 3, 3, 3, 2, 2, 2, 2},
+    {0, 0, 0, 0, 2, 2, 2, 2, 0, 0, 0, 0}, {0, 2, 0, 2, 3, 1, 3, 1, 2, 0, 2, 0},
+    {0, 0, 2, 2, 0, 0, 2, 2, 0, 0, 2, 2}, {0, 2, 2, 0, 1, 3, 3, 1, 0, 0, 0, 2},
+    {0, 2, 2, 0, 0, 2, 2, 0, 2, 2, 2, 0}, {0, 0, 2, 2, 1, 1, 3, 3, 2, 2, 0, 0},
+    {0, 2, 2, 0, 2, 0, 0, 2, 0, 2, 2, 0}, {0, 0, 2, 2, 3, 3, 1, 1, 2, 2, 0, 0},
+    {0, 0, 2, 2, 2, 2, 0, 0, 0, 0, 2, 2}, {0, 2, 2, 0, 3, 1, 1, 3, 2, 0, 0, 2}};
+static const signed char WLOP_Q3[8][6] = {
+    {0, 0, 0, 0, 0, 0}, {0, 0, 1, 1, 2, 2}, {0, 0, 2, 2, 0, 0}, {0, 0, 3, 3, 2, 2},
+    {0, 2, 0, 2, 0, 2}, {0, 2, 1, 3, 2, 0}, {0, 2, 2, 0, 0, 2}, {0, 2, 3, 1, 2, 0}};
+/* output k of each family -> operator slot (calcop_dg_33.cu:263-268, :680-690, :853-858) */
+static const int WLOP_SLOT1[6] = {1, 9, 17, 5, 13, 21};
+static const int WLOP_SLOT2[12] = {2, 4, 6, 8, 10, 11, 14, 15, 18, 19, 22, 23};
+static const int WLOP_SLOT3[6] = {3, 7, 12, 16, 20, 24};
+
+static void wlop_accumulate(int nc, mat *acc, int nout, const signed char *q, int p, const mat *L) {
+    for (int k = 0; k < nout; k++) {
+        if (p == 0)
+            acc[k] = *L;
+        else
+            acc_phase(nc, &acc[k], L, q[k]);
+    }
+}
+
+/* the 25 operators of calcop_dg_33.cu:95-860 at site x (radius r >= 1, direction mu) */
+static void wlop33_site(const lctx *c, const int x[4], int r, int mu, mat *out) {
+    const int nc = c->nc, M = mu + 1;
+    const int h = r / 2, h2 = (r + 1) / 2;
+    const int dd[2] = {(mu + 1) % 3, (mu + 2) % 3};
+    int xh[4] = {x[0], x[1], x[2], x[3]}, xs2[4] = {x[0], x[1], x[2], x[3]};
+    xh[mu] = (xh[mu] + h) % c->X[mu];
+    xs2[mu] = (xs2[mu] + h2) % c->X[mu];
+    int st[256];
+    int n;
+    mat acc[12], link0, link, a;
+    const double rt8 = 2.0 * sqrt(2.0);
+
+    /* slot 0: the straight line from the identity (calcop_dg_33.cu:103-113) */
+    n = path_seg(st, 0, M, r);
+    out[0] = path_walk(c, x, st, n, 1);
+
+    /* lines: line1 = links h..r-1 along mu (from x + h mu), line2 = links 0..h2-1 */
+    n = path_seg(st, 0, M, r - h);
+    const mat line1 = path_walk(c, xh, st, n, 0);
+    n = path_seg(st, 0, M, h2);
+    const mat line2 = path_walk(c, x, st, n, 0);
+
+    /* family I: a unit staple over the first (last) half of the line */
+    for (int part = 0; part < 2; part++)
+        for (int down = 0; down < 2; down++)
+            for (int il = 0; il < 2; il++) {
+                const int D = dd[il] + 1, s = down ? -1 : 1;
+                n = path_seg(st, 0, s * D, 1);
+                n = path_seg(st, n, M, part ? r - h2 : h);
+                n = path_seg(st, n, -s * D, 1);
+                link0 = path_walk(c, part ? xs2 : x, st, n, 0);
+                link = part ? mmul(nc, &line2, &link0) : mmul(nc, &link0, &line1);
+                int p = 4 * part + 2 * down + il;
+                wlop_accumulate(nc, acc, 6, WLOP_Q1[p], p, &link);
+            }
+    for (int k = 0; k < 6; k++) {
+        mat_div(nc, &acc[k], rt8);
+        out[WLOP_SLOT1[k]] = acc[k];
+    }
+
+    /* family II: the staple's top leg displaced one step along the other transverse
+     * direction (forward or backward) */
+    for (int part = 0; part < 2; part++)
+        for (int type = 0; type < 4; type++)
+            for (int il = 0; il < 2; il++) {
+                const int D = dd[il] + 1, E = dd[1 - il] + 1;
+                const int sd = (type < 2) ? 1 : -1, se = (type & 1) ? -1 : 1;
+                n = path_seg(st, 0, sd * D, 1);
+                n = path_seg(st, n, se * E, 1);
+                n = path_seg(st, n, M, part ? r - h2 : h);
+                n = path_seg(st, n, -se * E, 1);
+                n = path_seg(st, n, -sd * D, 1);
+                link0 = path_walk(c, part ? xs2 : x, st, n, 0);
+                link = part ? mmul(nc, &line2, &link0) : mmul(nc, &link0, &line1);
+                int p = 8 * part + 2 * type + il;
+                wlop_accumulate(nc, acc, 12, WLOP_Q2[p], p, &link);
+            }
+    for (int k = 0; k < 12; k++) {
+        mat_mulr(nc, &acc[k], 0.25);
+        out[WLOP_SLOT2[k]] = acc[k];
+    }
+
+    /* family III: the staple run backwards and forwards over half the line */
+    n = path_seg(st, 0, M, h > 0 ? h : 1);
+    const mat line0a = path_walk(c, x, st, n, 0);
+    n = path_seg(st, 0, M, r - h2 > 0 ? r - h2 : 1);
+    const mat line0b = path_walk(c, xs2, st, n, 0);
+    for (int part = 0; part < 2; part++)
+        for (int down = 0; down < 2; down++)
+            for (int il = 0; il < 2; il++) {
+                const int D = dd[il] + 1, s = down ? -1 : 1;
+                n = path_seg(st, 0, s * D, 1);
+                n = path_seg(st, n, M, part ? r - h2 : h);
+                n = path_seg(st, n, -s * D, 1);
+                link0 = path_walk(c, part ? xs2 : x, st, n, 0);
+                mat l0d = mdag(nc, &link0);
+                if (!part) {
+                    a = mmul(nc, &line0a, &l0d);
+                    a = mmul(nc, &a, &line0a);
+                    link = mmul(nc, &a, &line1);
+                } else {
+                    a = mmul(nc, &line2, &line0b);
+                    a = mmul(nc, &a, &l0d);
+                    link = mmul(nc, &a, &line0b);
+                }
+                int p = 4 * part + 2 * down + il;
+                wlop_accumulate(nc, acc, 6, WLOP_Q3[p], p, &link);
+            }
+    for (int k = 0; k < 6; k++) {
+        mat_div(nc, &acc[k], rt8);
+        out[WLOP_SLOT3[k]] = acc[k];
+    }
+}
+
+/* the 10 operators of calcop_dg_A0.cu:46-135 at site x: slot 0 the straight line,
+ * slots 1..8 MO0(l = 1..8) (mo0.cuh:12-60: the four staples of height l over the line
+ * in the two transverse spatial directions, / 2), slot 9 zero (the field is cleared
+ * before the kernel, calcop_dg_A0.cu:241, and nothing writes that slot) */
+static void wlopA0_site(const lctx *c, const int x[4], int r, int mu, mat *out) {
+    const int nc = c->nc, M = mu + 1;
+    const int dd[2] = {(mu + 1) % 3, (mu + 2) % 3};
+    int st[256];
+    int n = path_seg(st, 0, M, r);
+    out[0] = path_walk(c, x, st, n, 1);
+    for (int l = 1; l <= 8; l++) {
+        mat acc, link0;
+        mzero(nc, &acc);
+        for (int down = 0; down < 2; down++)
+            for (int il = 0; il < 2; il++) {
+                const int D = dd[il] + 1, s = down ? -1 : 1;
+                n = path_seg(st, 0, s * D, l);
+                n = path_seg(st, n, M, r);
+                n = path_seg(st, n, -s * D, l);
+                link0 = path_walk(c, x, st, n, 1);
+                madd_to(nc, &acc, &link0);
+            }
+        mat_div(nc, &acc, 2.0);
+        out[l] = acc;
+    }
+    mzero(nc, &out[9]);
+}
+
+int or_wlop_fields(int nc, const int X[4], const double *U, int family, int r, int mu, double *ops) {
+    int rc = check_args(nc, X, OR_STORE_18);
+    if (rc) return rc;
+    if (family != OR_WLOP_A0 && family != OR_WLOP_33) return -3;
+    if (mu < 0 || mu > 2 || r < 1 || r > X[mu] || r > 64) return -2;
+    const long V = volume_of(X);
+    const int nop = family == OR_WLOP_A0 ? 10 : 25;
+    const lctx c = {nc, X, V, U};
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < V; s++) {
+        int parity = s >= V / 2 ? 1 : 0;
+        int x[4];
+        mat out[25];
+        or_eo_to_coords((int)(s - parity * (V / 2)), parity, X, x);
+        if (family == OR_WLOP_A0)
+            wlopA0_site(&c, x, r, mu, out);
+        else
+            wlop33_site(&c, x, r, mu, out);
+        for (int k = 0; k < nop; k++) mstore(nc, &out[k], ops + ((size_t)k * V + s) * nc * nc * 2);
+    }
+    return 0;
+}
+
+int or_wlop_correlate(int nc, const int X[4], const double *U, const double *ops, int nop, int r, int mu, int tmax,
+                      double *out) {
+    int rc = check_args(nc, X, OR_STORE_18);
+    if (rc) return rc;
+    if (nop < 1 || mu < 0 || mu > 2 || r < 0 || r > X[mu] || tmax < 0 || tmax > X[3]) return -2;
+    const long V = volume_of(X);
+    const int npair = nop * nop;
+    const long nchunk = (V + CHUNK - 1) / CHUNK;
+    double *part = (double *)calloc((size_t)nchunk * npair * (tmax + 1), sizeof(double));
+    if (!part) return -4;
+#pragma omp parallel for schedule(static)
+    for (long ch = 0; ch < nchunk; ch++) {
+        double *p = part + (size_t)ch * npair * (tmax + 1);
+        long end = (ch + 1) * CHUNK < V ? (ch + 1) * CHUNK : V;
+        for (long s = ch * CHUNK; s < end; s++) {
+            int parity = s >= V / 2 ? 1 : 0;
+            int x[4];
+            or_eo_to_coords((int)(s - parity * (V / 2)), parity, X, x);
+            int xr[4] = {x[0], x[1], x[2], x[3]};
+            xr[mu] = (xr[mu] + r) % X[mu];
+            mat t0, t1, lb, lt, m, a;
+            munit(nc, &t0);
+            munit(nc, &t1);
+            for (int T = 0; T <= tmax; T++) {
+                int y[4] = {x[0], x[1], x[2], (x[3] + T) % X[3]};
+                const long top = or_coords_to_eo(y, X);
+                for (int i = 0; i < nop; i++) {
+                    mload(nc, ops + ((size_t)i * V + s) * nc * nc * 2, &lb);
+                    for (int j = 0; j < nop; j++) {
+                        mload(nc, ops + ((size_t)j * V + top) * nc * nc * 2, &a);
+                        lt = mdag(nc, &a);
+                        m = mmul(nc, &lb, &t1);
+                        m = mmul(nc, &m, &lt);
+                        a = mdag(nc, &t0);
+                        m = mmul(nc, &m, &a);
+                        p[(size_t)(i * nop + j) * (tmax + 1) + T] += mrealtrace(nc, &m);
+                    }
+                }
+                if (T < tmax) {
+                    mload(nc, link_at(nc, U, V, 3, top), &a);
+                    t0 = mmul(nc, &t0, &a);
+                    int z[4] = {xr[0], xr[1], xr[2], y[3]};
+                    mload(nc, link_at(nc, U, V, 3, or_coords_to_eo(z, X)), &a);
+                    t1 = mmul(nc, &t1, &a);
+                }
+            }
+        }
+    }
+    for (long k = 0; k < (long)npair * (tmax + 1); k++) {
+        double acc = 0.0;
+        for (long ch = 0; ch < nchunk; ch++) acc += part[(size_t)ch * npair * (tmax + 1) + k];
+        out[k] += acc;
+    }
+    free(part);
+    return 0;
+}
+
 void or_matmul(int nc, const double *A, const double *B, double *C) {
     mat a, b, c;
     mload(nc, A, &a);

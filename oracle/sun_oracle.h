@@ -137,6 +137,19 @@ int or_multihit_ext_margins(int nc, const int dims[4], double beta, uint64_t see
  * T <= tmax, averaged over sites and the three spatial directions, / (3 N V) */
 int or_wilson_loops(int nc, const int dims[4], const double *U, int rmax, int tmax, double *out);
 
+/* ---- operator-basis Wilson loops (src/wl/calcop_dg_A0.cu, src/wl/calcop_dg_33.cu,
+ * include/mop/mo0.cuh, src/wl/wilsonloop_dg_A0.cu, src/wl/wilsonloop_dg.cu) ---- */
+enum { OR_WLOP_A0 = 0, OR_WLOP_33 = 1 };
+/* the operator fields of spatial direction mu (0..2) at radius r >= 1:
+ * ops[((slot V) + s) N^2 2], s the even/odd site; family OR_WLOP_A0 has 10 slots
+ * (calcop_dg_A0.cu:46-135), OR_WLOP_33 25 (calcop_dg_33.cu:95-860) */
+int or_wlop_fields(int nc, const int dims[4], const double *U, int family, int r, int mu, double *ops);
+/* wilsonloop_dg_A0.cu:57-105 over every slot pair: out[(i nop + j)(tmax + 1) + T] +=
+ * sum over sites x of Re Tr(O_i(x) t1 O_j(x + T e3)^+ t0^+), t0 (t1) the product of T
+ * temporal links from x (x + r mu) */
+int or_wlop_correlate(int nc, const int dims[4], const double *U, const double *ops, int nop, int r, int mu,
+                      int tmax, double *out);
+
 /* ---- helpers ---------------------------------------------------------------- */
 void or_matmul(int nc, const double *A, const double *B, double *C);     /* C = A*B */
 void or_det(int nc, const double *A, double out[2]);

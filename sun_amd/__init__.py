@@ -47,6 +47,8 @@ EXPORTS = [
     "sun_link_monitor", "sun_link_monitor_async", "sun_link_trace", "sun_link_det",
     "sun_ape_smear", "sun_stout_smear", "sun_hyp_smear", "sun_multihit", "sun_multihit_space", "sun_multihit_ext",
     "sun_wilson_loops", "sun_wilson_loops_noalloc",
+    "sun_wlop_create", "sun_wlop_destroy", "sun_wlop_nops", "sun_wlop_fields", "sun_wlop_correlate",
+    "sun_wlop_clear", "sun_wlop_read", "sun_wlop_read_blocks", "sun_wlop_fields_to_host", "sun_wilson_loops_ops",
     "sun_links_to_host", "sun_links_from_host",
     "sun_generate",
     "sun_batch_create", "sun_batch_destroy", "sun_batch_get_launch", "sun_batch_set_launch",
@@ -165,6 +167,16 @@ def lib() -> ctypes.CDLL:
         "sun_multihit_ext": (ci, [vp, vp, u64, ci, vp]),
         "sun_wilson_loops": (ci, [vp, ci, ci, dp, vp]),
         "sun_wilson_loops_noalloc": (ci, [vp, ci, ci, vp, vp]),
+        "sun_wlop_create": (ci, [vp, ci, ci, ctypes.POINTER(vp)]),
+        "sun_wlop_destroy": (ci, [vp]),
+        "sun_wlop_nops": (ci, [vp]),
+        "sun_wlop_fields": (ci, [vp, ci, ci, vp]),
+        "sun_wlop_correlate": (ci, [vp, ci, ci, vp]),
+        "sun_wlop_clear": (ci, [vp, vp]),
+        "sun_wlop_read": (ci, [vp, dp, vp]),
+        "sun_wlop_read_blocks": (ci, [vp, ci, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), dp, vp]),
+        "sun_wlop_fields_to_host": (ci, [vp, dp]),
+        "sun_wilson_loops_ops": (ci, [vp, ci, ci, ci, dp, vp]),
         "sun_links_to_host": (ci, [vp, dp, ci]),
         "sun_links_from_host": (ci, [vp, dp, ci]),
         "sun_generate": (ci, [vp, ctypes.POINTER(GenerateDesc), vp, dp]),
@@ -399,6 +411,14 @@ class Lattice:
         _check(lib().sun_wilson_loops_noalloc(self.h, rmax, tmax, ctypes.c_void_p(dev_out), stream),
                "sun_wilson_loops_noalloc")
 
+    def wilson_loops_ops(self, rmax: int, tmax: int, family: int = 0, stream=None) -> np.ndarray:
+        """[rmax - 1, nop^2, tmax + 1]: for r = 2..rmax the operator-basis correlators of
+        every slot pair summed over the spatial directions, / (3 N V) (test.cpp:273-298)"""
+        nop = WLOP_NOPS[family]
+        out = np.zeros((max(rmax - 1, 0), nop * nop, tmax + 1))
+        _check(lib().sun_wilson_loops_ops(self.h, family, rmax, tmax, _dp(out), stream), "sun_wilson_loops_ops")
+        return out
+
     # -- host exchange (canonical layout, complex128 [4, V, N, N])
     def to_host(self) -> np.ndarray:
         U = np.zeros((4, self.V, self.nc, self.nc), dtype=np.complex128)
@@ -418,6 +438,68 @@ class Lattice:
         _check(lib().sun_generate(self.h, ctypes.byref(d), stream, _dp(obs) if measure else None),
                "sun_generate")
         return obs[:n_iter]
+
+
+WLOP_A0, WLOP_33 = 0, 1
+WLOP_NOPS = {WLOP_A0: 10, WLOP_33: 25}
+
+
+class WilsonLoopOps:
+    """Sigma_g_plus (wloopex.h:13-36) / the fieldOp + wloop pair of wlop.h: an
+    operator field of one family on the lattice's device and an FP64 accumulator of
+    the correlators of every slot pair."""
+
+    def __init__(self, lat: Lattice, family: int, tmax: int):
+        self.lat, self.family, self.tmax = lat, family, int(tmax)
+        self.h = ctypes.c_void_p()
+        _check(lib().sun_wlop_create(lat.h, family, tmax, ctypes.byref(self.h)), "sun_wlop_create")
+        self.nop = lib().sun_wlop_nops(self.h)
+
+    def close(self):
+        if self.h:
+            _check(lib().sun_wlop_destroy(self.h), "sun_wlop_destroy")
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def fields(self, r: int, mu: int, stream=None):
+        """CalcWLOPs_A0 / CalcWLOPs_dg_33: the operator field of direction mu at radius r"""
+        _check(lib().sun_wlop_fields(self.h, r, mu, stream), "sun_wlop_fields")
+
+    def correlate(self, r: int, mu: int, stream=None):
+        """CalcWilsonLoop_A0 / CalcWilsonLoop_dg: add every slot pair's correlators"""
+        _check(lib().sun_wlop_correlate(self.h, r, mu, stream), "sun_wlop_correlate")
+
+    def clear(self, stream=None):
+        _check(lib().sun_wlop_clear(self.h, stream), "sun_wlop_clear")
+
+    def read(self, stream=None) -> np.ndarray:
+        """the accumulator [nop^2, tmax + 1], row i nop + j"""
+        out = np.zeros((self.nop * self.nop, self.tmax + 1))
+        _check(lib().sun_wlop_read(self.h, _dp(out), stream), "sun_wlop_read")
+        return out
+
+    def read_blocks(self, comps, pos, stream=None) -> np.ndarray:
+        """the pairs of each component block (wilsonloop_dg.cu:101-122 order)"""
+        comps = [int(c) for c in comps]
+        pos = [int(p) for p in pos]
+        n = sum(c * c for c in comps)
+        out = np.zeros((n, self.tmax + 1))
+        ca = (ctypes.c_int * max(len(comps), 1))(*comps)
+        pa = (ctypes.c_int * max(len(pos), 1))(*pos)
+        _check(lib().sun_wlop_read_blocks(self.h, len(comps), ca, pa, _dp(out), stream), "sun_wlop_read_blocks")
+        return out
+
+    def fields_to_host(self) -> np.ndarray:
+        """complex128 [nop, V, N, N] in the canonical (even/odd) site order"""
+        nc = self.lat.nc
+        out = np.zeros((self.nop, self.lat.V, nc, nc), dtype=np.complex128)
+        _check(lib().sun_wlop_fields_to_host(self.h, _dp(out)), "sun_wlop_fields_to_host")
+        return out
 
 
 class Batch:

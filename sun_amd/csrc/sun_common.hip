@@ -107,6 +107,14 @@ __global__ void __launch_bounds__(kReduceBlock) k_wl_finalize(const double2* par
     if (threadIdx.x == 0) out[T] = make_double2(acc.x * scale, acc.y * scale);
 }
 
+__global__ void __launch_bounds__(kReduceBlock) k_wlop_finalize(const double* partials, int nb, double* accum) {
+    const double* p = partials + (size_t)blockIdx.x * nb;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kReduceBlock) acc += p[i];
+    const double2 v = block_sum_all(make_double2(acc, 0.0));
+    if (threadIdx.x == 0) accum[blockIdx.x] += v.x;
+}
+
 __global__ void __launch_bounds__(kReduceBlock) k_batch_finalize(const BatchEntry* table, double2* out) {
     const BatchEntry& E = table[blockIdx.x];
     double2 v = ordered_sum(E.partials, E.nblocks);
@@ -177,6 +185,12 @@ hipError_t launch_finalize_iteration(const double2* partials, int pb, int qb, do
 
 hipError_t launch_wl_finalize(const double2* partials, int nb, int tmax, double scale, double2* out, hipStream_t s) {
     hipLaunchKernelGGL(k_wl_finalize, dim3(tmax + 1), dim3(kReduceBlock), 0, s, partials, nb, tmax, scale, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_wlop_finalize(const double* partials, int nb, int nout, double* accum, hipStream_t s) {
+    if (nout <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_wlop_finalize, dim3(nout), dim3(kReduceBlock), 0, s, partials, nb, accum);
     return hipGetLastError();
 }
 

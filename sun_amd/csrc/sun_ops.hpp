@@ -165,6 +165,14 @@ struct OpsTable {
     hipError_t (*wl_lines)(const Field& f, const Field& lines, const Geom&, int r, hipStream_t);
     hipError_t (*wl_loops)(const Field& f, const Field& lines, const Geom&, int r, int tmax, double2* partials,
                            hipStream_t);
+    // operator-basis Wilson loops: the operator fields of direction mu at radius r
+    // (family 0: 10 slots, 1: 25; V 18-real entries per slot, slot k at k * stride
+    // bytes after ops0 in every plane; rt8 = 2 sqrt 2), and the correlator of every
+    // slot pair, nop^2 (tmax + 1) * monitor_blocks(geom) FP64 partials
+    hipError_t (*wlop_fields)(const Field& f, const Field& ops0, size_t stride, const Geom&, int family, int r,
+                              int mu, double rt8, hipStream_t);
+    hipError_t (*wlop_corr)(const Field& f, const Field& ops0, size_t stride, const Geom&, int nop, int r, int mu,
+                            int tmax, double* partials, hipStream_t);
     int max_block;   // largest workgroup the update kernels are compiled for
     int has_piped;   // the variant has the software-pipelined overrelaxation kernel
     int piped_max_block;
@@ -237,6 +245,8 @@ hipError_t launch_finalize_iteration(const double2* partials, int pb, int qb, do
 // Wilson loops of one radius: out[T] = scale * sum over mu < 3 and the nb partials
 // of (mu, T), T = 0..tmax, in a fixed order
 hipError_t launch_wl_finalize(const double2* partials, int nb, int tmax, double scale, double2* out, hipStream_t s);
+// accum[o] += the sum of partials[o nb .. o nb + nb) in a fixed order, o < nout
+hipError_t launch_wlop_finalize(const double* partials, int nb, int nout, double* accum, hipStream_t s);
 // link monitor: out[0] = trace sum * scale_tr, out[1] = determinant sum * scale_det,
 // out[2] = (max deviation, max |det - 1|) from the three segments of n partials
 hipError_t launch_monitor_finalize(const double2* partials, int n, double scale_tr, double scale_det, double2* out,

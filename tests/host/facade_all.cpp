@@ -41,6 +41,16 @@ void use_all(_gauge<Real>& U, _gauge<Real>& out, RNG& rng) {
     std::complex<double> res[4];
     WilsonLoop<Real>(out, res, 1, 1);
     CWilsonLoop<Real>(out, res, 1, 1);
+    Sigma_g_plus<Real> arg(2, 1);
+    arg.Clear();
+    CalcWLOPs_A0<Real>(out, &arg, 2, 0);
+    CalcWilsonLoop_A0<Real>(out, &arg, 2, 0);
+    arg.Fetch();
+    WLOpField33<Real> ops(out, 1);
+    ops.Clear();
+    CalcWLOPs_dg_33<Real>(out, ops, 2, 1);
+    std::vector<Real> wl(2 * 2 * 2);
+    CalcWilsonLoop_dg<Real>(out, ops, wl.data(), 2, 1, {1, 1}, {0, 4});
     SaveBin_Gauge<float>(U, "x.bin", true);
     ReadBin_Gauge<double>(U, "x.bin", false);
 }
