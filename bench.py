@@ -15,6 +15,8 @@ Prints ONE JSON line.  Extra objects:
   plain_fp64    the plain single-lattice path (FP64 18-real, eager launches)
   precision     FP32-12 vs FP64-18: lockstep per-iteration plaquette / Polyakov
                 differences and free-chain ensemble means
+  operator_loops  operator-basis Wilson loops (A0 measurement at 32^4 FP64, 33
+                family at r = 12) with the correlator launch's roofline
 """
 from __future__ import annotations
 
@@ -489,6 +491,60 @@ def measurement_leg(sun_amd, ctx, lat, stream):
     }
 
 
+def operator_loops_leg(sun_amd, ctx, stream, calibrate=False):
+    """Operator-basis Wilson loops on a 32^4 SU(3) FP64 18-real field (the
+    reference's A0 measurement, test.cpp:255-298: Sigma_g_plus(12, 16), r = 2..12,
+    the three spatial directions; the 33 family's operator field and correlator at
+    r = 12).  Roofline of the correlator launch: algorithmic bytes per launch = V x
+    144 B x (nop bottom operators + nop (T + 1) top operators + 2 T temporal links)
+    plus its FP64 partials, over the launch time measured with HIP events on its
+    stream.  calibrate: two plain copies of the 604 MB link field first (the size
+    reference for counter passes)."""
+    V = int(np.prod(DIMS))
+    lb, T = 144, 16
+    out = {"lattice": list(DIMS), "precision": "f64", "storage": 18, "rmax": 12, "tmax": T}
+    with sun_amd.Lattice(ctx, 3, DIMS, BETA, 18, 8) as lat:
+        lat.hot_start(21, stream)
+        for it in range(2):
+            lat.heatbath(22, it, stream)
+            lat.overrelax(stream)
+        if calibrate:
+            with sun_amd.Lattice(ctx, 3, DIMS, BETA, 18, 8) as tmp:
+                for _ in range(2):
+                    tmp.copy_from(lat, stream)
+                sun_amd.sync(stream)
+        t0 = time.perf_counter()
+        W = lat.wilson_loops_ops(12, T, sun_amd.WLOP_A0, stream)
+        out["a0_measurement_s"] = time.perf_counter() - t0
+        out["a0_W00_r4"] = [float(W[2, 0, t]) for t in range(0, 9)]
+        for fam, name in ((sun_amd.WLOP_A0, "a0"), (sun_amd.WLOP_33, "f33")):
+            with sun_amd.WilsonLoopOps(lat, fam, T) as ops:
+                nop = ops.nop
+                tm = sun_amd.Timer()
+                ops.fields(12, 0, stream)
+                ops.correlate(12, 0, stream)
+                sun_amd.sync(stream)
+                tm.start(stream)
+                for _ in range(5):
+                    ops.fields(12, 0, stream)
+                tm.stop(stream)
+                f_ms = tm.ms() / 5
+                tm.start(stream)
+                for _ in range(5):
+                    ops.correlate(12, 0, stream)
+                tm.stop(stream)
+                c_ms = tm.ms() / 5
+                tm.close()
+                nb = (V + 255) // 256
+                alg = V * lb * (nop + nop * (T + 1) + 2 * T) + nop * nop * (T + 1) * nb * 8
+                gbs = alg / (c_ms * 1e-3) / 1e9
+                out[name] = {"nop": nop, "fields_ms_r12": f_ms, "correlate_ms_r12": c_ms,
+                             "correlate_roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS,
+                                                    "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                                                    "alg_bytes_per_launch": alg}}
+    return out
+
+
 def kernels_only(sun_amd, large=False, shapes=None):
     """For rocprofv3 counter passes.  Default: on the 32^4 FP32 12-real field after
     100 iterations, 2 copies of the field (calibration: a plain copy of known size),
@@ -542,6 +598,8 @@ def main():
     ap.add_argument("--no-meas", action="store_true", help="skip the smearing / Wilson loop leg")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served leg")
     ap.add_argument("--large", action="store_true", help="with --kernels-only: the 64^4 FP32 8-real field")
+    ap.add_argument("--ops-only", action="store_true",
+                    help="only the operator-basis Wilson loop leg (with copy calibration, for rocprofv3)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="fixed short launch sequence for profiler counter passes (no JSON line)")
     ap.add_argument("--autotune", action="store_true",
@@ -556,6 +614,11 @@ def main():
     import sun_amd
     shapes32 = None if args.autotune else profiled_shapes(PROFILE_32)
     shapes64 = None if args.autotune else profiled_shapes(PROFILE_64)
+    if args.ops_only:
+        ctx = sun_amd.Context(0, 0, 0)
+        print(json.dumps(operator_loops_leg(sun_amd, ctx, sun_amd.new_stream(), calibrate=True)))
+        ctx.close()
+        return
     if args.kernels_only:
         kernels_only(sun_amd, large=args.large, shapes=shapes64 if args.large else shapes32)
         return
@@ -623,6 +686,8 @@ def main():
     if not args.no_meas:
         out["measurement_chain"] = measurement_leg(sun_amd, ctx, lat, stream)
     lat.close()
+    if not args.no_meas:
+        out["operator_loops"] = operator_loops_leg(sun_amd, ctx, stream)
 
     if not args.no_plain:
         out["plain_fp64"] = plain_fp64_leg(sun_amd, ctx, stream, value, updates_per_step)

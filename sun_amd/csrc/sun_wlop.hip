@@ -67,7 +67,7 @@ extern "C" int sun_wlop_create(sun_lattice* lat, int family, int tmax, sun_wlop*
     h->tmax = tmax;
     h->lb = (size_t)2 * lat->d.nc * lat->d.nc * (size_t)lat->d.precision;
     h->stride = (size_t)g.V * h->lb;
-    h->nb = monitor_blocks(g);
+    h->nb = monitor_blocks(g);  // the correlator's 256-site blocks
     const size_t nout = (size_t)nop * nop * (tmax + 1);
     HIP_TRY_CLEANUP(hipMalloc(&h->ops, (size_t)nop * h->stride), release(h));
     HIP_TRY_CLEANUP(hipMalloc((void**)&h->partials, nout * h->nb * sizeof(double)), release(h));
