@@ -390,26 +390,34 @@ def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
                 cs.wait(jid, 120.0)
                 st = cs.status(jid)
                 calib[(nc, dims)] = 20 * 5 * 4 * int(np.prod(dims)) / max(st.t_end - st.t_start, 1e-6)
-        # steady state of the served engine on the headline lattice: one SU(3) 32^4
-        # FP32-12 job of 220 iterations alone (the 20-iteration jobs above carry the
-        # job's set-up -- table upload, graph capture, start kernels -- in their time)
-        n_steady = 220
-        jid = cs.submit(3, (32, 32, 32, 32), 6.0, 2, n_steady, n_or=4, reunit_every=1, storage=12, precision=4,
-                        hot_start=False)
-        cs.wait(jid, 120.0)
-        st = cs.status(jid)
-        steady_s = max(st.t_end - st.t_start, 1e-6)
+        # steady state of the served engine on the headline lattice: SU(3) 32^4 FP32-12
+        # jobs of 40 and 260 iterations, each alone; the difference of their times is
+        # the cost of 220 iterations without the job's set-up (table upload, graph
+        # capture, start kernels) and retirement, which every job pays once
+        times = {}
+        for n_it in (40, 260, 40, 260):
+            jid = cs.submit(3, (32, 32, 32, 32), 6.0, 2, n_it, n_or=4, reunit_every=1, storage=12, precision=4,
+                            hot_start=False)
+            cs.wait(jid, 120.0)
+            st = cs.status(jid)
+            times.setdefault(n_it, []).append(max(st.t_end - st.t_start, 1e-6))
         cs.stop()
     V32 = 32 ** 4
+    n_steady = 220
+    steady_s = max(min(times[260]) - min(times[40]), 1e-6)
+    # HB + 4 OR (9 link transfers per update), reunitarisation (2 per link), plaquette
+    # (1) and Polyakov loop (1/4): 48.25 x 4V links of 48 B per iteration
+    it_bytes = 4 * V32 * 48 * 48.25
     steady = {"iterations": n_steady, "s": steady_s,
               "link_updates_per_s": n_steady * 5 * 4 * V32 / steady_s,
-              # HB + 4 OR (9 link transfers per update), reunitarisation (2 per link),
-              # plaquette (1) and Polyakov loop (1/4): 48.25 x 4V links of 48 B per iteration
-              "iteration_frac": n_steady * 4 * V32 * 48 * 48.25 / steady_s / (PEAK_HBM_GBS * 1e9),
-              "note": "one SU(3) 32^4 FP32-12 job alone through the job server (a large-volume group: the "
-                      "single-lattice update kernels, one launch per member, and the fused plaquette + Polyakov "
-                      "measurement, captured in a hipGraph), HB + 4 OR + reunit + plaquette + Polyakov per "
-                      "iteration, job start to end"}
+              "iteration_frac": n_steady * it_bytes / steady_s / (PEAK_HBM_GBS * 1e9),
+              "whole_job_260": {"s": min(times[260]),
+                                "iteration_frac": 260 * it_bytes / min(times[260]) / (PEAK_HBM_GBS * 1e9)},
+              "note": "SU(3) 32^4 FP32-12 jobs alone through the job server (a large-volume group: the "
+                      "single-lattice update kernels with the context's launch shapes for 32^4, one launch per "
+                      "member, and the fused plaquette + Polyakov measurement, captured in a hipGraph), HB + 4 OR "
+                      "+ reunit + plaquette + Polyakov per iteration; 220 iterations = a 260-iteration job minus a "
+                      "40-iteration one (best of two each), start to end"}
     est = [j["n_iter"] * 5 * 4 * int(np.prod(j["dims"])) / calib[(j["nc"], j["dims"])] for j in jobs]
     rate = util / float(np.mean(est))
     gaps = rng.exponential(1.0 / rate, n_jobs)
@@ -628,6 +636,12 @@ def main():
     storage, prec = 12, 4
     lat = sun_amd.Lattice(ctx, 3, DIMS, BETA, storage, prec)
     shape_source = pin_shapes(lat, shapes32)
+    if shapes32:
+        # the same shapes as the context's for this lattice description, so the job
+        # server's 32^4 FP32-12 jobs run them too (a tune-cache entry, in effect)
+        for kernel, key in ((1, "overrelax"), (0, "heatbath")):
+            c = shapes32[key]
+            ctx.set_launch(3, DIMS, BETA, storage, prec, kernel, c["block"], tuple(c["brick"]), c["lds"], c["piped"])
     stream = sun_amd.new_stream()
     lat.cold_start(stream)
     sun_amd.sync(stream)

@@ -112,6 +112,13 @@ int sun_lattice_get_launch(const sun_lattice *lat, int kernel, sun_launch *out);
 /* SUN_ERR_INVALID for a shape the lattice's kernels cannot run; marks the lattice
  * tuned, so the autotuner keeps the shape */
 int sun_lattice_set_launch(sun_lattice *lat, int kernel, const sun_launch *cfg);
+/* The context's launch shape for lattices of this description, as if the autotuner had
+ * chosen it: lattices created later on the context with this description start with
+ * it, and a job server on the context runs its large-volume jobs of this description
+ * with it (otherwise the batch defaults).  Replaces an entry of the reference's tune
+ * cache (tune.cpp:158-414).  SUN_ERR_INVALID for a shape the kernels cannot run on
+ * that geometry. */
+int sun_ctx_set_launch(sun_ctx *ctx, const sun_lattice_desc *desc, int kernel, const sun_launch *cfg);
 /* _gauge<Real>::Init() -> ColdStart; refuses 8-real storage as the reference does
  *                                                gaugearray.cu:276-305, gaugearray.h:765-775 */
 int sun_cold_start(sun_lattice *lat, void *stream);

@@ -40,7 +40,7 @@ EXPORTS = [
     "sun_init", "sun_finalize", "sun_last_error", "sun_version",
     "sun_lattice_create", "sun_lattice_destroy", "sun_lattice_bytes", "sun_lattice_copy",
     "sun_lattice_convert",
-    "sun_lattice_get_launch", "sun_lattice_set_launch", "sun_lattice_save", "sun_lattice_load",
+    "sun_lattice_get_launch", "sun_lattice_set_launch", "sun_ctx_set_launch", "sun_lattice_save", "sun_lattice_load",
     "sun_cold_start", "sun_hot_start",
     "sun_heatbath", "sun_overrelax", "sun_heatbath_sub", "sun_overrelax_sub", "sun_reunitarize",
     "sun_plaquette", "sun_polyakov", "sun_plaquette_async", "sun_polyakov_async",
@@ -140,6 +140,7 @@ def lib() -> ctypes.CDLL:
         "sun_lattice_bytes": (ctypes.c_size_t, [ctypes.POINTER(LatticeDesc)]),
         "sun_lattice_copy": (ci, [vp, vp, vp]),
         "sun_lattice_convert": (ci, [vp, vp, vp]),
+        "sun_ctx_set_launch": (ci, [vp, ctypes.POINTER(LatticeDesc), ci, ctypes.POINTER(Launch)]),
         "sun_lattice_get_launch": (ci, [vp, ci, ctypes.POINTER(Launch)]),
         "sun_lattice_set_launch": (ci, [vp, ci, ctypes.POINTER(Launch)]),
         "sun_lattice_save": (ci, [vp, ctypes.c_char_p, ci, ci, u64, u32]),
@@ -245,6 +246,14 @@ class Context:
         if self.h:
             _check(lib().sun_finalize(self.h), "sun_finalize")
             self.h = ctypes.c_void_p()
+
+    def set_launch(self, nc: int, dims, beta: float, storage: int, precision: int, kernel: int, block: int = 256,
+                   brick=(-1, 0, 0), lds: int = 0, piped: int = 0):
+        """the context's launch shape for lattices of this description (later lattices
+        and job-server jobs of it start with it)"""
+        d = LatticeDesc(nc, (ctypes.c_int * 4)(*[int(x) for x in dims]), beta, storage, precision)
+        c = Launch(block, (ctypes.c_int * 3)(*brick), lds, int(piped))
+        _check(lib().sun_ctx_set_launch(self.h, ctypes.byref(d), kernel, ctypes.byref(c)), "sun_ctx_set_launch")
 
     def device_name(self) -> str:
         buf = ctypes.create_string_buffer(256)

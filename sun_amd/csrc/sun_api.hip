@@ -338,35 +338,55 @@ extern "C" int sun_lattice_get_launch(const sun_lattice* lat, int kernel, sun_la
     return SUN_OK;
 }
 
-extern "C" int sun_lattice_set_launch(sun_lattice* lat, int kernel, const sun_launch* cfg) {
-    if (!lat || !cfg || (kernel != 0 && kernel != 1)) return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: bad argument");
-    if (cfg->piped && !lat->ops->has_piped)
-        return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: no pipelined kernel for this N/storage/precision");
-    const int max_block = cfg->piped ? lat->ops->piped_max_block : lat->ops->max_block;
-    if (cfg->piped < 0 || cfg->piped > 2)
-        return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: piped %d (0, 1 or 2)", cfg->piped);
+// a caller's launch shape checked against the kernels of `ops` on geometry g
+static int launch_from_user(const OpsTable* ops, const Geom& g, const int dims[4], const sun_launch* cfg,
+                            const char* what, LaunchCfg* out) {
+    if (cfg->piped && !ops->has_piped)
+        return fail(SUN_ERR_INVALID, "%s: no pipelined kernel for this N/storage/precision", what);
+    const int max_block = cfg->piped ? ops->piped_max_block : ops->max_block;
+    if (cfg->piped < 0 || cfg->piped > 2) return fail(SUN_ERR_INVALID, "%s: piped %d (0, 1 or 2)", what, cfg->piped);
     LaunchCfg c;
     c.piped = cfg->piped;
     c.lds = cfg->lds;
-    if (cfg->lds < 0 || cfg->lds > 65536 * 2) return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: lds %d", cfg->lds);
+    if (cfg->lds < 0 || cfg->lds > 65536 * 2) return fail(SUN_ERR_INVALID, "%s: lds %d", what, cfg->lds);
     if (cfg->brick[0] >= 0) {
-        const int block = brick_block(lat->g, cfg->brick[0], cfg->brick[1], cfg->brick[2], max_block);
+        const int block = brick_block(g, cfg->brick[0], cfg->brick[1], cfg->brick[2], max_block);
         if (!block)
-            return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: brick %d,%d,%d does not tile %dx%dx%dx%d",
-                        cfg->brick[0], cfg->brick[1], cfg->brick[2], lat->d.dims[0], lat->d.dims[1], lat->d.dims[2],
-                        lat->d.dims[3]);
+            return fail(SUN_ERR_INVALID, "%s: brick %d,%d,%d does not tile %dx%dx%dx%d", what, cfg->brick[0],
+                        cfg->brick[1], cfg->brick[2], dims[0], dims[1], dims[2], dims[3]);
         c.b1 = cfg->brick[0];
         c.b2 = cfg->brick[1];
         c.b3 = cfg->brick[2];
         c.block = block;
     } else {
         if (cfg->block < 64 || cfg->block > max_block || (cfg->block & 63))
-            return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: block %d (multiple of 64, <= %d)", cfg->block,
-                        max_block);
+            return fail(SUN_ERR_INVALID, "%s: block %d (multiple of 64, <= %d)", what, cfg->block, max_block);
         c.block = cfg->block;
     }
+    *out = c;
+    return SUN_OK;
+}
+
+extern "C" int sun_lattice_set_launch(sun_lattice* lat, int kernel, const sun_launch* cfg) {
+    if (!lat || !cfg || (kernel != 0 && kernel != 1)) return fail(SUN_ERR_INVALID, "sun_lattice_set_launch: bad argument");
+    LaunchCfg c;
+    SUN_TRY(launch_from_user(lat->ops, lat->g, lat->d.dims, cfg, "sun_lattice_set_launch", &c));
     (kernel == 0 ? lat->cfg_hb : lat->cfg_or) = c;
     lat->tuned = true;
+    return SUN_OK;
+}
+
+extern "C" int sun_ctx_set_launch(sun_ctx* ctx, const sun_lattice_desc* desc, int kernel, const sun_launch* cfg) {
+    if (!ctx || !desc || !cfg || (kernel != 0 && kernel != 1))
+        return fail(SUN_ERR_INVALID, "sun_ctx_set_launch: bad argument");
+    SUN_TRY(check_desc(desc));
+    const OpsTable* ops = find_ops(desc->nc, desc->storage, desc->precision);
+    if (!ops)
+        return fail(SUN_ERR_UNSUPPORTED, "no kernels for nc=%d storage=%d precision=%d", desc->nc, desc->storage,
+                    desc->precision);
+    LaunchCfg c;
+    SUN_TRY(launch_from_user(ops, make_geom(desc->dims), desc->dims, cfg, "sun_ctx_set_launch", &c));
+    ctx->tuner.record(kernel == 0 ? "heatbath" : "overrelax", *desc, c, 0.f);
     return SUN_OK;
 }
 

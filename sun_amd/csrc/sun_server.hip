@@ -258,7 +258,15 @@ int rebuild_tables(Group* g) {
             m.partials = tp[i].partials;
             m.sweep_dev = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(g->t_update + i) +
                                                             offsetof(BatchEntry, sweep));
+            const sun_lattice* L = g->members[i]->lat;
             for (int k = 0; k < 2; k++) {
+                // the context's shape for this lattice description when it has one
+                // (autotuner cache or sun_ctx_set_launch), else the batch defaults
+                const char* name = k == 0 ? "heatbath" : "overrelax";
+                if (L->ctx->tuner.has(name, L->d)) {
+                    m.cfg[k] = L->ctx->tuner.lookup(name, L->d);
+                    continue;
+                }
                 LaunchCfg c;
                 c.block = 256;
                 c.lds = g->lds[k];

@@ -129,6 +129,36 @@ def test_large_volume_groups_equal_alone(ctx):
 
 
 @pytest.mark.gpu
+def test_large_volume_jobs_run_the_context_shapes(ctx):
+    """A context's launch shapes for a lattice description (sun_ctx_set_launch) are
+    what its large-volume server jobs of that description run; lattices created on it
+    start with them; the history equals the job's alone run on a context without."""
+    job = LARGE_JOBS[0]
+    nc, dims, storage, prec = job[0], job[1], job[3], job[4]
+    with sun_amd.Context(0, 0, 0) as c2:
+        c2.set_launch(nc, dims, 6.0, storage, prec, 1, 256, (2, 2, 1), 16384, 1)
+        c2.set_launch(nc, dims, 6.0, storage, prec, 0, 128, (-1, 0, 0), 0, 0)
+        with sun_amd.Lattice(c2, nc, dims, 6.0, storage, prec) as lat:
+            o, h = lat.get_launch(1), lat.get_launch(0)
+            assert list(o.brick) == [2, 2, 1] and o.lds == 16384 and o.piped == 1 and o.block == 256
+            assert h.brick[0] < 0 and h.block == 128 and h.piped == 0
+        with pytest.raises(sun_amd.SunError) as e:
+            c2.set_launch(nc, dims, 6.0, storage, prec, 1, 256, (5, 0, 0), 0, 0)
+        assert e.value.code == sun_amd.ERR_INVALID
+        with pytest.raises(sun_amd.SunError):
+            c2.set_launch(2, dims, 2.4, 18, 8, 1, 256, (-1, 0, 0), 0, 2)  # no pipelined SU(2) kernel
+        with sun_amd.Server(c2, job_bytes(job) + (4 << 20), max_jobs=2) as srv:
+            jid = submit(srv, job)
+            steps = 0
+            while srv.stats().jobs_done < 1:
+                srv.step()
+                steps += 1
+                assert steps < 20
+            got = srv.observables(jid, job[6])
+    assert np.array_equal(got, alone(ctx, job))
+
+
+@pytest.mark.gpu
 def test_arrivals_while_running(ctx):
     """Jobs submitted while others are mid-run join at the next step and still equal
     their alone runs."""
