@@ -358,6 +358,50 @@ def batched_leg(sun_amd, ctx, stream, n_lat=64, n_iter=30, warm=3):
                       "per-lattice plaquette"}
 
 
+def batch_classes(sun_amd, ctx, stream, n_lat=64, reps=10, calibrate=False):
+    """The 64-lattice batch of batched_leg broken down by volume class: one
+    overrelaxation sweep (8 batched launches, the batch's default launch shapes) of
+    the whole batch, then of each class's lattices as a batch of their own, timed with
+    HIP events; algorithmic bytes 9 x 48 B per link update.  calibrate: two plain
+    copies of the 201 MB 32^4 FP32-12 field first (the size reference for counter
+    passes)."""
+    rng = np.random.default_rng(4)
+    vols = [BATCH_VOLUMES[i] for i in rng.integers(0, len(BATCH_VOLUMES), n_lat)]
+    betas = rng.uniform(5.7, 6.3, n_lat)
+    lats = [sun_amd.Lattice(ctx, 3, v, float(b), 12, 4) for v, b in zip(vols, betas)]
+    for i, l in enumerate(lats):
+        l.hot_start(1000 + i, stream)
+    if calibrate:
+        with sun_amd.Lattice(ctx, 3, DIMS, BETA, 12, 4) as a, sun_amd.Lattice(ctx, 3, DIMS, BETA, 12, 4) as b:
+            for _ in range(2):
+                b.copy_from(a, stream)
+            sun_amd.sync(stream)
+
+    def sweep(group):
+        b = sun_amd.Batch(ctx, group, list(range(len(group))))
+        b.overrelax(stream)
+        sun_amd.sync(stream)
+        t = sun_amd.Timer()
+        t.start(stream)
+        for _ in range(reps):
+            b.overrelax(stream)
+        t.stop(stream)
+        ms = t.ms() / reps
+        t.close()
+        b.close()
+        sites = sum(l.V for l in group)
+        gbs = 9 * 48 * 4 * sites / (ms * 1e-3) / 1e9
+        return {"lattices": len(group), "sites": sites, "overrelax_sweep_ms": ms, "gbs": gbs,
+                "frac": gbs / PEAK_HBM_GBS}
+
+    out = {"all": sweep(lats)}
+    for v in sorted(set(vols)):
+        out["x".join(map(str, v))] = sweep([l for l, vv in zip(lats, vols) if vv == v])
+    for l in lats:
+        l.close()
+    return out
+
+
 SERVE_VOLUMES = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 16), (24, 24, 24, 24), (32, 32, 32, 32)]
 SERVE_BETA = {2: (2.2, 2.6), 3: (5.6, 6.2), 4: (10.5, 11.2)}
 
@@ -606,6 +650,9 @@ def main():
     ap.add_argument("--no-meas", action="store_true", help="skip the smearing / Wilson loop leg")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served leg")
     ap.add_argument("--large", action="store_true", help="with --kernels-only: the 64^4 FP32 8-real field")
+    ap.add_argument("--batch-classes", action="store_true",
+                    help="only the 64-lattice batch's overrelaxation sweep by volume class (with copy calibration, "
+                         "for rocprofv3)")
     ap.add_argument("--ops-only", action="store_true",
                     help="only the operator-basis Wilson loop leg (with copy calibration, for rocprofv3)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -622,6 +669,11 @@ def main():
     import sun_amd
     shapes32 = None if args.autotune else profiled_shapes(PROFILE_32)
     shapes64 = None if args.autotune else profiled_shapes(PROFILE_64)
+    if args.batch_classes:
+        ctx = sun_amd.Context(0, 0, 0)
+        print(json.dumps(batch_classes(sun_amd, ctx, sun_amd.new_stream(), calibrate=True)))
+        ctx.close()
+        return
     if args.ops_only:
         ctx = sun_amd.Context(0, 0, 0)
         print(json.dumps(operator_loops_leg(sun_amd, ctx, sun_amd.new_stream(), calibrate=True)))
@@ -711,6 +763,7 @@ def main():
         out["shard_share_64x4_f32_8"] = shard_leg(sun_amd, ctx, stream)
     if not args.no_batch:
         out["batched"] = batched_leg(sun_amd, ctx, stream)
+        out["batched"]["by_volume_class"] = batch_classes(sun_amd, ctx, stream)
     if not args.no_serve:
         out["served"] = served_leg(sun_amd, ctx, args.serve_jobs)
     if not args.no_cpu:
