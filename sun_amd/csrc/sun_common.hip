@@ -132,6 +132,31 @@ __global__ void k_batch_record(BatchEntry* table, int n, const double2* plaq, co
     E.hist_iter += 1u;
 }
 
+// the end of a large-volume group's iteration in one launch (one workgroup per
+// member): the plaquette and Polyakov sums of k_batch_finalize, the history record of
+// k_batch_record and the sweep advance of k_batch_advance, with the same values
+__global__ void __launch_bounds__(kReduceBlock) k_group_close(BatchEntry* t_plaq, const BatchEntry* t_poly,
+                                                              BatchEntry* t_update, double2* plaq, double2* poly) {
+    const int i = blockIdx.x;
+    BatchEntry& P = t_plaq[i];
+    const BatchEntry& Q = t_poly[i];
+    double2 a = ordered_sum(P.partials, P.nblocks);
+    __syncthreads();
+    double2 b = ordered_sum(Q.partials, Q.nblocks);
+    if (threadIdx.x == 0) {
+        a = make_double2(a.x * P.scale, a.y * P.scale);
+        b = make_double2(b.x * Q.scale, b.y * Q.scale);
+        plaq[i] = a;
+        poly[i] = b;
+        if (P.hist) {
+            P.hist[2 * (size_t)P.hist_iter] = a;
+            P.hist[2 * (size_t)P.hist_iter + 1] = b;
+        }
+        P.hist_iter += 1u;
+        t_update[i].sweep += 1u;
+    }
+}
+
 __global__ void k_advance_counter(uint32_t* ctr) {
     if (threadIdx.x == 0) ctr[0] += 1u;
 }
@@ -197,6 +222,13 @@ hipError_t launch_wlop_finalize(const double* partials, int nb, int nout, double
 hipError_t launch_batch_finalize(const BatchEntry* table, int nlat, double2* out, hipStream_t s) {
     if (nlat <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_batch_finalize, dim3(nlat), dim3(kReduceBlock), 0, s, table, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_close(BatchEntry* t_plaq, const BatchEntry* t_poly, BatchEntry* t_update, int nlat,
+                              double2* plaq, double2* poly, hipStream_t s) {
+    if (nlat <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_group_close, dim3(nlat), dim3(kReduceBlock), 0, s, t_plaq, t_poly, t_update, plaq, poly);
     return hipGetLastError();
 }
 

@@ -254,6 +254,10 @@ hipError_t launch_monitor_finalize(const double2* partials, int n, double scale_
 // finalize for every lattice of a batch table: out[i] = (sum of entry i's partials)
 // * entry i's scale, in the same order as launch_finalize
 hipError_t launch_batch_finalize(const BatchEntry* table, int nlat, double2* out, hipStream_t s);
+// plaquette and Polyakov finalize, history record and sweep advance of a group whose
+// members' plaquette and Polyakov partials are separate buffers, in one launch
+hipError_t launch_group_close(BatchEntry* t_plaq, const BatchEntry* t_poly, BatchEntry* t_update, int nlat,
+                              double2* plaq, double2* poly, hipStream_t s);
 // entry i's history row hist_iter <- {plaq[i], poly[i]}; hist_iter += 1
 hipError_t launch_batch_record(BatchEntry* table, int nlat, const double2* plaq, const double2* poly,
                                hipStream_t s);

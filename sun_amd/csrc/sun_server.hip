@@ -48,8 +48,9 @@ namespace {
 constexpr int kMaxPassOver = 8;
 // work per group launch before the scheduler looks again.  Between two launches of a
 // group the device idles for the completion poll, the retire/admit step and the graph
-// launch (~0.1 ms); at 2 ms per launch that was 7% of a served 32^4 job's time.
-constexpr double kTargetMs = 10.0;
+// launch (~0.1 ms): 7% of a served 32^4 job's time at 2 ms per launch, 1% at 10 ms,
+// 0.25% at 40 ms.  A job arriving for a running group joins at its next launch.
+constexpr double kTargetMs = 40.0;
 constexpr int kMaxItersPerLaunch = 64;
 constexpr int kLargeVolume = 1 << 17;  // lattices at least this large get a group per volume
 
@@ -326,14 +327,14 @@ int group_iteration(Group* g) {
         // plaquette and Polyakov partials of a member in one launch (the same partial
         // sums as the two batched kernels)
         for (const Group::Solo& m : g->solo) HIP_TRY(g->ops->measure_partial(m.f, m.g, m.partials, s));
-        HIP_TRY(launch_batch_finalize(g->t_plaq, n, g->d_plaq, s));
-        HIP_TRY(launch_batch_finalize(g->t_poly, n, g->d_poly, s));
-    } else {
-        HIP_TRY(g->ops->batch_plaquette_partial(g->t_plaq, n, g->blocks_plaq, s));
-        HIP_TRY(launch_batch_finalize(g->t_plaq, n, g->d_plaq, s));
-        HIP_TRY(g->ops->batch_polyakov_partial(g->t_poly, n, g->blocks_poly, s));
-        HIP_TRY(launch_batch_finalize(g->t_poly, n, g->d_poly, s));
+        // both finalizes, the history record and the sweep advance in one launch
+        HIP_TRY(launch_group_close(g->t_plaq, g->t_poly, g->t_update, n, g->d_plaq, g->d_poly, s));
+        return SUN_OK;
     }
+    HIP_TRY(g->ops->batch_plaquette_partial(g->t_plaq, n, g->blocks_plaq, s));
+    HIP_TRY(launch_batch_finalize(g->t_plaq, n, g->d_plaq, s));
+    HIP_TRY(g->ops->batch_polyakov_partial(g->t_poly, n, g->blocks_poly, s));
+    HIP_TRY(launch_batch_finalize(g->t_poly, n, g->d_poly, s));
     HIP_TRY(launch_batch_record(g->t_plaq, n, g->d_plaq, g->d_poly, s));
     HIP_TRY(launch_batch_advance(g->t_update, n, s));
     return SUN_OK;
