@@ -597,20 +597,27 @@ def operator_loops_leg(sun_amd, ctx, stream, calibrate=False):
     return out
 
 
-def kernels_only(sun_amd, large=False, shapes=None):
+def kernels_only(sun_amd, large=False, shapes=None, variant=None):
     """For rocprofv3 counter passes.  Default: on the 32^4 FP32 12-real field after
     100 iterations, 2 copies of the field (calibration: a plain copy of known size),
     then 8 overrelaxation launches, 8 heatbath launches, reunitarize, plaquette,
     Polyakov loop.  With large: the same on the 64^4 FP32 8-real field (Haar start,
-    beta=6.2) after 20 iterations.  Launch shapes: those of the committed summary
-    (as in the benchmark), or with --autotune the tune cache in $SUN_RESOURCE_PATH
-    written by a preceding `bench.py --autotune` run with the same variable; without
-    a cache the autotuner runs first and its trial launches appear in the profile too."""
+    beta=6.2) after 20 iterations.  With variant (N, storage, precision): the same at
+    32^4 for that kernel variant (beta 2.3 / 6.0 / 11.0 for N = 2 / 3 / 4) after 20
+    iterations.  Launch shapes: those of the committed summary (as in the benchmark),
+    or with --autotune (and for a variant) the tune cache in $SUN_RESOURCE_PATH
+    written by a preceding run with the same variable; without a cache the autotuner
+    runs first and its trial launches appear in the profile too."""
     ctx = sun_amd.Context(0, 0, 1)
+    nc, prec = 3, 4
     dims, beta, storage = ((64, 64, 64, 64), 6.2, 8) if large else (DIMS, BETA, 12)
-    lat = sun_amd.Lattice(ctx, 3, dims, beta, storage, 4)
-    pin_shapes(lat, shapes)
-    tmp = sun_amd.Lattice(ctx, 3, dims, beta, storage, 4)
+    if variant:
+        nc, storage, prec = variant
+        beta = {2: 2.3, 3: 6.0, 4: 11.0}[nc]
+    lat = sun_amd.Lattice(ctx, nc, dims, beta, storage, prec)
+    if not variant:
+        pin_shapes(lat, shapes)
+    tmp = sun_amd.Lattice(ctx, nc, dims, beta, storage, prec)
     s = sun_amd.new_stream()
     if large:
         lat.hot_start(3, s)
@@ -618,7 +625,7 @@ def kernels_only(sun_amd, large=False, shapes=None):
         lat.cold_start(s)
     # iterations first, so clocks are at their sustained level when the launches
     # below run (the profile averages then match the benchmark's timings)
-    lat.generate(seed=2, n_iter=20 if large else 100, n_or=4, reunit_every=0 if large else 1, stream=s)
+    lat.generate(seed=2, n_iter=20 if (large or variant) else 100, n_or=4, reunit_every=0 if large else 1, stream=s)
     for _ in range(2):
         tmp.copy_from(lat, s)
     for parity in (0, 1):
@@ -631,7 +638,8 @@ def kernels_only(sun_amd, large=False, shapes=None):
     lat.plaquette(s)
     lat.polyakov(s)
     sun_amd.sync(s)
-    print(f"kernels-only: field {lat.bytes} bytes, copy calibration size {tmp.bytes} bytes")
+    print(f"kernels-only: field {lat.bytes} bytes, copy calibration size {tmp.bytes} bytes; shapes: overrelaxation "
+          f"{json.dumps(lat.get_launch(1).as_dict())}, heatbath {json.dumps(lat.get_launch(0).as_dict())}")
     tmp.close()
     lat.close()
     ctx.close()
@@ -650,6 +658,8 @@ def main():
     ap.add_argument("--no-meas", action="store_true", help="skip the smearing / Wilson loop leg")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the served leg")
     ap.add_argument("--large", action="store_true", help="with --kernels-only: the 64^4 FP32 8-real field")
+    ap.add_argument("--variant", default=None,
+                    help="with --kernels-only: N,storage,precision of another kernel variant at 32^4 (e.g. 4,18,4)")
     ap.add_argument("--batch-classes", action="store_true",
                     help="only the 64-lattice batch's overrelaxation sweep by volume class (with copy calibration, "
                          "for rocprofv3)")
@@ -680,7 +690,8 @@ def main():
         ctx.close()
         return
     if args.kernels_only:
-        kernels_only(sun_amd, large=args.large, shapes=shapes64 if args.large else shapes32)
+        variant = tuple(int(v) for v in args.variant.split(",")) if args.variant else None
+        kernels_only(sun_amd, large=args.large, shapes=shapes64 if args.large else shapes32, variant=variant)
         return
     ctx = sun_amd.Context(0, 0, 1)
     V = int(np.prod(DIMS))
