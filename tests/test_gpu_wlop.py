@@ -141,3 +141,31 @@ def test_wlop_fullsize_32x4(ctx):
     # |W_ij| <= 3 * 12 / 9 after the 3 N V norm
     assert np.abs(got).max() <= 4.0 + 1e-12
     assert not G0[:, 9, :].any() and not G0[:, :, 9].any()
+
+
+def test_wlop_33_fullsize_32x4(ctx):
+    """the 33 family at the reference's lattice size (32^4 SU(3) FP64, r = 12, T <= 16):
+    slot 0 is the line of the A0 family, so the (0, 0) correlator equals A0's; the T = 0
+    matrix is symmetric; and |W_ij(T)| <= sqrt(W_ii(0) W_jj(0)) (Cauchy-Schwarz over
+    sites and entries: the temporal transport keeps the Frobenius norm of O_i)"""
+    dims = (32, 32, 32, 32)
+    with sun_amd.Lattice(ctx, 3, dims, 6.0, 18, 8) as lat:
+        lat.hot_start(13)
+        for it in range(2):
+            lat.heatbath(14, it)
+            lat.overrelax()
+        print("32^4 operator-basis correlators: 33 and A0 families, r = 12, mu = 1, T <= 16", flush=True)
+        got = {}
+        for family in (sun_amd.WLOP_33, sun_amd.WLOP_A0):
+            with sun_amd.WilsonLoopOps(lat, family, 16) as ops:
+                ops.fields(12, 1)
+                ops.correlate(12, 1)
+                got[family] = ops.read().reshape(ops.nop, ops.nop, 17)
+    G = got[sun_amd.WLOP_33]
+    assert np.abs(G[0, 0] - got[sun_amd.WLOP_A0][0, 0]).max() <= 1e-12 * np.abs(G[0, 0]).max()
+    G0 = G[:, :, 0]
+    assert np.abs(G0 - G0.T).max() <= 1e-12 * np.abs(G0).max()
+    d = np.sqrt(np.diag(G0))
+    assert (np.diag(G0) > 0).all()
+    assert (np.abs(G) <= np.multiply.outer(d, d)[:, :, None] * (1 + 1e-12)).all()
+    assert np.abs(G[:, :, 16]).max() > 0
