@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DIMS = [(2, 2, 2, 2), (4, 4, 4, 4), (4, 6, 8, 10), (6, 2, 4, 12), (12, 12, 12, 12), (8, 8, 8, 8), (32, 4, 4, 8)]
+DIMS = [(2, 2, 2, 2), (4, 4, 4, 4), (4, 6, 8, 10), (6, 2, 4, 12), (12, 12, 12, 12), (8, 8, 8, 8), (32, 4, 4, 8), (24, 8, 8, 8)]
 
 
 def brute(dims):
