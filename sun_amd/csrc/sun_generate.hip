@@ -178,9 +178,17 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
     }
     b->lats.assign(lats, lats + n);
     std::vector<BatchEntry> tu(n), tr(n), tp(n);
-    int cu = 0, cr = 0, cp = 0;
+    std::vector<int> poff(n);  // each lattice's plaquette partials in b->partials
+    int cu = 0, cr = 0, cp = 0, cp_all = 0;
+    size_t chunk_bytes = 0;
     for (int i = 0; i < n; i++) {
         sun_lattice* L = lats[i];
+        if (b->chunks.empty() || (b->chunks.back().n > 0 && chunk_bytes + L->bytes > kBatchChunkBytes)) {
+            b->chunks.push_back(sun_batch::Chunk{i, 0, 0, 0, 0});
+            chunk_bytes = 0;
+            cu = cr = cp = 0;
+        }
+        chunk_bytes += L->bytes;
         BatchEntry e;
         memset(&e, 0, sizeof(e));
         e.f = L->f;
@@ -193,29 +201,35 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
         a.first_block = cu;
         a.nblocks = update_blocks(L->g, 256);
         cu += a.nblocks;
+        b->chunks.back().n++;
+        b->chunks.back().blocks = cu;
+        b->blocks_update += a.nblocks;
         r.first_block = cr;
         r.nblocks = link_blocks(L->g);
         cr += r.nblocks;
+        b->chunks.back().blocks_reunit = cr;
+        b->blocks_reunit += r.nblocks;
         p.first_block = cp;
         p.nblocks = plaquette_blocks(L->g);
         p.scale = 1.0 / ((double)(3 * L->d.nc) * (double)L->g.V);  // plaquette_cub.cu:234
         cp += p.nblocks;
+        b->chunks.back().blocks_plaq = cp;
+        poff[i] = cp_all;
+        cp_all += p.nblocks;
         tu[i] = a;
         tr[i] = r;
         tp[i] = p;
     }
-    b->blocks_update = cu;
-    b->blocks_reunit = cr;
-    b->blocks_plaq = cp;
+    b->blocks_plaq = cp_all;
     b->h_update = tu;
     hipError_t e = hipSetDevice(ctx->device);
     if (e == hipSuccess) e = hipMalloc(&b->t_update, sizeof(BatchEntry) * n);
     if (e == hipSuccess) e = hipMalloc(&b->t_reunit, sizeof(BatchEntry) * n);
     if (e == hipSuccess) e = hipMalloc(&b->t_plaq, sizeof(BatchEntry) * n);
-    if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double2) * (size_t)cp);
+    if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double2) * (size_t)cp_all);
     if (e == hipSuccess) e = hipMalloc(&b->d_out, sizeof(double2) * n);
     if (e == hipSuccess)
-        for (int i = 0; i < n; i++) tp[i].partials = b->partials + tp[i].first_block;
+        for (int i = 0; i < n; i++) tp[i].partials = b->partials + poff[i];
     if (e == hipSuccess) e = hipMalloc(&b->iter_dev, sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(&b->h_out, sizeof(double2) * n, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemcpy(b->t_update, tu.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
@@ -272,28 +286,39 @@ static int batch_set_sweeps(sun_batch* b, const uint32_t* sweeps, uint32_t unifo
     return SUN_OK;
 }
 
-static int batch_hb(sun_batch* b, hipStream_t s) {
-    const int n = (int)b->lats.size();
+// one heatbath (hb) or overrelaxation sweep of one update chunk
+static int chunk_sweep(sun_batch* b, const sun_batch::Chunk& c, bool hb, hipStream_t s) {
+    const int k = hb ? 0 : 1;
     for (int parity = 0; parity < 2; parity++)
         for (int mu = 0; mu < 4; mu++)
-            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, true, b->piped[0], b->brick[0],
-                                         b->lds[0], s));
+            HIP_TRY(b->ops->batch_update(b->t_update + c.first, c.n, c.blocks, parity, mu, hb, b->piped[k], b->brick[k],
+                                         b->lds[k], s));
+    return SUN_OK;
+}
+
+static int batch_hb(sun_batch* b, hipStream_t s) {
+    for (const sun_batch::Chunk& c : b->chunks) SUN_TRY(chunk_sweep(b, c, true, s));
     return SUN_OK;
 }
 
 static int batch_or(sun_batch* b, hipStream_t s) {
-    const int n = (int)b->lats.size();
-    for (int parity = 0; parity < 2; parity++)
-        for (int mu = 0; mu < 4; mu++)
-            HIP_TRY(b->ops->batch_update(b->t_update, n, b->blocks_update, parity, mu, false, b->piped[1], b->brick[1],
-                                         b->lds[1], s));
+    for (const sun_batch::Chunk& c : b->chunks) SUN_TRY(chunk_sweep(b, c, false, s));
+    return SUN_OK;
+}
+
+static int chunk_reunit(sun_batch* b, const sun_batch::Chunk& c, hipStream_t s) {
+    HIP_TRY(b->ops->batch_reunitarize(b->t_reunit + c.first, c.n, c.blocks_reunit, s));
+    return SUN_OK;
+}
+
+static int chunk_plaq(sun_batch* b, const sun_batch::Chunk& c, hipStream_t s) {
+    HIP_TRY(b->ops->batch_plaquette_partial(b->t_plaq + c.first, c.n, c.blocks_plaq, s));
+    HIP_TRY(launch_batch_finalize(b->t_plaq + c.first, c.n, b->d_out + c.first, s));
     return SUN_OK;
 }
 
 static int batch_plaq(sun_batch* b, hipStream_t s) {
-    const int n = (int)b->lats.size();
-    HIP_TRY(b->ops->batch_plaquette_partial(b->t_plaq, n, b->blocks_plaq, s));
-    HIP_TRY(launch_batch_finalize(b->t_plaq, n, b->d_out, s));
+    for (const sun_batch::Chunk& c : b->chunks) SUN_TRY(chunk_plaq(b, c, s));
     return SUN_OK;
 }
 
@@ -311,7 +336,7 @@ extern "C" int sun_batch_overrelax(sun_batch* b, void* stream) {
 
 extern "C" int sun_batch_reunitarize(sun_batch* b, void* stream) {
     if (!b) return fail(SUN_ERR_INVALID, "sun_batch_reunitarize: NULL");
-    HIP_TRY(b->ops->batch_reunitarize(b->t_reunit, (int)b->lats.size(), b->blocks_reunit, batch_stream(b, stream)));
+    for (const sun_batch::Chunk& c : b->chunks) SUN_TRY(chunk_reunit(b, c, batch_stream(b, stream)));
     return SUN_OK;
 }
 
@@ -329,15 +354,19 @@ extern "C" int sun_batch_plaquette(sun_batch* b, void* stream, double* out) {
     return SUN_OK;
 }
 
+// one iteration: chunk by chunk, the heatbath sweep, n_or overrelaxation sweeps,
+// reunitarisation and plaquette back to back (lattices are independent, so every
+// lattice sees the same launches in the same order as without chunks); then the
+// history row of the whole batch
 static int batch_iteration(sun_batch* b, int n_or, bool reunit, bool measure, hipStream_t s) {
     const int n = (int)b->lats.size();
-    SUN_TRY(batch_hb(b, s));
-    for (int k = 0; k < n_or; k++) SUN_TRY(batch_or(b, s));
-    if (reunit) HIP_TRY(b->ops->batch_reunitarize(b->t_reunit, n, b->blocks_reunit, s));
-    if (measure) {
-        SUN_TRY(batch_plaq(b, s));
-        HIP_TRY(launch_record_obs(b->d_out, n, b->obs_dev, b->iter_dev, 0, s));
+    for (const sun_batch::Chunk& c : b->chunks) {
+        SUN_TRY(chunk_sweep(b, c, true, s));
+        for (int k = 0; k < n_or; k++) SUN_TRY(chunk_sweep(b, c, false, s));
+        if (reunit) SUN_TRY(chunk_reunit(b, c, s));
+        if (measure) SUN_TRY(chunk_plaq(b, c, s));
     }
+    if (measure) HIP_TRY(launch_record_obs(b->d_out, n, b->obs_dev, b->iter_dev, 0, s));
     HIP_TRY(launch_batch_advance(b->t_update, n, s));
     HIP_TRY(launch_advance_counter(b->iter_dev, s));
     return SUN_OK;

@@ -127,6 +127,17 @@ struct sun_lattice {
     double2* hist = nullptr;       // observable history (server jobs), in the arena
 };
 
+// Link bytes of one batched update chunk: the lattices of a chunk stay in the GPU's
+// 256 MB memory-side cache (MI355X Infinity Cache) through the chunk's consecutive
+// launches, so the links a launch reads again (neighbours shared across workgroups and
+// XCDs, the same field in the next (parity, mu) launch) come from there instead of
+// HBM.  24^4 FP32-12 lattices, one overrelaxation sweep: 4 in one batch (256 MB) 0.62
+// of 8 TB/s, 18 (1.15 GB) 0.55.  The 64-lattice benchmark mix (1.36 GB) per
+// iteration, by chunk budget: none 9.7-9.8e9 link updates/s, 96 MB 8.5e9, 128 MB
+// 9.0e9, 192 MB 9.9-10.1e9, 256 MB 10.3-10.6e9, 320 MB 10.1-10.3e9, 384 MB 9.6-9.8e9
+// (smaller chunks: launches too small to fill the GPU).
+constexpr size_t kBatchChunkBytes = size_t(256) << 20;
+
 struct sun_batch {
     sun_ctx* ctx = nullptr;
     std::vector<sun_lattice*> lats;
@@ -136,6 +147,15 @@ struct sun_batch {
     sun::BatchEntry* t_reunit = nullptr;
     sun::BatchEntry* t_plaq = nullptr;
     int blocks_update = 0, blocks_reunit = 0, blocks_plaq = 0;
+    // chunks: consecutive lattices whose link fields together stay within
+    // kBatchChunkBytes (one larger lattice is a chunk of its own); an iteration runs
+    // each chunk's heatbath and overrelaxation sweeps, reunitarisation and plaquette
+    // back to back, while its fields sit in the memory-side cache.  The tables'
+    // first_block counts from the chunk's start; blocks_* are the chunk's totals.
+    struct Chunk {
+        int first = 0, n = 0, blocks = 0, blocks_reunit = 0, blocks_plaq = 0;
+    };
+    std::vector<Chunk> chunks;
     int piped[2] = {0, 0};  // staple kernel of the heatbath / overrelaxation launches
     int brick[2] = {0, 0};  // their lattices' update bricks instead of linear order
     int lds[2] = {0, 0};    // unused dynamic LDS per workgroup (occupancy cap)
