@@ -162,6 +162,34 @@ def test_batch_bitwise_equals_alone(ctx, storage, prec):
         lat.close()
 
 
+def test_batch_chunks_bitwise_equal_alone(ctx):
+    """A batch runs in chunks of consecutive lattices with at most 256 MB of links; a
+    lattice larger than that is a chunk of its own.  Here 8^4 + 16^4 (one chunk), a
+    302 MB 32^3x48 lattice (its own chunk), 12^4 + 8^4 (a third chunk): every lattice's
+    observables and links equal its alone run bit for bit."""
+    specs = [((8, 8, 8, 8), 5.8, 41), ((16, 16, 16, 16), 6.0, 42), ((32, 32, 32, 48), 6.1, 43),
+             ((12, 12, 12, 12), 6.2, 44), ((8, 8, 8, 8), 6.3, 45)]
+    alone = []
+    for dims, beta, seed in specs:
+        with sun_amd.Lattice(ctx, 3, dims, beta, 12, 4) as lat:
+            lat.hot_start(seed)
+            obs = lat.generate(seed=seed, n_iter=2, n_or=4, reunit_every=1, first_sweep=0, use_graph=False)
+            alone.append((obs[:, :2], lat.to_host()))
+    lats = []
+    for dims, beta, seed in specs:
+        lat = sun_amd.Lattice(ctx, 3, dims, beta, 12, 4)
+        lat.hot_start(seed)
+        lats.append(lat)
+    b = sun_amd.Batch(ctx, lats, [s[2] for s in specs])
+    obs = b.generate(first_sweep=0, n_iter=2, n_or=4, reunit_every=1, use_graph=True)
+    for i, lat in enumerate(lats):
+        assert np.array_equal(obs[:, i, :], alone[i][0]), i
+        assert np.array_equal(lat.to_host(), alone[i][1]), i
+    b.close()
+    for lat in lats:
+        lat.close()
+
+
 @pytest.mark.parametrize("storage", [12, 8])
 def test_batch_launch_shapes_bitwise_equal(ctx, storage):
     """Every batched launch shape -- plain, direction- and half-direction-pipelined
