@@ -196,7 +196,7 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
         e.seed = seeds ? seeds[i] : 0;
         e.bnc = L->d.beta / L->d.nc;
         e.sweep = 0;
-        batch_brick(L->g, e.ub1, e.ub2, e.ub3);
+        batch_brick(L->g, e.ub1, e.ub2, e.ub3, e.spt);
         BatchEntry a = e, r = e, p = e;
         a.first_block = cu;
         a.nblocks = update_blocks(L->g, 256);

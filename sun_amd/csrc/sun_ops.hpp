@@ -84,23 +84,37 @@ struct BatchEntry {
     double scale;        // normalisation of the reduced sum
     double2* hist;       // observable history: {plaquette, polyakov} per iteration
     uint32_t hist_iter;  // next history row
-    int ub1, ub2, ub3;   // brick of the 256-thread update workgroups (ub1 < 0: none)
+    int ub1, ub2, ub3;   // update brick (ub1 < 0: none)
+    int spt;             // 256-thread workgroups per update brick (the brick holds 256 spt sites)
 };
 
-// The update brick of a batched lattice: 256 threads = (X0/2) * 2^k rows.  The k
-// doublings go to x1, x3, x1, x3 (the tuned single-lattice shape at 32^4 is such a
-// (2, 0, 2) brick), then x2, x2, then round-robin, each only where the extent stays
-// divisible.  b1 < 0 when no brick tiles the lattice.
-inline void batch_brick(const Geom& g, int& b1, int& b2, int& b3) {
+// The update brick of a batched lattice: 256 * spt sites = (X0/2) * 2^k rows, taken by
+// spt consecutive 256-thread workgroups (same XCD, side by side; every workgroup of a
+// batch does the same work, so the XCDs' contiguous ranges stay balanced).  spt = 1
+// where rows of X0/2 tile 256 sites: the k doublings go to x1, x3, x1, x3 (the tuned
+// single-lattice shape at 32^4 is such a (2, 0, 2) brick), then x2, x2, then
+// round-robin, each only where the extent stays divisible.  Else spt = 3 where rows
+// tile 768 sites (X0/2 = 12: 24^4, 64 rows), doublings x1, x3, x1, x3, x1, x3 (8 x 1 x
+// 8 rows at 24^4: in the cache-sized chunks of a batch the best of five shapes
+// measured, 4 x 2 x 8 and 4 x 4 x 4 rows 1-2% slower).  b1 < 0 (spt = 1) when no brick
+// tiles the lattice.
+inline void batch_brick(const Geom& g, int& b1, int& b2, int& b3, int& spt) {
     b1 = -1;
     b2 = b3 = 0;
+    spt = 1;
     const int L = g.X[0] / 2;
-    if (L <= 0 || L > 256 || 256 % L) return;
+    if (L <= 0 || L > 768) return;
+    int m = 0;
+    if (256 % L == 0) m = 1;
+    else if (768 % L == 0 && ((768 / L) & (768 / L - 1)) == 0) m = 3;
+    if (!m) return;
     int k = 0;
-    while ((L << k) < 256) k++;
-    static const int seq[] = {1, 3, 1, 3, 2, 2, 1, 3, 2, 1, 3, 2, 1, 3, 2, 1, 3, 2};
+    while ((L << k) < 256 * m) k++;
+    static const int seq1[] = {1, 3, 1, 3, 2, 2, 1, 3, 2, 1, 3, 2, 1, 3, 2, 1, 3, 2};
+    static const int seq3[] = {1, 3, 1, 3, 1, 3, 2, 2, 1, 3, 2, 1, 3, 2, 1, 3, 2, 2};
+    const int* seq = m == 1 ? seq1 : seq3;
     int c[4] = {0, 0, 0, 0};
-    for (int i = 0; i < (int)(sizeof(seq) / sizeof(seq[0])) && k > 0; i++) {
+    for (int i = 0; i < 18 && k > 0; i++) {
         const int d = seq[i];
         if (g.X[d] % (2 << c[d]) == 0) {
             c[d]++;
@@ -111,6 +125,7 @@ inline void batch_brick(const Geom& g, int& b1, int& b2, int& b3) {
     b1 = c[1];
     b2 = c[2];
     b3 = c[3];
+    spt = m;
 }
 
 struct OpsTable {

@@ -222,7 +222,7 @@ int rebuild_tables(Group* g) {
         e.sweep = (uint32_t)J->iters_done;
         e.hist = L->hist;
         e.hist_iter = (uint32_t)J->iters_done;
-        batch_brick(L->g, e.ub1, e.ub2, e.ub3);
+        batch_brick(L->g, e.ub1, e.ub2, e.ub3, e.spt);
         BatchEntry a = e, r = e, p = e, q = e;
         a.first_block = cu;
         a.nblocks = update_blocks(L->g, 256);
@@ -272,7 +272,7 @@ int rebuild_tables(Group* g) {
                 c.block = 256;
                 c.lds = g->lds[k];
                 c.piped = g->piped[k];
-                if (g->brick[k] && a.ub1 >= 0) {
+                if (g->brick[k] && a.ub1 >= 0 && a.spt == 1) {  // single-lattice launches: 256-thread bricks
                     c.b1 = a.ub1;
                     c.b2 = a.ub2;
                     c.b3 = a.ub3;

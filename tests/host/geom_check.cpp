@@ -90,18 +90,33 @@ int main(int argc, char** argv) {
         fprintf(stderr, "reduce brick %d %d %d ok\n", g.rb1, g.rb2, g.rb3);
     }
     // the batched update brick, where one exists, is a shape brick_block accepts with
-    // exactly 256 threads, so ceil(V/2 / 256) workgroups tile the parity exactly
+    // exactly 256 * spt sites, V/2 / (256 spt) bricks tile the parity exactly (spt
+    // workgroups of 256 threads each), and the brick mapping is a bijection
     {
-        int b1, b2, b3;
-        sun::batch_brick(g, b1, b2, b3);
+        int b1, b2, b3, spt;
+        sun::batch_brick(g, b1, b2, b3, spt);
         const int L = g.X[0] / 2;
         const bool pow2 = L > 0 && L <= 256 && (256 % L) == 0;
         if (b1 >= 0) {
-            if (sun::brick_block(g, b1, b2, b3, 256) != 256 || g.half % 256) {
-                printf("BATCH BRICK FAIL %d %d %d\n", b1, b2, b3);
+            const int sites = 256 * spt;
+            if ((spt != 1 && spt != 3) || sun::brick_block(g, b1, b2, b3, sites) != sites || g.half % sites) {
+                printf("BATCH BRICK FAIL %d %d %d spt %d\n", b1, b2, b3, spt);
                 return 1;
             }
-            fprintf(stderr, "batch brick %d %d %d ok\n", b1, b2, b3);
+            std::vector<int> seen(g.half, 0);
+            for (int lb = 0; lb < g.half / 256; lb++)
+                for (int t = 0; t < 256; t++) {
+                    const int B = lb / spt;
+                    const int id = sun::brick_site(g, B, (lb - B * spt) * 256 + t, b1, b2, b3);
+                    if (id < 0 || id >= g.half || seen[id]++) {
+                        printf("BATCH BRICK MAP FAIL lb=%d t=%d\n", lb, t);
+                        return 1;
+                    }
+                }
+            fprintf(stderr, "batch brick %d %d %d spt %d ok\n", b1, b2, b3, spt);
+        } else if (spt != 1) {
+            printf("BATCH BRICK SPT WITHOUT BRICK %d\n", spt);
+            return 1;
         } else if (pow2 && (long long)g.half % 256 == 0) {
             // no brick although X0/2 divides 256: only when no split of the rows over
             // x1, x2, x3 divides the extents
