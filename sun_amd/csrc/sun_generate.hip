@@ -155,8 +155,19 @@ static void batch_free(sun_batch* b) {
     delete b;
 }
 
+static int sun_batch_create_body(sun_ctx* ctx, int n, sun_lattice* const* lats, const uint64_t* seeds,
+                                 sun_batch** out, sun_batch*& b);
+
 extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, const uint64_t* seeds,
                                 sun_batch** out) {
+    sun_batch* b = nullptr;  // freed here when a host allocation fails part way
+    const int rc = guard_alloc("sun_batch_create", [&] { return sun_batch_create_body(ctx, n, lats, seeds, out, b); });
+    if (rc == SUN_ERR_NOMEM && b && out && *out != b) batch_free(b);
+    return rc;
+}
+
+static int sun_batch_create_body(sun_ctx* ctx, int n, sun_lattice* const* lats, const uint64_t* seeds,
+                                 sun_batch** out, sun_batch*& b) {
     if (!ctx || !lats || !out || n <= 0 || n > 1024)
         return fail(SUN_ERR_INVALID, "sun_batch_create: bad argument (n=%d, 1..1024)", n);
     *out = nullptr;
@@ -167,7 +178,7 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
             return fail(SUN_ERR_INVALID, "sun_batch_create: lattice %d differs in N/storage/precision", i);
         if (lats[i]->ctx != ctx) return fail(SUN_ERR_INVALID, "sun_batch_create: lattice %d has another context", i);
     }
-    sun_batch* b = new (std::nothrow) sun_batch();
+    b = new (std::nothrow) sun_batch();
     if (!b) return fail(SUN_ERR_NOMEM, "host allocation");
     b->ctx = ctx;
     b->ops = ops;
@@ -238,6 +249,7 @@ extern "C" int sun_batch_create(sun_ctx* ctx, int n, sun_lattice* const* lats, c
     if (e != hipSuccess) {
         hipGetLastError();
         batch_free(b);
+        b = nullptr;
         return fail(SUN_ERR_NOMEM, "sun_batch_create: %s", hipGetErrorString(e));
     }
     *out = b;
