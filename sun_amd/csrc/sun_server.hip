@@ -118,6 +118,13 @@ struct Group {
         LaunchCfg cfg[2];
     };
     std::vector<Solo> solo;
+    // update chunks (sun_batch's kBatchChunkBytes): consecutive members whose link
+    // fields together fit the memory-side cache run their heatbath and overrelaxation
+    // launches back to back; t_update's first_block counts from the chunk's start
+    struct Chunk {
+        int first = 0, n = 0, blocks = 0;
+    };
+    std::vector<Chunk> chunks;
 };
 
 double now_since(std::chrono::steady_clock::time_point t0) {
@@ -210,9 +217,18 @@ int rebuild_tables(Group* g) {
     const int n = (int)g->members.size();
     std::vector<BatchEntry> tu(n), tr(n), tp(n), tl(n);
     int cu = 0, cr = 0, cp = 0, cl = 0;
+    size_t chunk_bytes = 0;
+    g->chunks.clear();
+    g->blocks_update = 0;
     for (int i = 0; i < n; i++) {
         Job* J = g->members[i];
         sun_lattice* L = J->lat;
+        if (g->chunks.empty() || (g->chunks.back().n > 0 && chunk_bytes + L->bytes > kBatchChunkBytes)) {
+            g->chunks.push_back(Group::Chunk{i, 0, 0});
+            chunk_bytes = 0;
+            cu = 0;
+        }
+        chunk_bytes += L->bytes;
         BatchEntry e;
         memset(&e, 0, sizeof(e));
         e.f = L->f;
@@ -227,6 +243,9 @@ int rebuild_tables(Group* g) {
         a.first_block = cu;
         a.nblocks = update_blocks(L->g, 256);
         cu += a.nblocks;
+        g->chunks.back().n++;
+        g->chunks.back().blocks = cu;
+        g->blocks_update += a.nblocks;
         r.first_block = cr;
         r.nblocks = link_blocks(L->g);
         cr += r.nblocks;
@@ -282,7 +301,6 @@ int rebuild_tables(Group* g) {
             g->solo.push_back(m);
         }
     }
-    g->blocks_update = cu;
     g->blocks_reunit = cr;
     g->blocks_plaq = cp;
     g->blocks_poly = cl;
@@ -301,26 +319,33 @@ int rebuild_tables(Group* g) {
 int group_iteration(Group* g) {
     const int n = (int)g->members.size();
     hipStream_t s = g->stream;
-    if (!g->solo.empty()) {
-        for (int parity = 0; parity < 2; parity++)
-            for (int mu = 0; mu < 4; mu++)
-                for (const Group::Solo& m : g->solo)
-                    HIP_TRY(g->ops->update(m.f, m.g, parity, mu, true, 0, m.sweep_dev, m.bnc, m.seed, m.cfg[0], s));
-        for (int k = 0; k < g->n_or; k++)
+    // chunk by chunk (members are independent: each sees its launches in the same order)
+    for (const Group::Chunk& c : g->chunks) {
+        if (!g->solo.empty()) {
             for (int parity = 0; parity < 2; parity++)
                 for (int mu = 0; mu < 4; mu++)
-                    for (const Group::Solo& m : g->solo)
-                        HIP_TRY(g->ops->update(m.f, m.g, parity, mu, false, 0, nullptr, 0.0, 0, m.cfg[1], s));
-    } else {
-        for (int parity = 0; parity < 2; parity++)
-            for (int mu = 0; mu < 4; mu++)
-                HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, true, g->piped[0], g->brick[0],
-                                             g->lds[0], s));
-        for (int k = 0; k < g->n_or; k++)
+                    for (int i = c.first; i < c.first + c.n; i++) {
+                        const Group::Solo& m = g->solo[i];
+                        HIP_TRY(g->ops->update(m.f, m.g, parity, mu, true, 0, m.sweep_dev, m.bnc, m.seed, m.cfg[0], s));
+                    }
+            for (int k = 0; k < g->n_or; k++)
+                for (int parity = 0; parity < 2; parity++)
+                    for (int mu = 0; mu < 4; mu++)
+                        for (int i = c.first; i < c.first + c.n; i++) {
+                            const Group::Solo& m = g->solo[i];
+                            HIP_TRY(g->ops->update(m.f, m.g, parity, mu, false, 0, nullptr, 0.0, 0, m.cfg[1], s));
+                        }
+        } else {
             for (int parity = 0; parity < 2; parity++)
                 for (int mu = 0; mu < 4; mu++)
-                    HIP_TRY(g->ops->batch_update(g->t_update, n, g->blocks_update, parity, mu, false, g->piped[1],
-                                                 g->brick[1], g->lds[1], s));
+                    HIP_TRY(g->ops->batch_update(g->t_update + c.first, c.n, c.blocks, parity, mu, true, g->piped[0],
+                                                 g->brick[0], g->lds[0], s));
+            for (int k = 0; k < g->n_or; k++)
+                for (int parity = 0; parity < 2; parity++)
+                    for (int mu = 0; mu < 4; mu++)
+                        HIP_TRY(g->ops->batch_update(g->t_update + c.first, c.n, c.blocks, parity, mu, false,
+                                                     g->piped[1], g->brick[1], g->lds[1], s));
+        }
     }
     if (g->reunit) HIP_TRY(g->ops->batch_reunitarize(g->t_reunit, n, g->blocks_reunit, s));
     if (!g->solo.empty()) {
