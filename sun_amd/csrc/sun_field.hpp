@@ -268,7 +268,9 @@ SUN_HD void reconstruct8(Mat<R, 3>& m, R th0, R th1) {
 // FP32: the same steps with every multiply-add an explicit fmaf.  Left to the
 // compiler, the products and sums above are fused into FMAs differently depending on
 // the code the decode is inlined into, and the kernels that decode the same link
-// (plain and software-pipelined staples) would round it differently.
+// (plain and software-pipelined staples) would round it differently.  The clamps of
+// the square-root arguments are fmaxf (one v_max_f32 instead of a compare and a
+// select): the same value, as diff is never -0 and a NaN clamps to 0 either way.
 namespace r8f {
 SUN_HD cplx<float> mul(cplx<float> a, cplx<float> b) {  // a*b
     return {fmaf(-a.im, b.im, a.re * b.re), fmaf(a.im, b.re, a.re * b.im)};
@@ -296,7 +298,7 @@ SUN_HD void reconstruct8(Mat<float, 3>& m, float th0, float th1) {
     p1 = fmaf(m.e[0][2].im, m.e[0][2].im, p1);
     const float p2 = decode_rcp(p1);
     float diff = 1.f - p1;
-    p1 = decode_sqrt(diff >= 0.f ? diff : 0.f);
+    p1 = decode_sqrt(fmaxf(diff, 0.f));
     float c0, s0, c1, s1;
     phase(th0, s0, c0);
     phase(th1, s1, c1);
@@ -306,7 +308,7 @@ SUN_HD void reconstruct8(Mat<float, 3>& m, float th0, float th1) {
     diff = fmaf(-m.e[0][0].im, m.e[0][0].im, diff);
     diff = fmaf(-m.e[1][0].re, m.e[1][0].re, diff);
     diff = fmaf(-m.e[1][0].im, m.e[1][0].im, diff);
-    p1 = decode_sqrt(diff >= 0.f ? diff : 0.f);
+    p1 = decode_sqrt(fmaxf(diff, 0.f));
     m.e[2][0].re = p1 * c1;
     m.e[2][0].im = p1 * s1;
     const cplx<float> t22 = mul_ca(m.e[0][0], m.e[1][0]);
