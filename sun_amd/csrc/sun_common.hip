@@ -29,10 +29,25 @@ __device__ __forceinline__ double2 block_sum_all(double2 v) {
     return r;
 }
 
-// thread t sums partials t, t+256, t+512, ... in order; then the block tree
+// thread t sums partials t, t+256, t+512, ... in order; then the block tree.  The
+// loads are issued kSumBatch at a time before the additions (which keep their order),
+// so a thread waits for one memory latency per batch instead of one per partial.
+constexpr int kSumBatch = 8;
+
 __device__ __forceinline__ double2 ordered_sum(const double2* p, int n) {
     double2 acc = make_double2(0.0, 0.0);
-    for (int i = threadIdx.x; i < n; i += kReduceBlock) {
+    int i = threadIdx.x;
+    for (; i + (kSumBatch - 1) * kReduceBlock < n; i += kSumBatch * kReduceBlock) {
+        double2 v[kSumBatch];
+#pragma unroll
+        for (int k = 0; k < kSumBatch; k++) v[k] = p[i + k * kReduceBlock];
+#pragma unroll
+        for (int k = 0; k < kSumBatch; k++) {
+            acc.x += v[k].x;
+            acc.y += v[k].y;
+        }
+    }
+    for (; i < n; i += kReduceBlock) {
         acc.x += p[i].x;
         acc.y += p[i].y;
     }
@@ -110,7 +125,15 @@ __global__ void __launch_bounds__(kReduceBlock) k_wl_finalize(const double2* par
 __global__ void __launch_bounds__(kReduceBlock) k_wlop_finalize(const double* partials, int nb, double* accum) {
     const double* p = partials + (size_t)blockIdx.x * nb;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < nb; i += kReduceBlock) acc += p[i];
+    int i = threadIdx.x;
+    for (; i + (kSumBatch - 1) * kReduceBlock < nb; i += kSumBatch * kReduceBlock) {
+        double v[kSumBatch];
+#pragma unroll
+        for (int k = 0; k < kSumBatch; k++) v[k] = p[i + k * kReduceBlock];
+#pragma unroll
+        for (int k = 0; k < kSumBatch; k++) acc += v[k];
+    }
+    for (; i < nb; i += kReduceBlock) acc += p[i];
     const double2 v = block_sum_all(make_double2(acc, 0.0));
     if (threadIdx.x == 0) accum[blockIdx.x] += v.x;
 }
