@@ -462,6 +462,23 @@ def served_leg(sun_amd, ctx, n_jobs, util=0.75, budget=64 << 30):
                       "member, and the fused plaquette + Polyakov measurement, captured in a hipGraph), HB + 4 OR "
                       "+ reunit + plaquette + Polyakov per iteration; 220 iterations = a 260-iteration job minus a "
                       "40-iteration one (best of two each), start to end"}
+    # the generation driver on the same lattice description right after those jobs:
+    # the device runs ~1% slower a minute into sustained work than at the start of the
+    # run, where the headline is measured, so the served figure is set beside the
+    # driver's at the same point
+    gs = sun_amd.new_stream()
+    with sun_amd.Lattice(ctx, 3, (32, 32, 32, 32), 6.0, 12, 4) as gl:
+        gl.cold_start(gs)
+        gl.generate(seed=2, n_iter=20, n_or=4, reunit_every=1, stream=gs)
+        sun_amd.sync(gs)
+        t0 = time.perf_counter()
+        gl.generate(seed=2, n_iter=n_steady, n_or=4, reunit_every=1, first_sweep=20, stream=gs)
+        sun_amd.sync(gs)
+        gen_s = time.perf_counter() - t0
+    steady["generate_same_point"] = {"s": gen_s,
+                                     "iteration_frac": n_steady * it_bytes / gen_s / (PEAK_HBM_GBS * 1e9),
+                                     "note": "sun_generate, 220 iterations of the same 32^4 FP32-12 description "
+                                             "(context launch shapes), run right after the steady-state jobs"}
     est = [j["n_iter"] * 5 * 4 * int(np.prod(j["dims"])) / calib[(j["nc"], j["dims"])] for j in jobs]
     rate = util / float(np.mean(est))
     gaps = rng.exponential(1.0 / rate, n_jobs)
