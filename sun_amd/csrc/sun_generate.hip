@@ -6,6 +6,7 @@
 // graph itself advances, so every replay continues the Markov chain.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <functional>
 #include <vector>
@@ -144,15 +145,83 @@ extern "C" int sun_generate(sun_lattice* lat, const sun_generate_desc* d, void* 
 static void batch_free(sun_batch* b) {
     if (!b) return;
     hipSetDevice(b->ctx->device);
-    if (b->t_update) hipFree(b->t_update);
-    if (b->t_reunit) hipFree(b->t_reunit);
-    if (b->t_plaq) hipFree(b->t_plaq);
+    for (sun_batch::Schedule* sc : {&b->gen, &b->step}) {
+        if (sc->t_update) hipFree(sc->t_update);
+        if (sc->t_reunit) hipFree(sc->t_reunit);
+        if (sc->t_plaq) hipFree(sc->t_plaq);
+    }
     if (b->partials) hipFree(b->partials);
     if (b->d_out) hipFree(b->d_out);
     if (b->h_out) hipHostFree(b->h_out);
     if (b->obs_dev) hipFree(b->obs_dev);
     if (b->iter_dev) hipFree(b->iter_dev);
+    for (int k = 1; k < kBatchBranches; k++) {
+        if (b->branch[k]) hipStreamDestroy(b->branch[k]);
+        if (b->join[k]) hipEventDestroy(b->join[k]);
+    }
+    if (b->fork) hipEventDestroy(b->fork);
     delete b;
+}
+
+// the chunks and tables of one schedule: chunks of consecutive lattices within budget
+// link bytes; entries with first_block counted from their chunk's start
+static void build_schedule(sun_batch::Schedule& sc, sun_lattice* const* lats, int n, const uint64_t* seeds,
+                           size_t budget, int branches, std::vector<BatchEntry>& tr, std::vector<BatchEntry>& tp) {
+    std::vector<BatchEntry> tu(n);
+    tr.assign(n, BatchEntry{});
+    tp.assign(n, BatchEntry{});
+    sc.branches = branches;
+    int cu = 0, cr = 0, cp = 0;
+    size_t chunk_bytes = 0;
+    for (int i = 0; i < n; i++) {
+        sun_lattice* L = lats[i];
+        if (sc.chunks.empty() || (sc.chunks.back().n > 0 && chunk_bytes + L->bytes > budget)) {
+            sc.chunks.push_back(sun_batch::Chunk{i, 0, 0, 0, 0});
+            chunk_bytes = 0;
+            cu = cr = cp = 0;
+        }
+        chunk_bytes += L->bytes;
+        BatchEntry e;
+        memset(&e, 0, sizeof(e));
+        e.f = L->f;
+        e.g = L->g;
+        e.seed = seeds ? seeds[i] : 0;
+        e.bnc = L->d.beta / L->d.nc;
+        e.sweep = 0;
+        batch_brick(L->g, e.ub1, e.ub2, e.ub3, e.spt);
+        BatchEntry a = e, r = e, p = e;
+        sun_batch::Chunk& c = sc.chunks.back();
+        c.n++;
+        a.first_block = cu;
+        a.nblocks = update_blocks(L->g, 256);
+        cu += a.nblocks;
+        c.blocks = cu;
+        r.first_block = cr;
+        r.nblocks = link_blocks(L->g);
+        cr += r.nblocks;
+        c.blocks_reunit = cr;
+        p.first_block = cp;
+        p.nblocks = plaquette_blocks(L->g);
+        p.scale = 1.0 / ((double)(3 * L->d.nc) * (double)L->g.V);  // plaquette_cub.cu:234
+        cp += p.nblocks;
+        c.blocks_plaq = cp;
+        tu[i] = a;
+        tr[i] = r;
+        tp[i] = p;
+    }
+    sc.h_update = tu;
+}
+
+static hipError_t upload_schedule(sun_batch::Schedule& sc, const std::vector<BatchEntry>& tr,
+                                  const std::vector<BatchEntry>& tp) {
+    const size_t bytes = sizeof(BatchEntry) * sc.h_update.size();
+    hipError_t e = hipMalloc(&sc.t_update, bytes);
+    if (e == hipSuccess) e = hipMalloc(&sc.t_reunit, bytes);
+    if (e == hipSuccess) e = hipMalloc(&sc.t_plaq, bytes);
+    if (e == hipSuccess) e = hipMemcpy(sc.t_update, sc.h_update.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(sc.t_reunit, tr.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(sc.t_plaq, tp.data(), bytes, hipMemcpyHostToDevice);
+    return e;
 }
 
 static int sun_batch_create_body(sun_ctx* ctx, int n, sun_lattice* const* lats, const uint64_t* seeds,
@@ -188,64 +257,30 @@ static int sun_batch_create_body(sun_ctx* ctx, int n, sun_lattice* const* lats, 
         b->lds[k] = default_batch_lds(ops, k);
     }
     b->lats.assign(lats, lats + n);
-    std::vector<BatchEntry> tu(n), tr(n), tp(n);
-    std::vector<int> poff(n);  // each lattice's plaquette partials in b->partials
-    int cu = 0, cr = 0, cp = 0, cp_all = 0;
-    size_t chunk_bytes = 0;
+    // each lattice's plaquette partials in b->partials (the same in both schedules)
+    std::vector<int> poff(n);
+    int cp_all = 0;
     for (int i = 0; i < n; i++) {
-        sun_lattice* L = lats[i];
-        if (b->chunks.empty() || (b->chunks.back().n > 0 && chunk_bytes + L->bytes > kBatchChunkBytes)) {
-            b->chunks.push_back(sun_batch::Chunk{i, 0, 0, 0, 0});
-            chunk_bytes = 0;
-            cu = cr = cp = 0;
-        }
-        chunk_bytes += L->bytes;
-        BatchEntry e;
-        memset(&e, 0, sizeof(e));
-        e.f = L->f;
-        e.g = L->g;
-        e.seed = seeds ? seeds[i] : 0;
-        e.bnc = L->d.beta / L->d.nc;
-        e.sweep = 0;
-        batch_brick(L->g, e.ub1, e.ub2, e.ub3, e.spt);
-        BatchEntry a = e, r = e, p = e;
-        a.first_block = cu;
-        a.nblocks = update_blocks(L->g, 256);
-        cu += a.nblocks;
-        b->chunks.back().n++;
-        b->chunks.back().blocks = cu;
-        b->blocks_update += a.nblocks;
-        r.first_block = cr;
-        r.nblocks = link_blocks(L->g);
-        cr += r.nblocks;
-        b->chunks.back().blocks_reunit = cr;
-        b->blocks_reunit += r.nblocks;
-        p.first_block = cp;
-        p.nblocks = plaquette_blocks(L->g);
-        p.scale = 1.0 / ((double)(3 * L->d.nc) * (double)L->g.V);  // plaquette_cub.cu:234
-        cp += p.nblocks;
-        b->chunks.back().blocks_plaq = cp;
         poff[i] = cp_all;
-        cp_all += p.nblocks;
-        tu[i] = a;
-        tr[i] = r;
-        tp[i] = p;
+        cp_all += plaquette_blocks(lats[i]->g);
     }
-    b->blocks_plaq = cp_all;
-    b->h_update = tu;
+    std::vector<BatchEntry> tr_gen, tp_gen, tr_step, tp_step;
+    build_schedule(b->gen, lats, n, seeds, kBatchChunkBytes, kBatchBranches, tr_gen, tp_gen);
+    build_schedule(b->step, lats, n, seeds, kCacheChunkBytes, 1, tr_step, tp_step);
     hipError_t e = hipSetDevice(ctx->device);
-    if (e == hipSuccess) e = hipMalloc(&b->t_update, sizeof(BatchEntry) * n);
-    if (e == hipSuccess) e = hipMalloc(&b->t_reunit, sizeof(BatchEntry) * n);
-    if (e == hipSuccess) e = hipMalloc(&b->t_plaq, sizeof(BatchEntry) * n);
     if (e == hipSuccess) e = hipMalloc(&b->partials, sizeof(double2) * (size_t)cp_all);
     if (e == hipSuccess) e = hipMalloc(&b->d_out, sizeof(double2) * n);
     if (e == hipSuccess)
-        for (int i = 0; i < n; i++) tp[i].partials = b->partials + poff[i];
+        for (int i = 0; i < n; i++) tp_gen[i].partials = tp_step[i].partials = b->partials + poff[i];
     if (e == hipSuccess) e = hipMalloc(&b->iter_dev, sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(&b->h_out, sizeof(double2) * n, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMemcpy(b->t_update, tu.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->t_reunit, tr.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->t_plaq, tp.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = upload_schedule(b->gen, tr_gen, tp_gen);
+    if (e == hipSuccess) e = upload_schedule(b->step, tr_step, tp_step);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (int k = 1; k < kBatchBranches && e == hipSuccess; k++) {
+        e = hipStreamCreateWithFlags(&b->branch[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         hipGetLastError();
         batch_free(b);
@@ -290,73 +325,85 @@ static hipStream_t batch_stream(sun_batch* b, void* stream) {
     return stream ? (hipStream_t)stream : b->ctx->stream;
 }
 
-static int batch_set_sweeps(sun_batch* b, const uint32_t* sweeps, uint32_t uniform, hipStream_t s) {
+static int batch_set_sweeps(sun_batch* b, sun_batch::Schedule& sc, const uint32_t* sweeps, uint32_t uniform,
+                            hipStream_t s) {
     const int n = (int)b->lats.size();
-    for (int i = 0; i < n; i++) b->h_update[i].sweep = sweeps ? sweeps[i] : uniform;
-    HIP_TRY(hipMemcpyAsync(b->t_update, b->h_update.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < n; i++) sc.h_update[i].sweep = sweeps ? sweeps[i] : uniform;
+    HIP_TRY(hipMemcpyAsync(sc.t_update, sc.h_update.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     return SUN_OK;
 }
 
+using Schedule = sun_batch::Schedule;
+using Chunk = sun_batch::Chunk;
+
 // one heatbath (hb) or overrelaxation sweep of one update chunk
-static int chunk_sweep(sun_batch* b, const sun_batch::Chunk& c, bool hb, hipStream_t s) {
+static int chunk_sweep(sun_batch* b, const Schedule& sc, const Chunk& c, bool hb, hipStream_t s) {
     const int k = hb ? 0 : 1;
     for (int parity = 0; parity < 2; parity++)
         for (int mu = 0; mu < 4; mu++)
-            HIP_TRY(b->ops->batch_update(b->t_update + c.first, c.n, c.blocks, parity, mu, hb, b->piped[k], b->brick[k],
-                                         b->lds[k], s));
+            HIP_TRY(b->ops->batch_update(sc.t_update + c.first, c.n, c.blocks, parity, mu, hb, b->piped[k],
+                                         b->brick[k], b->lds[k], s));
     return SUN_OK;
 }
 
-static int batch_hb(sun_batch* b, hipStream_t s) {
-    for (const sun_batch::Chunk& c : b->chunks) SUN_TRY(chunk_sweep(b, c, true, s));
+static int chunk_reunit(sun_batch* b, const Schedule& sc, const Chunk& c, hipStream_t s) {
+    HIP_TRY(b->ops->batch_reunitarize(sc.t_reunit + c.first, c.n, c.blocks_reunit, s));
     return SUN_OK;
 }
 
-static int batch_or(sun_batch* b, hipStream_t s) {
-    for (const sun_batch::Chunk& c : b->chunks) SUN_TRY(chunk_sweep(b, c, false, s));
+static int chunk_plaq(sun_batch* b, const Schedule& sc, const Chunk& c, hipStream_t s) {
+    HIP_TRY(b->ops->batch_plaquette_partial(sc.t_plaq + c.first, c.n, c.blocks_plaq, s));
+    HIP_TRY(launch_batch_finalize(sc.t_plaq + c.first, c.n, b->d_out + c.first, s));
     return SUN_OK;
 }
 
-static int chunk_reunit(sun_batch* b, const sun_batch::Chunk& c, hipStream_t s) {
-    HIP_TRY(b->ops->batch_reunitarize(b->t_reunit + c.first, c.n, c.blocks_reunit, s));
-    return SUN_OK;
-}
-
-static int chunk_plaq(sun_batch* b, const sun_batch::Chunk& c, hipStream_t s) {
-    HIP_TRY(b->ops->batch_plaquette_partial(b->t_plaq + c.first, c.n, c.blocks_plaq, s));
-    HIP_TRY(launch_batch_finalize(b->t_plaq + c.first, c.n, b->d_out + c.first, s));
-    return SUN_OK;
-}
-
-static int batch_plaq(sun_batch* b, hipStream_t s) {
-    for (const sun_batch::Chunk& c : b->chunks) SUN_TRY(chunk_plaq(b, c, s));
+// body(chunk, stream) for every chunk of the schedule, chunk i on branch
+// i % sc.branches: branch 0 is s itself; the other branches first wait for the work
+// already on s, and s waits for them at the end, so to the caller everything ran on s.
+// A chunk's lattices are in no other chunk, so each lattice's launches keep their order.
+template <typename Body>
+static int for_chunks(sun_batch* b, const Schedule& sc, hipStream_t s, Body&& body) {
+    const int nb = std::min<int>(sc.branches, (int)sc.chunks.size());
+    if (nb > 1) {
+        HIP_TRY(hipEventRecord(b->fork, s));
+        for (int k = 1; k < nb; k++) HIP_TRY(hipStreamWaitEvent(b->branch[k], b->fork, 0));
+    }
+    for (size_t i = 0; i < sc.chunks.size(); i++) {
+        const int k = (int)(i % (size_t)nb);
+        SUN_TRY(body(sc.chunks[i], k == 0 ? s : b->branch[k]));
+    }
+    for (int k = 1; k < nb; k++) {
+        HIP_TRY(hipEventRecord(b->join[k], b->branch[k]));
+        HIP_TRY(hipStreamWaitEvent(s, b->join[k], 0));
+    }
     return SUN_OK;
 }
 
 extern "C" int sun_batch_heatbath(sun_batch* b, const uint32_t* sweeps, void* stream) {
     if (!b || !sweeps) return fail(SUN_ERR_INVALID, "sun_batch_heatbath: NULL argument");
     hipStream_t s = batch_stream(b, stream);
-    SUN_TRY(batch_set_sweeps(b, sweeps, 0, s));
-    return batch_hb(b, s);
+    SUN_TRY(batch_set_sweeps(b, b->step, sweeps, 0, s));
+    return for_chunks(b, b->step, s, [&](const Chunk& c, hipStream_t cs) { return chunk_sweep(b, b->step, c, true, cs); });
 }
 
 extern "C" int sun_batch_overrelax(sun_batch* b, void* stream) {
     if (!b) return fail(SUN_ERR_INVALID, "sun_batch_overrelax: NULL");
-    return batch_or(b, batch_stream(b, stream));
+    return for_chunks(b, b->step, batch_stream(b, stream),
+                      [&](const Chunk& c, hipStream_t cs) { return chunk_sweep(b, b->step, c, false, cs); });
 }
 
 extern "C" int sun_batch_reunitarize(sun_batch* b, void* stream) {
     if (!b) return fail(SUN_ERR_INVALID, "sun_batch_reunitarize: NULL");
-    for (const sun_batch::Chunk& c : b->chunks) SUN_TRY(chunk_reunit(b, c, batch_stream(b, stream)));
-    return SUN_OK;
+    return for_chunks(b, b->step, batch_stream(b, stream),
+                      [&](const Chunk& c, hipStream_t cs) { return chunk_reunit(b, b->step, c, cs); });
 }
 
 extern "C" int sun_batch_plaquette(sun_batch* b, void* stream, double* out) {
     if (!b || !out) return fail(SUN_ERR_INVALID, "sun_batch_plaquette: NULL");
     hipStream_t s = batch_stream(b, stream);
     const int n = (int)b->lats.size();
-    SUN_TRY(batch_plaq(b, s));
+    SUN_TRY(for_chunks(b, b->step, s, [&](const Chunk& c, hipStream_t cs) { return chunk_plaq(b, b->step, c, cs); }));
     HIP_TRY(hipMemcpyAsync(b->h_out, b->d_out, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     for (int i = 0; i < n; i++) {
@@ -366,20 +413,22 @@ extern "C" int sun_batch_plaquette(sun_batch* b, void* stream, double* out) {
     return SUN_OK;
 }
 
-// one iteration: chunk by chunk, the heatbath sweep, n_or overrelaxation sweeps,
-// reunitarisation and plaquette back to back (lattices are independent, so every
-// lattice sees the same launches in the same order as without chunks); then the
-// history row of the whole batch
+// one iteration: chunk by chunk (over the branches of for_chunks), the heatbath sweep,
+// n_or overrelaxation sweeps, reunitarisation and plaquette back to back (lattices are
+// independent, so every lattice sees the same launches in the same order as without
+// chunks); then the history row of the whole batch
 static int batch_iteration(sun_batch* b, int n_or, bool reunit, bool measure, hipStream_t s) {
     const int n = (int)b->lats.size();
-    for (const sun_batch::Chunk& c : b->chunks) {
-        SUN_TRY(chunk_sweep(b, c, true, s));
-        for (int k = 0; k < n_or; k++) SUN_TRY(chunk_sweep(b, c, false, s));
-        if (reunit) SUN_TRY(chunk_reunit(b, c, s));
-        if (measure) SUN_TRY(chunk_plaq(b, c, s));
-    }
+    const Schedule& sc = b->gen;
+    SUN_TRY(for_chunks(b, sc, s, [&](const Chunk& c, hipStream_t cs) {
+        SUN_TRY(chunk_sweep(b, sc, c, true, cs));
+        for (int k = 0; k < n_or; k++) SUN_TRY(chunk_sweep(b, sc, c, false, cs));
+        if (reunit) SUN_TRY(chunk_reunit(b, sc, c, cs));
+        if (measure) SUN_TRY(chunk_plaq(b, sc, c, cs));
+        return SUN_OK;
+    }));
     if (measure) HIP_TRY(launch_record_obs(b->d_out, n, b->obs_dev, b->iter_dev, 0, s));
-    HIP_TRY(launch_batch_advance(b->t_update, n, s));
+    HIP_TRY(launch_batch_advance(sc.t_update, n, s));
     HIP_TRY(launch_advance_counter(b->iter_dev, s));
     return SUN_OK;
 }
@@ -392,7 +441,7 @@ static int sun_batch_generate_body(sun_batch* b, uint32_t first_sweep, int n_ite
     const int n = (int)b->lats.size();
     const bool measure = obs != nullptr;
     if (measure) SUN_TRY(ensure_obs(&b->obs_dev, &b->obs_cap, n * n_iter));
-    SUN_TRY(batch_set_sweeps(b, nullptr, first_sweep, s));
+    SUN_TRY(batch_set_sweeps(b, b->gen, nullptr, first_sweep, s));
     const uint32_t zero = 0;
     HIP_TRY(hipMemcpyAsync(b->iter_dev, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -412,7 +461,7 @@ static int sun_batch_generate_body(sun_batch* b, uint32_t first_sweep, int n_ite
         }
     }
     // keep the host copy of the sweep counters in step with the device table
-    for (int i = 0; i < n; i++) b->h_update[i].sweep = first_sweep + (uint32_t)n_iter;
+    for (int i = 0; i < n; i++) b->gen.h_update[i].sweep = first_sweep + (uint32_t)n_iter;
     if (measure) {
         std::vector<double2> h((size_t)n * n_iter);
         HIP_TRY(hipMemcpyAsync(h.data(), b->obs_dev, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, s));

@@ -127,35 +127,54 @@ struct sun_lattice {
     double2* hist = nullptr;       // observable history (server jobs), in the arena
 };
 
-// Link bytes of one batched update chunk: the lattices of a chunk stay in the GPU's
-// 256 MB memory-side cache (MI355X Infinity Cache) through the chunk's consecutive
-// launches, so the links a launch reads again (neighbours shared across workgroups and
-// XCDs, the same field in the next (parity, mu) launch) come from there instead of
-// HBM.  24^4 FP32-12 lattices, one overrelaxation sweep: 4 in one batch (256 MB) 0.62
-// of 8 TB/s, 18 (1.15 GB) 0.55.  The 64-lattice benchmark mix (1.36 GB) per
-// iteration, by chunk budget: none 9.7-9.8e9 link updates/s, 96 MB 8.5e9, 128 MB
-// 9.0e9, 192 MB 9.9-10.1e9, 256 MB 10.3-10.6e9, 320 MB 10.1-10.3e9, 384 MB 9.6-9.8e9
-// (smaller chunks: launches too small to fill the GPU).
-constexpr size_t kBatchChunkBytes = size_t(256) << 20;
+// Link bytes of one update chunk of a job-server group: the lattices of a chunk stay
+// in the GPU's 256 MB memory-side cache (MI355X Infinity Cache) through the chunk's
+// consecutive launches, so the links a launch reads again (neighbours shared across
+// workgroups and XCDs, the same field in the next (parity, mu) launch) come from there
+// instead of HBM.  24^4 FP32-12 lattices, one overrelaxation sweep: 4 in one batch
+// (256 MB) 0.62 of 8 TB/s, 18 (1.15 GB) 0.55.  The 64-lattice benchmark mix (1.36 GB)
+// per iteration, chunks run one after another, by chunk budget: none 9.7-9.8e9 link
+// updates/s, 96 MB 8.5e9, 128 MB 9.0e9, 192 MB 9.9-10.1e9, 256 MB 10.3-10.6e9, 320 MB
+// 10.1-10.3e9, 384 MB 9.6-9.8e9 (smaller chunks: launches too small to fill the GPU).
+constexpr size_t kCacheChunkBytes = size_t(256) << 20;
+
+// sun_batch runs its chunks on kBatchBranches streams at once (chunk i on branch
+// i % kBatchBranches), so one chunk's launches fill the tails of another's, with
+// chunks of a third of the cache: the chunks in flight together still fit it.  The
+// 64-lattice mix per iteration (ms, one box): one branch of 256 MB chunks 15.3; two
+// branches of 96 / 128 / 160 MB 15.1 / 14.9 / 15.7, of 256 MB 17.3; three of 72 / 85 /
+// 100 MB 14.5-14.7; four of 56 / 64 MB 15.0-15.3; five of 51 MB 16.6-16.8.
+constexpr int kBatchBranches = 3;
+constexpr size_t kBatchChunkBytes = kCacheChunkBytes / kBatchBranches;
 
 struct sun_batch {
     sun_ctx* ctx = nullptr;
     std::vector<sun_lattice*> lats;
     const sun::OpsTable* ops = nullptr;
-    // device tables: update blocks (half/256), reunit blocks (4V/256), plaquette blocks (V/256)
-    sun::BatchEntry* t_update = nullptr;
-    sun::BatchEntry* t_reunit = nullptr;
-    sun::BatchEntry* t_plaq = nullptr;
-    int blocks_update = 0, blocks_reunit = 0, blocks_plaq = 0;
-    // chunks: consecutive lattices whose link fields together stay within
-    // kBatchChunkBytes (one larger lattice is a chunk of its own); an iteration runs
-    // each chunk's heatbath and overrelaxation sweeps, reunitarisation and plaquette
-    // back to back, while its fields sit in the memory-side cache.  The tables'
-    // first_block counts from the chunk's start; blocks_* are the chunk's totals.
+    // chunks: consecutive lattices whose link fields together stay within a budget (one
+    // larger lattice is a chunk of its own), run back to back while their fields sit in
+    // the memory-side cache.  blocks_* are the chunk's update (half/256), reunit
+    // (4V/256) and plaquette (V/256) blocks.
     struct Chunk {
         int first = 0, n = 0, blocks = 0, blocks_reunit = 0, blocks_plaq = 0;
     };
-    std::vector<Chunk> chunks;
+    // one chunking of the batch: its chunks, the device tables of its lattices (whose
+    // first_block counts from the start of the lattice's chunk), the host copy of the
+    // update table, and the number of streams its chunks run on
+    struct Schedule {
+        std::vector<Chunk> chunks;
+        sun::BatchEntry* t_update = nullptr;
+        sun::BatchEntry* t_reunit = nullptr;
+        sun::BatchEntry* t_plaq = nullptr;
+        std::vector<sun::BatchEntry> h_update;
+        int branches = 1;
+    };
+    // sun_batch_generate: chunks of kBatchChunkBytes on kBatchBranches streams (an
+    // iteration's heatbath, overrelaxation, reunitarisation and plaquette launches of
+    // different chunks overlap); the single-sweep entry points: chunks of
+    // kCacheChunkBytes one after another (one overrelaxation sweep alone ran at 0.65 of
+    // 8 TB/s that way and 0.60 on the branches of generate's chunks)
+    Schedule gen, step;
     int piped[2] = {0, 0};  // staple kernel of the heatbath / overrelaxation launches
     int brick[2] = {0, 0};  // their lattices' update bricks instead of linear order
     int lds[2] = {0, 0};    // unused dynamic LDS per workgroup (occupancy cap)
@@ -166,6 +185,11 @@ struct sun_batch {
     double2* obs_dev = nullptr;
     int obs_cap = 0;
     uint32_t* iter_dev = nullptr;
+    // branches 1.. of the chunk schedule (branch 0 is the caller's stream) and the
+    // events that fork them from it and join them back
+    hipStream_t branch[kBatchBranches] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kBatchBranches] = {};
 };
 
 namespace sun {

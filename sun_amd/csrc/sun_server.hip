@@ -118,7 +118,7 @@ struct Group {
         LaunchCfg cfg[2];
     };
     std::vector<Solo> solo;
-    // update chunks (sun_batch's kBatchChunkBytes): consecutive members whose link
+    // update chunks (kCacheChunkBytes): consecutive members whose link
     // fields together fit the memory-side cache run their heatbath and overrelaxation
     // launches back to back; t_update's first_block counts from the chunk's start
     struct Chunk {
@@ -223,7 +223,7 @@ int rebuild_tables(Group* g) {
     for (int i = 0; i < n; i++) {
         Job* J = g->members[i];
         sun_lattice* L = J->lat;
-        if (g->chunks.empty() || (g->chunks.back().n > 0 && chunk_bytes + L->bytes > kBatchChunkBytes)) {
+        if (g->chunks.empty() || (g->chunks.back().n > 0 && chunk_bytes + L->bytes > kCacheChunkBytes)) {
             g->chunks.push_back(Group::Chunk{i, 0, 0});
             chunk_bytes = 0;
             cu = 0;

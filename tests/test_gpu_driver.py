@@ -162,13 +162,17 @@ def test_batch_bitwise_equals_alone(ctx, storage, prec):
         lat.close()
 
 
-def test_batch_chunks_bitwise_equal_alone(ctx):
-    """A batch runs in chunks of consecutive lattices with at most 256 MB of links; a
-    lattice larger than that is a chunk of its own.  Here 8^4 + 16^4 (one chunk), a
-    302 MB 32^3x48 lattice (its own chunk), 12^4 + 8^4 (a third chunk): every lattice's
-    observables and links equal its alone run bit for bit."""
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_chunks_bitwise_equal_alone(ctx, use_graph):
+    """A batch runs in chunks of consecutive lattices with at most a third of 256 MB of
+    links, three chunks at a time on three streams (chunk i on stream i % 3); a lattice
+    larger than that is a chunk of its own.  Here 8^4 + 16^4, a 302 MB 32^3x48 lattice,
+    12^4 + 8^4 + 24^4, then two more 24^4 (64 MB each): five chunks, so two of the
+    streams take two chunks each.  Every lattice's observables and links equal its alone
+    run bit for bit, with the iterations captured in a graph and launched eagerly."""
     specs = [((8, 8, 8, 8), 5.8, 41), ((16, 16, 16, 16), 6.0, 42), ((32, 32, 32, 48), 6.1, 43),
-             ((12, 12, 12, 12), 6.2, 44), ((8, 8, 8, 8), 6.3, 45)]
+             ((12, 12, 12, 12), 6.2, 44), ((8, 8, 8, 8), 6.3, 45), ((24, 24, 24, 24), 5.9, 46),
+             ((24, 24, 24, 24), 6.05, 47), ((24, 24, 24, 24), 6.15, 48)]
     alone = []
     for dims, beta, seed in specs:
         with sun_amd.Lattice(ctx, 3, dims, beta, 12, 4) as lat:
@@ -181,7 +185,7 @@ def test_batch_chunks_bitwise_equal_alone(ctx):
         lat.hot_start(seed)
         lats.append(lat)
     b = sun_amd.Batch(ctx, lats, [s[2] for s in specs])
-    obs = b.generate(first_sweep=0, n_iter=2, n_or=4, reunit_every=1, use_graph=True)
+    obs = b.generate(first_sweep=0, n_iter=2, n_or=4, reunit_every=1, use_graph=use_graph)
     for i, lat in enumerate(lats):
         assert np.array_equal(obs[:, i, :], alone[i][0]), i
         assert np.array_equal(lat.to_host(), alone[i][1]), i
