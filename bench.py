@@ -342,6 +342,14 @@ def batched_leg(sun_amd, ctx, stream, n_lat=64, n_iter=30, warm=3):
     or_sweep_ms = t.ms() / reps
     t.close()
     or_gbs = 9 * link_bytes(3, 12, 4) * 4 * sites / (or_sweep_ms * 1e-3) / 1e9
+    # the single 32^4 FP32-12 lattice's overrelaxation launch (context launch shapes)
+    # timed right after: the device runs a few % slower here, after the legs before
+    # this one, than where the headline is measured
+    with sun_amd.Lattice(ctx, 3, (32, 32, 32, 32), 6.0, 12, 4) as sl:
+        sl.cold_start(stream)
+        sl.generate(seed=2, n_iter=2, n_or=4, reunit_every=1, stream=stream)
+        single_ms = time_launches(sun_amd, sl, stream, hb=False, reps=32)
+    single_frac = 9 * link_bytes(3, 12, 4) * (32 ** 4 // 2) / (single_ms * 1e-3) / 1e9 / PEAK_HBM_GBS
     shapes = {k: dict(zip(("piped", "brick", "lds"), batch.get_launch(i)))
               for i, k in enumerate(("heatbath", "overrelax"))}
     batch.close()
@@ -350,6 +358,7 @@ def batched_leg(sun_amd, ctx, stream, n_lat=64, n_iter=30, warm=3):
     return {"value": 5 * 4 * sites * n_iter / dt, "unit": "link-updates/s", "lattices": n_lat,
             "total_sites": sites, "iterations": n_iter, "ms_per_iteration": dt / n_iter * 1e3,
             "overrelax_sweep_ms": or_sweep_ms, "overrelax_gbs": or_gbs, "overrelax_frac": or_gbs / PEAK_HBM_GBS,
+            "single_32x4_overrelax_frac_same_point": single_frac,
             "launch_shapes": shapes,
             "plaquette_range_last": [float(obs[-1, :, :].mean(axis=1).min()),
                                      float(obs[-1, :, :].mean(axis=1).max())],
