@@ -355,8 +355,12 @@ def batched_leg(sun_amd, ctx, stream, n_lat=64, n_iter=30, warm=3):
     batch.close()
     for l in lats:
         l.close()
+    # algorithmic bytes of an iteration: HB + 4 OR (9 link transfers per update, 5 x 4
+    # updates per site), reunitarisation (2 per link) and plaquette (1 per link)
+    it_bytes = 4 * sites * link_bytes(3, 12, 4) * (45 + 2 + 1)
     return {"value": 5 * 4 * sites * n_iter / dt, "unit": "link-updates/s", "lattices": n_lat,
             "total_sites": sites, "iterations": n_iter, "ms_per_iteration": dt / n_iter * 1e3,
+            "iteration_frac": it_bytes * n_iter / dt / 1e9 / PEAK_HBM_GBS,
             "overrelax_sweep_ms": or_sweep_ms, "overrelax_gbs": or_gbs, "overrelax_frac": or_gbs / PEAK_HBM_GBS,
             "single_32x4_overrelax_frac_same_point": single_frac,
             "launch_shapes": shapes,
