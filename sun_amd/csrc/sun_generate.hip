@@ -369,15 +369,20 @@ static int for_chunks(sun_batch* b, const Schedule& sc, hipStream_t s, Body&& bo
         HIP_TRY(hipEventRecord(b->fork, s));
         for (int k = 1; k < nb; k++) HIP_TRY(hipStreamWaitEvent(b->branch[k], b->fork, 0));
     }
-    for (size_t i = 0; i < sc.chunks.size(); i++) {
+    int rc = SUN_OK;
+    for (size_t i = 0; i < sc.chunks.size() && rc == SUN_OK; i++) {
         const int k = (int)(i % (size_t)nb);
-        SUN_TRY(body(sc.chunks[i], k == 0 ? s : b->branch[k]));
+        rc = body(sc.chunks[i], k == 0 ? s : b->branch[k]);
     }
+    // joined also after a failed launch, so no branch is left outside s (or outside a
+    // capture begun on s)
     for (int k = 1; k < nb; k++) {
-        HIP_TRY(hipEventRecord(b->join[k], b->branch[k]));
-        HIP_TRY(hipStreamWaitEvent(s, b->join[k], 0));
+        const hipError_t e1 = hipEventRecord(b->join[k], b->branch[k]);
+        const hipError_t e2 = hipStreamWaitEvent(s, b->join[k], 0);
+        if (rc == SUN_OK && (e1 != hipSuccess || e2 != hipSuccess))
+            rc = fail(SUN_ERR_HIP, "batch branch join: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
     }
-    return SUN_OK;
+    return rc;
 }
 
 extern "C" int sun_batch_heatbath(sun_batch* b, const uint32_t* sweeps, void* stream) {
