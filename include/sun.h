@@ -331,7 +331,9 @@ int sun_batch_reunitarize(sun_batch *b, void *stream);
 /* out: double[2*n] plaquette (spatial, temporal) per lattice, blocking */
 int sun_batch_plaquette(sun_batch *b, void *stream, double *out);
 /* n_iter x (HB + n_or OR [+ reunit]); hipGraph-captured when use_graph; per-lattice
- * plaquette of every iteration into obs (host, n_iter*2*n doubles, may be NULL) */
+ * plaquette of every iteration into obs (host, n_iter*2*n doubles, may be NULL).
+ * Chunks of lattices run on the caller's stream and two streams of the batch, which
+ * wait for the caller's stream first and which it waits for at the end. */
 int sun_batch_generate(sun_batch *b, uint32_t first_sweep, int n_iter, int n_or, int reunit_every,
                        int use_graph, void *stream, double *obs);
 
